@@ -1,0 +1,548 @@
+// api.cpp -- host side of the C ABI in include/frhip.h.
+//
+// Mirrors the planning, verification and ownership rules of
+// crates/fritillaria-cuda/src/{backend,bam}.rs and the reader loop of
+// crates/fritillaria-bgzf/src/device_read.rs, over HIP runtime calls and the
+// kernels in inflate.hip / bam.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/frhip.h"
+#include "crc_math.h"
+#include "frhip_internal.h"
+#include "host_internal.h"
+
+using namespace frhip;
+
+// ------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------
+
+namespace frhip {
+
+int set_err(frhip_error *e, int kind, uint64_t offset, const char *fmt, ...)
+{
+    if (e) {
+        std::memset(e, 0, sizeof *e);
+        e->kind = kind;
+        e->offset = offset;
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(e->reason, sizeof e->reason, fmt, ap);
+        va_end(ap);
+    }
+    return kind;
+}
+
+int hip_err(frhip_error *e, hipError_t r, const char *what)
+{
+    return set_err(e, FRHIP_E_DEVICE, 0, "%s: %s", what, hipGetErrorString(r));
+}
+
+const char *inflate_status_message(uint32_t status)
+{
+    // crates/fritillaria-cuda/src/backend.rs:66-79
+    switch (status) {
+    case 1: return "truncated deflate stream";
+    case 2: return "reserved deflate block type";
+    case 3: return "stored block LEN/NLEN mismatch";
+    case 4: return "output overran the size declared by ISIZE";
+    case 5: return "invalid Huffman code";
+    case 6: return "invalid or out-of-range back-reference distance";
+    case 7: return "malformed Huffman table";
+    case 8: return "invalid code-length repeat";
+    case 9: return "code counts exceed the DEFLATE maximum";
+    default: return "unknown device error";
+    }
+}
+
+// ------------------------------------------------------------------------
+// buffers
+// ------------------------------------------------------------------------
+
+frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err)
+{
+    frhip_buf *b = new (std::nothrow) frhip_buf;
+    if (!b) {
+        set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+        return nullptr;
+    }
+    b->len = len;
+    b->ordinal = ctx->ordinal;
+    b->ptr = nullptr;
+    // zero-length allocations are padded to one byte with a logical length of 0
+    // (crates/fritillaria-cuda/src/backend.rs:202-205,608-613)
+    hipError_t r = hipMalloc(&b->ptr, len ? len : 1);
+    if (r != hipSuccess) {
+        delete b;
+        hip_err(err, r, "hipMalloc");
+        return nullptr;
+    }
+    return b;
+}
+
+void buf_free(frhip_buf *b)
+{
+    if (!b) return;
+    if (b->ptr) (void)hipFree(b->ptr);
+    delete b;
+}
+
+// Grow-only device scratch owned by the context.
+void *Scratch::get(uint64_t bytes, frhip_error *err, int *rc)
+{
+    *rc = FRHIP_OK;
+    if (bytes <= cap && ptr) return ptr;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    uint64_t want = std::max<uint64_t>(bytes, 1 << 16);
+    want = want + want / 4;
+    hipError_t r = hipMalloc(&ptr, want);
+    if (r != hipSuccess) {
+        ptr = nullptr;
+        *rc = hip_err(err, r, "hipMalloc(scratch)");
+        return nullptr;
+    }
+    cap = want;
+    return ptr;
+}
+
+void Scratch::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+void *PinnedScratch::get(uint64_t bytes, frhip_error *err, int *rc)
+{
+    *rc = FRHIP_OK;
+    if (bytes <= cap && ptr) return ptr;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    uint64_t want = std::max<uint64_t>(bytes, 1 << 16);
+    want = want + want / 4;
+    hipError_t r = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+    if (r != hipSuccess) {
+        ptr = nullptr;
+        *rc = hip_err(err, r, "hipHostMalloc");
+        return nullptr;
+    }
+    cap = want;
+    return ptr;
+}
+
+void PinnedScratch::release()
+{
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+} // namespace frhip
+
+// ------------------------------------------------------------------------
+// device and context
+// ------------------------------------------------------------------------
+
+extern "C" int frhip_device_available(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 0;
+    return std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+extern "C" int frhip_ctx_create(int ordinal, void *stream, frhip_ctx **out, frhip_error *err)
+{
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return set_err(err, FRHIP_E_UNAVAILABLE, 0, "no HIP device is available");
+    if (ordinal < 0 || ordinal >= n)
+        return set_err(err, FRHIP_E_UNAVAILABLE, 0, "device %d does not exist (%d present)", ordinal, n);
+    hipError_t r = hipSetDevice(ordinal);
+    if (r != hipSuccess) return hip_err(err, r, "hipSetDevice");
+    hipDeviceProp_t p;
+    r = hipGetDeviceProperties(&p, ordinal);
+    if (r != hipSuccess) return hip_err(err, r, "hipGetDeviceProperties");
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
+        return set_err(err, FRHIP_E_UNAVAILABLE, 0, "device %d is %s, this build targets gfx950", ordinal,
+                       p.gcnArchName);
+    frhip_ctx *c = new (std::nothrow) frhip_ctx();
+    if (!c) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    c->ordinal = ordinal;
+    c->n_cu = p.multiProcessorCount;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        r = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (r != hipSuccess) {
+            delete c;
+            return hip_err(err, r, "hipStreamCreate");
+        }
+        c->own_stream = true;
+    }
+    uint32_t x2n[32];
+    fr_x2n_table(x2n);
+    r = hipMalloc(&c->x2n, sizeof x2n);
+    if (r == hipSuccess) r = hipMemcpy(c->x2n, x2n, sizeof x2n, hipMemcpyHostToDevice);
+    if (r != hipSuccess) {
+        frhip_ctx_destroy(c);
+        return hip_err(err, r, "uploading CRC tables");
+    }
+    const char *w = std::getenv("FRHIP_BAM_WINDOW");
+    if (w) c->bam_window = (uint32_t)std::strtoul(w, nullptr, 10);
+    *out = c;
+    return FRHIP_OK;
+}
+
+extern "C" void frhip_ctx_destroy(frhip_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->ordinal);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->in_scratch.release();
+    c->meta_scratch.release();
+    c->status_scratch.release();
+    c->bam_scratch.release();
+    c->pinned.release();
+    if (c->x2n) (void)hipFree(c->x2n);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int frhip_ctx_ordinal(const frhip_ctx *c) { return c->ordinal; }
+extern "C" void *frhip_ctx_stream(const frhip_ctx *c) { return (void *)c->stream; }
+extern "C" int frhip_ctx_set_bam_window(frhip_ctx *c, uint32_t window)
+{
+    c->bam_window = window < 1 ? 1 : window;
+    return FRHIP_OK;
+}
+extern "C" const char *frhip_codec_name(void) { return "hip"; }
+extern "C" const char *frhip_device_codec_name(void) { return "hip-device"; }
+
+// ------------------------------------------------------------------------
+// buffers
+// ------------------------------------------------------------------------
+
+extern "C" uint64_t frhip_buf_len(const frhip_buf *b) { return b ? b->len : 0; }
+extern "C" int frhip_buf_ordinal(const frhip_buf *b) { return b ? b->ordinal : -1; }
+extern "C" void *frhip_buf_device_ptr(const frhip_buf *b) { return b ? b->ptr : nullptr; }
+
+extern "C" int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t dst_len, frhip_error *err)
+{
+    // crates/fritillaria-cuda/src/backend.rs:333-358
+    if (!b) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no buffer");
+    if (dst_len != b->len)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0,
+                       "copy_to_host destination is %llu bytes, the allocation is %llu",
+                       (unsigned long long)dst_len, (unsigned long long)b->len);
+    if (b->len == 0) return FRHIP_OK;
+    (void)hipSetDevice(b->ordinal);
+    hipError_t r = hipMemcpy(dst, b->ptr, b->len, hipMemcpyDeviceToHost);
+    if (r != hipSuccess) return hip_err(err, r, "copy_to_host");
+    return FRHIP_OK;
+}
+
+extern "C" void frhip_buf_free(frhip_buf *b) { buf_free(b); }
+
+// ------------------------------------------------------------------------
+// discovery (host)
+// ------------------------------------------------------------------------
+
+namespace {
+
+uint16_t rd16(const uint8_t *p) { return (uint16_t)(p[0] | (p[1] << 8)); }
+uint32_t rd32(const uint8_t *p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+// crates/fritillaria-bgzf/src/block.rs:99-164. Returns nullptr or the reason.
+const char *parse_bgzf_header(const uint8_t *buf, uint64_t len, uint32_t *block_size, uint32_t *header_size)
+{
+    if (len < 12) return "truncated header";
+    if (buf[0] != 0x1f || buf[1] != 0x8b) return "not a gzip member (bad magic)";
+    if (buf[2] != 8) return "unsupported compression method";
+    if ((buf[3] & 0x04) == 0) return "missing FEXTRA; not BGZF (plain gzip?)";
+    uint32_t xlen = rd16(buf + 10);
+    uint32_t hs = 12 + xlen;
+    if (len < hs) return "truncated extra field";
+    const uint8_t *x = buf + 12;
+    uint32_t xl = xlen;
+    bool found = false;
+    uint32_t bsize = 0;
+    while (xl >= 4) {
+        uint32_t slen = rd16(x + 2);
+        if (4 + slen > xl) break;
+        if (x[0] == 'B' && x[1] == 'C') {
+            if (slen != 2) break;
+            bsize = rd16(x + 4);
+            found = true;
+            break;
+        }
+        x += 4 + slen;
+        xl -= 4 + slen;
+    }
+    if (!found) return "missing BC subfield; not BGZF";
+    uint32_t bs = bsize + 1;
+    if (bs > 65536) return "block size exceeds 64 KiB";
+    if (bs < hs + 8) return "block size smaller than its own framing";
+    *block_size = bs;
+    *header_size = hs;
+    return nullptr;
+}
+
+} // namespace
+
+namespace frhip {
+
+// discover.rs:51-113
+int discover(const uint8_t *buf, uint64_t len, uint64_t base, std::vector<frhip_block_span> &spans,
+             uint64_t *position, frhip_error *err)
+{
+    uint64_t pos = 0;
+    while (pos < len) {
+        const uint8_t *rest = buf + pos;
+        uint64_t rl = len - pos;
+        if (rl < 12) break;
+        uint64_t hl = 12 + (uint64_t)rd16(rest + 10);
+        if (hl > rl) break;
+        uint32_t bs = 0, hs = 0;
+        const char *why = parse_bgzf_header(rest, rl, &bs, &hs);
+        if (why) {
+            *position = pos;
+            return set_err(err, FRHIP_E_INVALID_BLOCK, base + pos, "%s", why);
+        }
+        if (bs > rl) break;
+        uint32_t crc = rd32(rest + bs - 8);
+        uint32_t isz = rd32(rest + bs - 4);
+        if (isz > 65536) {
+            *position = pos;
+            return set_err(err, FRHIP_E_INVALID_BLOCK, base + pos, "ISIZE exceeds 64 KiB");
+        }
+        spans.push_back(frhip_block_span{base + pos, pos + hs, bs - hs - 8ull, isz, crc});
+        pos += bs;
+    }
+    *position = pos;
+    return FRHIP_OK;
+}
+
+} // namespace frhip
+
+extern "C" int frhip_discover(const uint8_t *buf, uint64_t len, uint64_t base, frhip_block_span *spans,
+                              uint64_t cap, uint64_t *n, uint64_t *position, frhip_error *err)
+{
+    std::vector<frhip_block_span> v;
+    int rc = discover(buf, len, base, v, position, err);
+    uint64_t k = std::min<uint64_t>(cap, v.size());
+    if (spans && k) std::memcpy(spans, v.data(), k * sizeof(frhip_block_span));
+    *n = k;
+    return rc;
+}
+
+// ------------------------------------------------------------------------
+// inflate
+// ------------------------------------------------------------------------
+
+namespace frhip {
+
+// Plan + upload + launch + verify. On success *out owns the output.
+int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t)
+{
+    *out = nullptr;
+    (void)hipSetDevice(ctx->ordinal);
+    // BatchPlan::new (backend.rs:95-133)
+    std::vector<uint64_t> meta(3 * n + 1);
+    uint64_t *in_off = meta.data();
+    uint64_t *out_off = meta.data() + n;
+    uint32_t *lens = reinterpret_cast<uint32_t *>(meta.data() + 2 * n); // in_len[n], out_cap[n]
+    std::vector<uint32_t> caps(n), inl(n);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const frhip_block_span &s = spans[i];
+        if (s.isize > 65536)
+            return set_err(err, FRHIP_E_INVALID_BLOCK, s.compressed_offset, "ISIZE exceeds 64 KiB");
+        uint64_t end = s.payload_start + s.payload_len;
+        if (end < s.payload_start || end > batch_len)
+            return set_err(err, FRHIP_E_INVALID_BLOCK, s.compressed_offset,
+                           "payload extends past the end of the batch");
+        in_off[i] = s.payload_start;
+        out_off[i] = total;
+        inl[i] = (uint32_t)s.payload_len;
+        caps[i] = s.isize;
+        total += s.isize;
+    }
+    (void)lens;
+    frhip_buf *o = buf_alloc(ctx, total, err);
+    if (!o) return err ? err->kind : FRHIP_E_DEVICE;
+    if (n == 0) {
+        offsets[0] = 0;
+        *out = o;
+        return FRHIP_OK;
+    }
+    int rc;
+    // compressed bytes, padded: the kernel stages 1 KiB-aligned chunks past the end
+    const uint64_t pad = 4096;
+    uint8_t *d_in = (uint8_t *)ctx->in_scratch.get(batch_len + pad, err, &rc);
+    if (!d_in) { buf_free(o); return rc; }
+    // meta: in_off[n] u64, out_off[n] u64, in_len[n] u32, out_cap[n] u32
+    const uint64_t meta_bytes = n * 8 * 2 + n * 4 * 2;
+    uint8_t *h_meta = (uint8_t *)ctx->pinned.get(meta_bytes, err, &rc);
+    if (!h_meta) { buf_free(o); return rc; }
+    std::memcpy(h_meta, in_off, n * 8);
+    std::memcpy(h_meta + n * 8, out_off, n * 8);
+    std::memcpy(h_meta + n * 16, inl.data(), n * 4);
+    std::memcpy(h_meta + n * 20, caps.data(), n * 4);
+    uint8_t *d_meta = (uint8_t *)ctx->meta_scratch.get(meta_bytes, err, &rc);
+    if (!d_meta) { buf_free(o); return rc; }
+    uint32_t *d_st = (uint32_t *)ctx->status_scratch.get(n * 12, err, &rc);
+    if (!d_st) { buf_free(o); return rc; }
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (t) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventCreate(&e2);
+        (void)hipEventRecord(e0, ctx->stream);
+    }
+    hipError_t r = hipMemcpyAsync(d_in, batch, batch_len, hipMemcpyHostToDevice, ctx->stream);
+    if (r == hipSuccess) r = hipMemsetAsync(d_in + batch_len, 0, pad, ctx->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(d_meta, h_meta, meta_bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "uploading the batch"); }
+    if (t) (void)hipEventRecord(e1, ctx->stream);
+    InflateArgs a{};
+    a.in = d_in;
+    a.in_off = (const uint64_t *)d_meta;
+    a.out_off = (const uint64_t *)(d_meta + n * 8);
+    a.in_len = (const uint32_t *)(d_meta + n * 16);
+    a.out_cap = (const uint32_t *)(d_meta + n * 20);
+    a.out = (uint8_t *)o->ptr;
+    a.status = d_st;
+    a.produced = d_st + n;
+    a.crc = d_st + 2 * n;
+    a.n = (uint32_t)n;
+    a.queue = nullptr;
+    a.order = nullptr;
+    a.x2n = ctx->x2n;
+    r = launch_inflate(a, (uint32_t)n, ctx->stream);
+    if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
+    if (t) (void)hipEventRecord(e2, ctx->stream);
+    uint32_t *h_st = (uint32_t *)ctx->pinned.get(n * 12 > meta_bytes ? n * 12 : meta_bytes, err, &rc);
+    if (!h_st) { buf_free(o); return rc; }
+    r = hipMemcpyAsync(h_st, d_st, n * 12, hipMemcpyDeviceToHost, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "inflate"); }
+    if (t) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t->upload_ms += ms;
+        (void)hipEventElapsedTime(&ms, e1, e2);
+        t->kernel_ms += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+    }
+    // verify_and_offsets (backend.rs:221-256): status, then size, then CRC
+    const uint32_t *st = h_st, *produced = h_st + n, *crc = h_st + 2 * n;
+    offsets[0] = 0;
+    uint64_t cursor = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (st[i] != 0) {
+            buf_free(o);
+            int k = set_err(err, FRHIP_E_INFLATE, spans[i].compressed_offset, "%s", inflate_status_message(st[i]));
+            if (err) err->code = st[i];
+            return k;
+        }
+        if (produced[i] != spans[i].isize) {
+            buf_free(o);
+            int k = set_err(err, FRHIP_E_SIZE_MISMATCH, spans[i].compressed_offset,
+                            "ISIZE says %u bytes, inflated %u", spans[i].isize, produced[i]);
+            if (err) { err->expected = spans[i].isize; err->actual = produced[i]; }
+            return k;
+        }
+        if (crc[i] != spans[i].crc32) {
+            buf_free(o);
+            int k = set_err(err, FRHIP_E_CHECKSUM_MISMATCH, spans[i].compressed_offset,
+                            "expected crc32 %#010x, got %#010x", spans[i].crc32, crc[i]);
+            if (err) { err->expected = spans[i].crc32; err->actual = crc[i]; }
+            return k;
+        }
+        cursor += produced[i];
+        offsets[i + 1] = cursor;
+    }
+    *out = o;
+    return FRHIP_OK;
+}
+
+} // namespace frhip
+
+extern "C" int frhip_inflate_batch_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                          const frhip_block_span *spans, uint64_t n, frhip_buf **out_data,
+                                          uint64_t *out_offsets, frhip_error *err)
+{
+    return inflate_device(ctx, batch, batch_len, spans, n, out_data, out_offsets, err, nullptr);
+}
+
+extern "C" int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                   const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
+                                   uint64_t *out_offsets, frhip_error *err)
+{
+    frhip_buf *b = nullptr;
+    int rc = inflate_device(ctx, batch, batch_len, spans, n, &b, out_offsets, err, nullptr);
+    if (rc) return rc;
+    if (b->len > out_cap) {
+        buf_free(b);
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "output buffer holds %llu bytes, batch needs %llu",
+                       (unsigned long long)out_cap, (unsigned long long)b->len);
+    }
+    rc = frhip_buf_copy_to_host(b, out, b->len, err);
+    buf_free(b);
+    return rc;
+}
+
+extern "C" uint32_t frhip_crc32_combine(uint32_t crc1, uint32_t crc2, uint64_t len2)
+{
+    uint32_t x2n[32];
+    fr_x2n_table(x2n);
+    return fr_mulmod(crc1, fr_xpow8(x2n, len2)) ^ crc2;
+}
+
+extern "C" int frhip_crc32_blocks(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *offsets,
+                                  const uint32_t *lengths, uint64_t n, uint32_t *crc_out, frhip_error *err)
+{
+    if (n == 0) return FRHIP_OK;
+    (void)hipSetDevice(ctx->ordinal);
+    for (uint64_t i = 0; i < n; ++i)
+        if (offsets[i] + lengths[i] > data->len)
+            return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "range %llu lies outside the buffer",
+                           (unsigned long long)i);
+    int rc;
+    uint8_t *d = (uint8_t *)ctx->meta_scratch.get(n * 16, err, &rc);
+    if (!d) return rc;
+    std::vector<uint8_t> h(n * 12);
+    std::memcpy(h.data(), offsets, n * 8);
+    std::memcpy(h.data() + n * 8, lengths, n * 4);
+    hipError_t r = hipMemcpyAsync(d, h.data(), n * 12, hipMemcpyHostToDevice, ctx->stream);
+    uint32_t *d_crc = (uint32_t *)ctx->status_scratch.get(n * 4, err, &rc);
+    if (!d_crc) return rc;
+    if (r == hipSuccess)
+        r = launch_crc32_blocks((const uint8_t *)data->ptr, (const uint64_t *)d, (const uint32_t *)(d + n * 8),
+                                (uint32_t)n, d_crc, ctx->x2n, ctx->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(crc_out, d_crc, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "crc32_blocks");
+    return FRHIP_OK;
+}

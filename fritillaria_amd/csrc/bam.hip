@@ -1,0 +1,376 @@
+// bam.hip -- BAM record boundaries and SoA column decode over an inflated batch.
+//
+// Result semantics follow the reference's device scan
+// (crates/fritillaria-bam/src/columnar/blocked.rs:106-282 and
+//  crates/fritillaria-cuda/kernels/bam_decode.cu:63-291): the true record chain
+// from `start`, `block_size >= 32` and `fields_fit` on every record reached from
+// a proven boundary, a tail at the first record the buffer cuts short, and a
+// malformed record reported at the chain's entry into the member holding it.
+//
+// The speculation is ours. The reference guesses that each member starts on a
+// record, which never holds for noodles-written files (it cuts members every
+// 65,495 bytes whatever the records). Here every byte offset in the first
+// `window` bytes of a member is a candidate: a wave tests 64 at a time with a
+// short walk to the window's end, the survivors must agree on where they land,
+// and one exact walk from that landing to the member end records the offsets.
+// A later reconcile adopts a member's walk only when the true chain enters it at
+// one of its survivors, so a wrong guess costs a fallback walk, never a wrong
+// answer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frhip_internal.h"
+
+namespace frhip {
+namespace {
+
+enum : uint64_t { BAM_OK = 0, BAM_MALFORMED = 1 };
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__device__ __forceinline__ uint32_t ld32(const uint8_t *p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+__device__ __forceinline__ uint32_t ld16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
+
+// One step of the chain from `cur` (a record start candidate); the walk stops
+// at `until`. Returns 0 advanced, 1 reached until, 2 ran out of buffer
+// (partial record: the tail), 3 malformed.
+struct Step {
+    uint64_t next;
+    int kind;
+};
+
+__device__ __forceinline__ Step step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until)
+{
+    if (cur >= until) return {cur, 1};
+    if (cur + 4 > len) return {cur, 2};
+    uint32_t bs = ld32(buf + cur);
+    if (bs < 32) return {cur, 3};
+    uint64_t end = cur + 4 + (uint64_t)bs;
+    if (end > len) return {cur, 2};
+    uint32_t name_len = buf[cur + 12];
+    uint64_t n_cigar = ld16(buf + cur + 16);
+    uint64_t l_seq = ld32(buf + cur + 20);
+    uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
+    if (name_len == 0 || cur + need > end) return {cur, 3};
+    return {end, 0};
+}
+
+struct Walk {
+    uint64_t land;
+    uint32_t count;
+    int complete;
+    int valid;
+};
+
+// walk_segment (blocked.rs:141-187), uniform.
+__device__ Walk walk_uniform(const uint8_t *buf, uint64_t len, uint64_t from, uint64_t until)
+{
+    Walk w{from, 0, 1, 1};
+    uint64_t cur = from;
+    for (;;) {
+        Step s = step(buf, len, cur, until);
+        if (s.kind == 1) { w.land = cur; return w; }
+        if (s.kind == 2) { w.land = cur; w.complete = 0; return w; }
+        if (s.kind == 3) { w.land = cur; w.valid = 0; return w; }
+        w.count++;
+        cur = s.next;
+    }
+}
+
+__global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint64_t sb = a.block_starts[b], sb1 = a.block_starts[b + 1];
+    if (sb >= sb1) {
+        if (lane == 0) {
+            a.spec_ok[b] = 0;
+            a.n_surv[b] = 0;
+        }
+        return;
+    }
+    const uint64_t T = (sb1 - sb) > a.window ? sb + a.window : sb1;
+    // candidate walks to the window end
+    uint32_t n_surv = 0;
+    int agree = 1;
+    uint64_t L = 0;
+    int Lcomplete = 1;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (uint64_t base = sb; base < T; base += 64) {
+        uint64_t p = base + lane;
+        int alive = 0;
+        uint64_t cur = p;
+        uint32_t cnt = 0;
+        int complete = 1;
+        if (p < T) {
+            alive = 1;
+            for (;;) {
+                Step s = step(a.buf, a.len, cur, T);
+                if (s.kind == 1) break;
+                if (s.kind == 2) { complete = 0; break; }
+                if (s.kind == 3) { alive = 0; break; }
+                ++cnt;
+                cur = s.next;
+            }
+        }
+        uint64_t m = __ballot(alive);
+        if (m) {
+            uint32_t first = __ffsll((unsigned long long)m) - 1;
+            if (n_surv == 0) {
+                L = uni64(__shfl(cur, first, 64));
+                Lcomplete = (int)uni((uint32_t)__shfl(complete, first, 64));
+            }
+            if (alive && (cur != L || complete != Lcomplete)) agree = 0;
+            uint32_t rank = n_surv + (uint32_t)__popcll(m & lt);
+            if (alive && rank < BAM_K) {
+                a.surv_pos[(uint64_t)b * BAM_K + rank] = p;
+                a.surv_cnt[(uint64_t)b * BAM_K + rank] = cnt;
+            }
+            n_surv += (uint32_t)__popcll(m);
+        }
+    }
+    agree = (int)uni((uint32_t)(__ballot(!agree) == 0));
+    if (n_surv == 0 || !agree) {
+        if (lane == 0) {
+            a.spec_ok[b] = 0;
+            a.n_surv[b] = (uint8_t)(n_surv < BAM_K ? n_surv : BAM_K);
+        }
+        return;
+    }
+    // exact walk from the agreed landing to the member end
+    uint64_t cur = L;
+    uint32_t cnt = 0;
+    int complete = Lcomplete, valid = 1;
+    uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
+    if (complete) {
+        for (;;) {
+            Step s = step(a.buf, a.len, cur, sb1);
+            if (s.kind == 1) break;
+            if (s.kind == 2) { complete = 0; break; }
+            if (s.kind == 3) { valid = 0; break; }
+            if (lane == 0) rel[cnt] = (uint16_t)(cur - sb);
+            ++cnt;
+            cur = s.next;
+        }
+    }
+    if (lane == 0) {
+        a.spec_ok[b] = (uint8_t)valid;
+        a.n_surv[b] = (uint8_t)(n_surv < BAM_K ? n_surv : BAM_K);
+        a.land[b] = L;
+        a.exitp[b] = cur;
+        a.cnt_tail[b] = cnt;
+        a.complete[b] = (uint8_t)complete;
+    }
+}
+
+// Lane j of the result takes v, the other lanes keep old (v is wave-uniform).
+__device__ __forceinline__ uint32_t wl32(uint32_t v, uint32_t j, uint32_t old)
+{
+    return threadIdx.x == j ? v : old;
+}
+__device__ __forceinline__ uint64_t wl64(uint64_t v, uint32_t j, uint64_t old)
+{
+    return threadIdx.x == j ? v : old;
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, uint32_t j)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)j);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t j)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)j);
+}
+
+// The true chain over members, 64 members per step (one wave). Each lane first
+// checks, in parallel, whether its member can be adopted assuming the previous
+// member was; the uniform pass then follows the chain with register reads and
+// walks only the members whose speculation does not apply.
+__global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
+{
+    const uint32_t lane = threadIdx.x;
+    uint64_t cur = a.start, emitted = 0, tail = a.len, fallbacks = 0;
+    int stopped = 0;
+    uint64_t status = BAM_OK;
+    for (uint32_t base = 0; base < a.nb; base += 64) {
+        const uint32_t b = base + lane;
+        const bool in = b < a.nb;
+        uint64_t s1 = in ? a.block_starts[b + 1] : 0;
+        uint32_t ok = in ? a.spec_ok[b] : 0;
+        uint64_t X = in && ok ? a.exitp[b] : 0;
+        uint32_t comp = in && ok ? a.complete[b] : 0;
+        uint32_t ctail = in && ok ? a.cnt_tail[b] : 0;
+        // expected entry = previous lane's speculative exit (lane 0: the chain)
+        uint64_t prevX = __shfl_up(X, 1, 64);
+        uint64_t expect = lane == 0 ? cur : prevX;
+        uint32_t match = 0, mcnt = 0;
+        if (ok) {
+            uint32_t ns = a.n_surv[b];
+            for (uint32_t k = 0; k < ns; ++k) {
+                if (a.surv_pos[(uint64_t)b * BAM_K + k] == expect) {
+                    match = k + 1;
+                    mcnt = a.surv_cnt[(uint64_t)b * BAM_K + k] + ctail;
+                    break;
+                }
+            }
+        }
+        uint64_t o_entry = BAM_NONE, o_first = 0;
+        uint32_t o_cnt = 0, o_mode = 0;
+        uint32_t lim = a.nb - base < 64 ? a.nb - base : 64;
+        for (uint32_t j = 0; j < lim; ++j) {
+            uint64_t js1 = rl64(s1, j);
+            if (stopped || cur >= js1) {
+                o_first = wl64(emitted, j, o_first);
+                continue;
+            }
+            uint32_t cnt, comp_j, valid = 1, mode;
+            uint64_t land;
+            if (rl32(match, j) && rl64(expect, j) == cur) {
+                cnt = rl32(mcnt, j);
+                land = rl64(X, j);
+                comp_j = rl32(comp, j);
+                mode = rl32(match, j);
+            } else {
+                Walk w = walk_uniform(a.buf, a.len, cur, js1);
+                cnt = w.count;
+                land = w.land;
+                comp_j = (uint32_t)w.complete;
+                valid = (uint32_t)w.valid;
+                mode = BAM_MODE_FALLBACK;
+                ++fallbacks;
+            }
+            o_first = wl64(emitted, j, o_first);
+            if (!valid) {
+                status = BAM_MALFORMED;
+                tail = cur;
+                stopped = 1;
+                continue;
+            }
+            o_entry = wl64(cur, j, o_entry);
+            o_cnt = wl32(cnt, j, o_cnt);
+            o_mode = wl32(mode, j, o_mode);
+            emitted += cnt;
+            cur = land;
+            if (!comp_j) {
+                tail = land;
+                stopped = 1;
+            }
+        }
+        if (in) {
+            a.entry[b] = o_entry;
+            a.count[b] = o_cnt;
+            a.first_index[b] = o_first;
+            a.mode[b] = (uint8_t)o_mode;
+        }
+    }
+    if (!stopped) tail = cur;
+    if (lane == 0) {
+        a.totals[0] = emitted;
+        a.totals[1] = tail;
+        a.totals[2] = status;
+        a.totals[3] = fallbacks;
+    }
+}
+
+__global__ __launch_bounds__(64) void bam_emit_kernel(BamEmitArgs a)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint64_t e = a.entry[b];
+    if (e == BAM_NONE) return;
+    const uint32_t cnt = a.count[b];
+    const uint64_t first = a.first_index[b];
+    const uint32_t mode = a.mode[b];
+    const uint64_t sb = a.block_starts[b];
+    const uint64_t vbase = a.coffsets ? (a.coffsets[b] << 16) : 0;
+    uint32_t nwalk = cnt;
+    if (mode != BAM_MODE_FALLBACK) nwalk = a.surv_cnt[(uint64_t)b * BAM_K + mode - 1];
+    uint64_t cur = e;
+    for (uint32_t i = 0; i < nwalk; ++i) {
+        if (lane == 0) {
+            a.record_offsets[first + i] = cur;
+            if (a.virtual_offsets) a.virtual_offsets[first + i] = vbase | (cur - sb);
+        }
+        cur += 4 + (uint64_t)ld32(a.buf + cur);
+    }
+    if (mode != BAM_MODE_FALLBACK) {
+        const uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
+        for (uint32_t i = lane; i < cnt - nwalk; i += 64) {
+            uint64_t r = rel[i];
+            a.record_offsets[first + nwalk + i] = sb + r;
+            if (a.virtual_offsets) a.virtual_offsets[first + nwalk + i] = vbase | r;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void bam_decode_kernel(BamDecodeArgs a)
+{
+    const uint64_t n = a.n_dev ? *a.n_dev : a.n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = a.c.record_offsets[i];
+        const uint8_t *p = a.buf + r;
+        uint32_t bs = ld32(p);
+        uint32_t name_len = p[12];
+        uint32_t nc = ld16(p + 16);
+        uint32_t ls = ld32(p + 20);
+        a.c.ref_id[i] = (int32_t)ld32(p + 4);
+        a.c.pos[i] = (int32_t)ld32(p + 8);
+        a.c.mapq[i] = p[13];
+        a.c.flags[i] = (uint16_t)ld16(p + 18);
+        a.c.l_seq[i] = ls;
+        a.c.next_ref_id[i] = (int32_t)ld32(p + 24);
+        a.c.next_pos[i] = (int32_t)ld32(p + 28);
+        a.c.tlen[i] = (int32_t)ld32(p + 32);
+        uint32_t c = 36u + name_len;
+        uint32_t s = c + 4u * nc;
+        uint32_t q = s + (ls + 1u) / 2u;
+        a.c.cigar_start[i] = c;
+        a.c.seq_start[i] = s;
+        a.c.qual_start[i] = q;
+        a.c.aux_start[i] = q + ls;
+        a.c.record_end[i] = 4u + bs;
+    }
+}
+
+} // namespace
+
+hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
+{
+    if (a.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(bam_spec_kernel, dim3(a.nb), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(bam_reconcile_kernel, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_emit(const BamEmitArgs &a, hipStream_t s)
+{
+    if (a.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(bam_emit_kernel, dim3(a.nb), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_decode(const BamDecodeArgs &a, uint64_t max_records, hipStream_t s)
+{
+    if (max_records == 0) return hipSuccess;
+    uint64_t blocks = (max_records + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(bam_decode_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+} // namespace frhip

@@ -1,0 +1,388 @@
+// bam_host.cpp -- BamDecoder::decode and DeviceBgzfReader behind the C ABI.
+//
+// decode follows crates/fritillaria-cuda/src/bam.rs:324-574: an empty batch
+// gives no records and tail = start; otherwise boundaries on the device, one
+// host read of {records, tail, status} to size the columns (bam.rs:447-468),
+// then offsets and columns on the device. The reader follows
+// crates/fritillaria-bgzf/src/device_read.rs:155-280 line for line in
+// behaviour: compressed-member carry, window growth, truncation and
+// "stream ends inside a record" errors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/frhip.h"
+#include "frhip_internal.h"
+#include "host_internal.h"
+
+using namespace frhip;
+
+static const int BAM_WIDTHS[FRHIP_BAM_N_COLUMNS] = {4, 4, 2, 1, 4, 4, 4, 4, 8, 4, 4, 4, 4, 4, 8};
+
+struct frhip_bam_batch {
+    uint64_t n = 0;
+    uint64_t tail = 0;
+    uint64_t fallbacks = 0;
+    frhip_buf *cols[FRHIP_BAM_N_COLUMNS] = {};
+};
+
+namespace {
+
+struct BamScratchLayout {
+    uint64_t bytes;
+    uint64_t o_bs, o_coff, o_ok, o_ns, o_spos, o_scnt, o_land, o_exit, o_ctail, o_comp, o_rel, o_entry, o_count,
+        o_first, o_mode, o_totals;
+};
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
+BamScratchLayout layout(uint64_t nb)
+{
+    BamScratchLayout L{};
+    uint64_t p = 0;
+    auto take = [&](uint64_t bytes) { uint64_t o = p; p = align256(p + bytes); return o; };
+    L.o_bs = take((nb + 1) * 8);
+    L.o_coff = take(nb * 8);
+    L.o_ok = take(nb);
+    L.o_ns = take(nb);
+    L.o_spos = take(nb * BAM_K * 8);
+    L.o_scnt = take(nb * BAM_K * 4);
+    L.o_land = take(nb * 8);
+    L.o_exit = take(nb * 8);
+    L.o_ctail = take(nb * 4);
+    L.o_comp = take(nb);
+    L.o_rel = take(nb * (uint64_t)BAM_CAPB * 2);
+    L.o_entry = take(nb * 8);
+    L.o_count = take(nb * 4);
+    L.o_first = take(nb * 8);
+    L.o_mode = take(nb);
+    L.o_totals = take(4 * 8);
+    L.bytes = p;
+    return L;
+}
+
+int empty_batch(frhip_ctx *ctx, uint64_t start, bool with_voff, frhip_bam_batch **out, frhip_error *err)
+{
+    frhip_bam_batch *b = new (std::nothrow) frhip_bam_batch;
+    if (!b) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    b->tail = start;
+    for (int i = 0; i < FRHIP_BAM_N_COLUMNS; ++i) {
+        if (i == FRHIP_BAM_VIRTUAL_OFFSETS && !with_voff) continue;
+        b->cols[i] = buf_alloc(ctx, 0, err);
+        if (!b->cols[i]) {
+            frhip_bam_batch_free(b);
+            return err ? err->kind : FRHIP_E_DEVICE;
+        }
+    }
+    *out = b;
+    return FRHIP_OK;
+}
+
+} // namespace
+
+extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
+                                uint64_t nb, const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out,
+                                frhip_error *err)
+{
+    *out = nullptr;
+    if (!data || data->len == 0) return empty_batch(ctx, start, coffsets != nullptr, out, err);
+    if (data->ordinal != ctx->ordinal)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "batch is on device %d but this decoder is on %d",
+                       data->ordinal, ctx->ordinal);
+    const uint64_t len = data->len;
+    if (nb == 0 || block_offsets[0] != 0 || block_offsets[nb] != len)
+        return set_err(err, FRHIP_E_MALFORMED, 0, "block starts must span the buffer from 0 to its length");
+    for (uint64_t i = 0; i < nb; ++i)
+        if (block_offsets[i + 1] < block_offsets[i] || block_offsets[i + 1] - block_offsets[i] > 65536)
+            return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "member %llu has an invalid extent",
+                           (unsigned long long)i);
+    (void)hipSetDevice(ctx->ordinal);
+    BamScratchLayout L = layout(nb);
+    int rc;
+    uint8_t *s = (uint8_t *)ctx->bam_scratch.get(L.bytes, err, &rc);
+    if (!s) return rc;
+    hipError_t r = hipMemcpyAsync(s + L.o_bs, block_offsets, (nb + 1) * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (r == hipSuccess && coffsets)
+        r = hipMemcpyAsync(s + L.o_coff, coffsets, nb * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "uploading block starts");
+
+    BamSpecArgs sa{};
+    sa.buf = (const uint8_t *)data->ptr;
+    sa.len = len;
+    sa.block_starts = (const uint64_t *)(s + L.o_bs);
+    sa.nb = (uint32_t)nb;
+    sa.window = ctx->bam_window;
+    sa.spec_ok = s + L.o_ok;
+    sa.n_surv = s + L.o_ns;
+    sa.surv_pos = (uint64_t *)(s + L.o_spos);
+    sa.surv_cnt = (uint32_t *)(s + L.o_scnt);
+    sa.land = (uint64_t *)(s + L.o_land);
+    sa.exitp = (uint64_t *)(s + L.o_exit);
+    sa.cnt_tail = (uint32_t *)(s + L.o_ctail);
+    sa.complete = s + L.o_comp;
+    sa.rel_offs = (uint16_t *)(s + L.o_rel);
+    r = launch_bam_spec(sa, ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "launching bam_spec");
+
+    BamReconcileArgs ra{};
+    ra.buf = sa.buf;
+    ra.len = len;
+    ra.block_starts = sa.block_starts;
+    ra.nb = (uint32_t)nb;
+    ra.start = start;
+    ra.spec_ok = sa.spec_ok;
+    ra.n_surv = sa.n_surv;
+    ra.surv_pos = sa.surv_pos;
+    ra.surv_cnt = sa.surv_cnt;
+    ra.exitp = sa.exitp;
+    ra.cnt_tail = sa.cnt_tail;
+    ra.complete = sa.complete;
+    ra.entry = (uint64_t *)(s + L.o_entry);
+    ra.count = (uint32_t *)(s + L.o_count);
+    ra.first_index = (uint64_t *)(s + L.o_first);
+    ra.mode = s + L.o_mode;
+    ra.totals = (uint64_t *)(s + L.o_totals);
+    r = launch_bam_reconcile(ra, ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "launching bam_reconcile");
+    uint64_t totals[4];
+    r = hipMemcpyAsync(totals, ra.totals, sizeof totals, hipMemcpyDeviceToHost, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "bam boundaries");
+    if (totals[2] != 0) {
+        int k = set_err(err, FRHIP_E_MALFORMED, totals[1], "malformed record reached from a confirmed boundary");
+        if (err) {
+            err->code = (uint32_t)totals[2];
+            std::strcpy(err->format, "bam");
+        }
+        return k;
+    }
+    const uint64_t n = totals[0];
+    frhip_bam_batch *b = new (std::nothrow) frhip_bam_batch;
+    if (!b) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    b->n = n;
+    b->tail = totals[1];
+    b->fallbacks = totals[3];
+    for (int i = 0; i < FRHIP_BAM_N_COLUMNS; ++i) {
+        if (i == FRHIP_BAM_VIRTUAL_OFFSETS && !coffsets) continue;
+        b->cols[i] = buf_alloc(ctx, n * BAM_WIDTHS[i], err);
+        if (!b->cols[i]) {
+            frhip_bam_batch_free(b);
+            return err ? err->kind : FRHIP_E_DEVICE;
+        }
+    }
+    BamEmitArgs ea{};
+    ea.buf = sa.buf;
+    ea.block_starts = sa.block_starts;
+    ea.coffsets = coffsets ? (const uint64_t *)(s + L.o_coff) : nullptr;
+    ea.nb = (uint32_t)nb;
+    ea.entry = ra.entry;
+    ea.count = ra.count;
+    ea.first_index = ra.first_index;
+    ea.mode = ra.mode;
+    ea.surv_cnt = sa.surv_cnt;
+    ea.rel_offs = sa.rel_offs;
+    ea.record_offsets = (uint64_t *)b->cols[FRHIP_BAM_RECORD_OFFSETS]->ptr;
+    ea.virtual_offsets = coffsets ? (uint64_t *)b->cols[FRHIP_BAM_VIRTUAL_OFFSETS]->ptr : nullptr;
+    r = n ? launch_bam_emit(ea, ctx->stream) : hipSuccess;
+    if (r != hipSuccess) {
+        frhip_bam_batch_free(b);
+        return hip_err(err, r, "launching bam_emit");
+    }
+    BamDecodeArgs da{};
+    da.buf = sa.buf;
+    da.n = n;
+    da.n_dev = nullptr;
+    da.c.ref_id = (int32_t *)b->cols[0]->ptr;
+    da.c.pos = (int32_t *)b->cols[1]->ptr;
+    da.c.flags = (uint16_t *)b->cols[2]->ptr;
+    da.c.mapq = (uint8_t *)b->cols[3]->ptr;
+    da.c.l_seq = (uint32_t *)b->cols[4]->ptr;
+    da.c.next_ref_id = (int32_t *)b->cols[5]->ptr;
+    da.c.next_pos = (int32_t *)b->cols[6]->ptr;
+    da.c.tlen = (int32_t *)b->cols[7]->ptr;
+    da.c.record_offsets = (uint64_t *)b->cols[8]->ptr;
+    da.c.cigar_start = (uint32_t *)b->cols[9]->ptr;
+    da.c.seq_start = (uint32_t *)b->cols[10]->ptr;
+    da.c.qual_start = (uint32_t *)b->cols[11]->ptr;
+    da.c.aux_start = (uint32_t *)b->cols[12]->ptr;
+    da.c.record_end = (uint32_t *)b->cols[13]->ptr;
+    r = launch_bam_decode(da, n, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r != hipSuccess) {
+        frhip_bam_batch_free(b);
+        return hip_err(err, r, "bam decode");
+    }
+    *out = b;
+    return FRHIP_OK;
+}
+
+extern "C" uint64_t frhip_bam_batch_len(const frhip_bam_batch *b) { return b->n; }
+extern "C" uint64_t frhip_bam_batch_tail(const frhip_bam_batch *b) { return b->tail; }
+extern "C" uint64_t frhip_bam_batch_fallbacks(const frhip_bam_batch *b) { return b->fallbacks; }
+extern "C" const frhip_buf *frhip_bam_batch_column(const frhip_bam_batch *b, int column)
+{
+    if (!b || column < 0 || column >= FRHIP_BAM_N_COLUMNS) return nullptr;
+    return b->cols[column];
+}
+extern "C" void frhip_bam_batch_free(frhip_bam_batch *b)
+{
+    if (!b) return;
+    for (auto *c : b->cols) buf_free(c);
+    delete b;
+}
+
+// crates/fritillaria-bam/src/columnar/header.rs:43-103
+extern "C" int frhip_bam_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start, uint32_t *n_ref,
+                                      frhip_error *err)
+{
+    auto mal = [&](uint64_t pos, const char *why) {
+        int k = set_err(err, FRHIP_E_MALFORMED, pos, "%s", why);
+        if (err) std::strcpy(err->format, "bam header");
+        return k;
+    };
+    auto rd32 = [&](uint64_t p) -> int32_t {
+        return (int32_t)((uint32_t)buf[p] | ((uint32_t)buf[p + 1] << 8) | ((uint32_t)buf[p + 2] << 16) |
+                         ((uint32_t)buf[p + 3] << 24));
+    };
+    if (len < 12) return mal(0, "truncated header");
+    if (std::memcmp(buf, "BAM\1", 4) != 0) return mal(0, "bad magic; expected BAM\\1");
+    int32_t l_text = rd32(4);
+    if (l_text < 0) return mal(4, "negative l_text");
+    uint64_t text_end = 8 + (uint64_t)l_text;
+    if (text_end > len) return mal(4, "l_text runs past the end of the buffer");
+    if (text_end + 4 > len) return mal(text_end, "truncated n_ref");
+    int32_t nref = rd32(text_end);
+    if (nref < 0) return mal(text_end, "negative n_ref");
+    uint64_t pos = text_end + 4;
+    for (int32_t i = 0; i < nref; ++i) {
+        if (pos + 4 > len) return mal(pos, "truncated l_name");
+        int32_t l_name = rd32(pos);
+        if (l_name < 0) return mal(pos, "negative l_name");
+        pos += 4;
+        if (pos + (uint64_t)l_name > len) return mal(pos, "truncated name");
+        pos += (uint64_t)l_name;
+        if (pos + 4 > len) return mal(pos, "truncated l_ref");
+        pos += 4;
+    }
+    *records_start = pos;
+    *n_ref = (uint32_t)nref;
+    return FRHIP_OK;
+}
+
+// ------------------------------------------------------------------------
+// DeviceBgzfReader
+// ------------------------------------------------------------------------
+
+struct frhip_reader {
+    frhip_ctx *ctx = nullptr;
+    const uint8_t *src = nullptr;
+    uint64_t src_len = 0, src_pos = 0;
+    uint64_t blocks_per_batch = 256;
+    std::vector<uint8_t> compressed;
+    uint64_t batch_offset = 0;
+    std::vector<uint8_t> carry;
+    uint64_t carry_start = 0;
+    std::vector<frhip_block_span> spans;
+    std::vector<uint64_t> offsets, coffsets;
+    bool inner_exhausted = false;
+};
+
+extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t len, uint64_t blocks_per_batch,
+                                 frhip_reader **out, frhip_error *err)
+{
+    frhip_reader *r = new (std::nothrow) frhip_reader;
+    if (!r) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    r->ctx = ctx;
+    r->src = data;
+    r->src_len = len;
+    r->blocks_per_batch = blocks_per_batch ? blocks_per_batch : 256;
+    *out = r;
+    return FRHIP_OK;
+}
+
+extern "C" void frhip_reader_free(frhip_reader *r) { delete r; }
+
+extern "C" int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_error *err)
+{
+    // device_read.rs:155-182
+    uint64_t total = r->offsets.empty() ? 0 : r->offsets.back();
+    if (tail > total)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "carry offset %llu is past the batch's %llu bytes",
+                       (unsigned long long)tail, (unsigned long long)total);
+    if (tail == total) {
+        r->carry_start = 0;
+        return FRHIP_OK;
+    }
+    uint64_t block = (uint64_t)(std::upper_bound(r->offsets.begin(), r->offsets.end(), tail) - r->offsets.begin()) - 1;
+    uint64_t block_start = r->spans[block].compressed_offset - r->batch_offset;
+    r->carry.assign(r->compressed.begin() + (ptrdiff_t)block_start, r->compressed.end());
+    r->carry_start = tail - r->offsets[block];
+    return FRHIP_OK;
+}
+
+extern "C" int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_buf **out_data,
+                                       const uint64_t **out_offsets, const uint64_t **coffsets, uint64_t *n_blocks,
+                                       uint64_t *start, frhip_error *err)
+{
+    *has_batch = 0;
+    *out_data = nullptr;
+    for (;;) {
+        if (r->inner_exhausted && r->carry.empty()) return FRHIP_OK;
+        if (r->inner_exhausted && !r->compressed.empty() && r->carry.size() == r->compressed.size()) {
+            int k = set_err(err, FRHIP_E_MALFORMED, r->batch_offset,
+                            "stream ends inside a record: the trailing bytes are not a complete record");
+            if (err) std::strcpy(err->format, "bgzf");
+            return k;
+        }
+        r->batch_offset += r->compressed.size();
+        r->batch_offset -= r->carry.size();
+        r->compressed.swap(r->carry);
+        r->carry.clear();
+        const uint64_t want = r->blocks_per_batch * 65536ull;
+        const uint64_t remaining = r->src_len - r->src_pos;
+        const uint64_t take = std::min(want, remaining);
+        r->compressed.insert(r->compressed.end(), r->src + r->src_pos, r->src + r->src_pos + take);
+        r->src_pos += take;
+        if (remaining < want) r->inner_exhausted = true;
+
+        r->spans.clear();
+        uint64_t consumed = 0;
+        int rc = discover(r->compressed.data(), r->compressed.size(), r->batch_offset, r->spans, &consumed, err);
+        if (rc) return rc;
+        r->carry.assign(r->compressed.begin() + (ptrdiff_t)consumed, r->compressed.end());
+        if (r->inner_exhausted && !r->carry.empty()) {
+            int k = set_err(err, FRHIP_E_MALFORMED, r->batch_offset + consumed,
+                            "truncated BGZF: %llu trailing bytes are not a complete block",
+                            (unsigned long long)r->carry.size());
+            if (err) std::strcpy(err->format, "bgzf");
+            return k;
+        }
+        if (r->spans.empty()) {
+            if (r->inner_exhausted) return FRHIP_OK;
+            continue;
+        }
+        r->offsets.assign(r->spans.size() + 1, 0);
+        frhip_buf *data = nullptr;
+        rc = inflate_device(r->ctx, r->compressed.data(), r->compressed.size(), r->spans.data(), r->spans.size(),
+                            &data, r->offsets.data(), err, nullptr);
+        if (rc) return rc;
+        if (data->len == 0) {
+            buf_free(data);
+            if (r->inner_exhausted) return FRHIP_OK;
+            continue;
+        }
+        r->coffsets.resize(r->spans.size());
+        for (size_t i = 0; i < r->spans.size(); ++i) r->coffsets[i] = r->spans[i].compressed_offset;
+        *has_batch = 1;
+        *out_data = data;
+        *out_offsets = r->offsets.data();
+        *coffsets = r->coffsets.data();
+        *n_blocks = r->spans.size();
+        *start = r->carry_start;
+        r->carry_start = 0;
+        return FRHIP_OK;
+    }
+}

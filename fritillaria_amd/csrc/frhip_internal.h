@@ -1,0 +1,116 @@
+// frhip_internal.h -- kernel argument blocks and launchers shared by the HIP
+// sources and the host implementation. Not part of the public ABI.
+#ifndef FRHIP_INTERNAL_H
+#define FRHIP_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frhip {
+
+struct InflateArgs {
+    const uint8_t *in;       // compressed bytes (padded by at least 4 KiB)
+    const uint64_t *in_off;  // payload start per member, absolute in `in`
+    const uint32_t *in_len;  // payload length per member
+    uint8_t *out;            // inflated output
+    const uint64_t *out_off; // output offset per member (prefix sum of ISIZE)
+    const uint32_t *out_cap; // ISIZE per member
+    uint32_t *status;
+    uint32_t *produced;
+    uint32_t *crc;
+    uint32_t n;
+    uint32_t *queue;         // work-queue counter (persistent form) or null
+    const uint32_t *order;   // member order for the queue, or null
+    const uint32_t *x2n;     // x^(2^k) mod P, k = 0..31
+};
+
+hipError_t launch_inflate(const InflateArgs &a, uint32_t grid, hipStream_t stream);
+
+// Record-boundary discovery for BAM over an inflated batch.
+constexpr uint32_t BAM_K = 8;                 // survivors kept per block
+constexpr uint64_t BAM_NONE = ~0ull;          // block has no entry
+constexpr uint32_t BAM_CAPB = 65536 / 36 + 1; // most records that can start in one member
+constexpr uint32_t BAM_MODE_FALLBACK = 255;
+
+struct BamSpecArgs {
+    const uint8_t *buf;
+    uint64_t len;
+    const uint64_t *block_starts; // nb + 1
+    uint32_t nb;
+    uint32_t window;
+    uint8_t *spec_ok;
+    uint8_t *n_surv;
+    uint64_t *surv_pos;  // nb * BAM_K
+    uint32_t *surv_cnt;  // nb * BAM_K
+    uint64_t *land;
+    uint64_t *exitp;
+    uint32_t *cnt_tail;
+    uint8_t *complete;
+    uint16_t *rel_offs;  // nb * BAM_CAPB
+};
+
+struct BamReconcileArgs {
+    const uint8_t *buf;
+    uint64_t len;
+    const uint64_t *block_starts;
+    uint32_t nb;
+    uint64_t start;
+    const uint8_t *spec_ok;
+    const uint8_t *n_surv;
+    const uint64_t *surv_pos;
+    const uint32_t *surv_cnt;
+    const uint64_t *exitp;
+    const uint32_t *cnt_tail;
+    const uint8_t *complete;
+    uint64_t *entry;
+    uint32_t *count;
+    uint64_t *first_index;
+    uint8_t *mode;
+    uint64_t *totals;    // {records, tail, status, fallbacks}
+};
+
+struct BamEmitArgs {
+    const uint8_t *buf;
+    const uint64_t *block_starts;
+    const uint64_t *coffsets; // compressed offset per block (for virtual offsets), or null
+    uint32_t nb;
+    const uint64_t *entry;
+    const uint32_t *count;
+    const uint64_t *first_index;
+    const uint8_t *mode;
+    const uint32_t *surv_cnt;
+    const uint16_t *rel_offs;
+    uint64_t *record_offsets;
+    uint64_t *virtual_offsets; // or null
+};
+
+struct BamColumns {
+    int32_t *ref_id;
+    int32_t *pos;
+    uint16_t *flags;
+    uint8_t *mapq;
+    uint32_t *l_seq;
+    int32_t *next_ref_id;
+    int32_t *next_pos;
+    int32_t *tlen;
+    uint64_t *record_offsets;
+    uint32_t *cigar_start;
+    uint32_t *seq_start;
+    uint32_t *qual_start;
+    uint32_t *aux_start;
+    uint32_t *record_end;
+};
+
+struct BamDecodeArgs {
+    const uint8_t *buf;
+    uint64_t n;
+    const uint64_t *n_dev; // device-side record count (batched form), or null
+    BamColumns c;
+};
+
+hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s);
+hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s);
+hipError_t launch_bam_emit(const BamEmitArgs &a, hipStream_t s);
+hipError_t launch_bam_decode(const BamDecodeArgs &a, uint64_t max_records, hipStream_t s);
+
+} // namespace frhip
+#endif

@@ -1,0 +1,65 @@
+// host_internal.h -- host-side types behind the opaque handles of frhip.h.
+#ifndef FRHIP_HOST_INTERNAL_H
+#define FRHIP_HOST_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/frhip.h"
+
+namespace frhip {
+
+struct Scratch {
+    void *ptr = nullptr;
+    uint64_t cap = 0;
+    void *get(uint64_t bytes, frhip_error *err, int *rc);
+    void release();
+};
+
+struct PinnedScratch {
+    void *ptr = nullptr;
+    uint64_t cap = 0;
+    void *get(uint64_t bytes, frhip_error *err, int *rc);
+    void release();
+};
+
+struct InflateTiming {
+    double upload_ms = 0, kernel_ms = 0;
+};
+
+int set_err(frhip_error *e, int kind, uint64_t offset, const char *fmt, ...);
+int hip_err(frhip_error *e, hipError_t r, const char *what);
+const char *inflate_status_message(uint32_t status);
+int discover(const uint8_t *buf, uint64_t len, uint64_t base, std::vector<frhip_block_span> &spans,
+             uint64_t *position, frhip_error *err);
+int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t);
+hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n,
+                               uint32_t *crc, const uint32_t *x2n, hipStream_t s);
+
+} // namespace frhip
+
+struct frhip_buf {
+    void *ptr;
+    uint64_t len;
+    int ordinal;
+};
+
+struct frhip_ctx {
+    int ordinal = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t *x2n = nullptr;
+    uint32_t bam_window = 2048;
+    frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch;
+    frhip::PinnedScratch pinned;
+};
+
+namespace frhip {
+frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err);
+void buf_free(frhip_buf *b);
+} // namespace frhip
+
+#endif

@@ -1,0 +1,590 @@
+"""ctypes binding of ``include/frhip.h`` mirroring fritillaria's device API.
+
+Names follow the reference crates: ``HipCodec`` (CudaCodec,
+crates/fritillaria-cuda/src/lib.rs:152-294), ``DeviceInflateBatch``
+(crates/fritillaria-core/src/device.rs:145-296), ``BamDecoder`` /
+``DeviceRecordBatch`` (crates/fritillaria-cuda/src/bam.rs:103-187,
+crates/fritillaria-bam/src/columnar/device.rs:135-282), ``DeviceBgzfReader``
+(crates/fritillaria-bgzf/src/device_read.rs:47-280) and ``select_codec`` /
+``Backend`` (crates/fritillaria/src/lib.rs:151-260).
+
+There is no CPU fallback anywhere in this module: every call goes to the HIP
+library, and a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _build
+
+# ---------------------------------------------------------------------------
+# errors (crates/fritillaria-core/src/error.rs:14-70)
+# ---------------------------------------------------------------------------
+
+OK, E_INVALID_BLOCK, E_CHECKSUM_MISMATCH, E_SIZE_MISMATCH, E_INFLATE, E_MALFORMED, \
+    E_INVALID_DEVICE_BATCH, E_UNAVAILABLE, E_DEVICE, E_IO = range(10)
+
+
+class FrError(Exception):
+    kind = -1
+
+    def __init__(self, err: "_Err"):
+        self.offset = err.offset
+        self.code = err.code
+        self.expected = err.expected
+        self.actual = err.actual
+        self.format = err.format.decode(errors="replace")
+        self.reason = err.reason.decode(errors="replace")
+        super().__init__(self.describe())
+
+    def describe(self) -> str:
+        return f"{type(self).__name__} at {self.offset}: {self.reason}"
+
+
+class InvalidBlock(FrError):
+    kind = E_INVALID_BLOCK
+
+
+class ChecksumMismatch(FrError):
+    kind = E_CHECKSUM_MISMATCH
+
+
+class SizeMismatch(FrError):
+    kind = E_SIZE_MISMATCH
+
+
+class InflateError(FrError):
+    kind = E_INFLATE
+
+
+class Malformed(FrError):
+    kind = E_MALFORMED
+
+    @property
+    def position(self) -> int:
+        return self.offset
+
+
+class InvalidDeviceBatch(FrError):
+    kind = E_INVALID_DEVICE_BATCH
+
+
+class HipUnavailable(FrError):
+    kind = E_UNAVAILABLE
+
+
+class HipError(FrError):
+    kind = E_DEVICE
+
+
+_ERRORS = {c.kind: c for c in (InvalidBlock, ChecksumMismatch, SizeMismatch, InflateError, Malformed,
+                               InvalidDeviceBatch, HipUnavailable, HipError)}
+
+
+class _Err(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("code", C.c_uint32), ("offset", C.c_uint64),
+                ("expected", C.c_uint64), ("actual", C.c_uint64), ("format", C.c_char * 16),
+                ("reason", C.c_char * 200)]
+
+
+class BlockSpanC(C.Structure):
+    _fields_ = [("compressed_offset", C.c_uint64), ("payload_start", C.c_uint64),
+                ("payload_len", C.c_uint64), ("isize", C.c_uint32), ("crc32", C.c_uint32)]
+
+
+SPAN_DTYPE = np.dtype([("compressed_offset", "<u8"), ("payload_start", "<u8"),
+                       ("payload_len", "<u8"), ("isize", "<u4"), ("crc32", "<u4")])
+
+
+def _raise(rc: int, err: _Err):
+    if rc != OK:
+        raise _ERRORS.get(rc, FrError)(err)
+
+
+# ---------------------------------------------------------------------------
+# library
+# ---------------------------------------------------------------------------
+
+_lib = None
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_ep = C.POINTER(_Err)
+
+
+def lib():
+    """Loads libfrhip.so. Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = _build.frhip_lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    sig = {
+        "frhip_device_available": (C.c_int, []),
+        "frhip_ctx_create": (C.c_int, [C.c_int, _vp, C.POINTER(_vp), _ep]),
+        "frhip_ctx_destroy": (None, [_vp]),
+        "frhip_ctx_ordinal": (C.c_int, [_vp]),
+        "frhip_ctx_stream": (_vp, [_vp]),
+        "frhip_ctx_set_bam_window": (C.c_int, [_vp, C.c_uint32]),
+        "frhip_codec_name": (C.c_char_p, []),
+        "frhip_device_codec_name": (C.c_char_p, []),
+        "frhip_buf_len": (_u64, [_vp]),
+        "frhip_buf_ordinal": (C.c_int, [_vp]),
+        "frhip_buf_device_ptr": (_vp, [_vp]),
+        "frhip_buf_copy_to_host": (C.c_int, [_vp, _vp, _u64, _ep]),
+        "frhip_buf_free": (None, [_vp]),
+        "frhip_discover": (C.c_int, [_vp, _u64, _u64, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64), _ep]),
+        "frhip_inflate_batch_device": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _ep]),
+        "frhip_inflate_batch": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _ep]),
+        "frhip_crc32_blocks": (C.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _ep]),
+        "frhip_crc32_combine": (C.c_uint32, [C.c_uint32, C.c_uint32, _u64]),
+        "frhip_bam_decode": (C.c_int, [_vp, _vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_bam_batch_len": (_u64, [_vp]),
+        "frhip_bam_batch_tail": (_u64, [_vp]),
+        "frhip_bam_batch_fallbacks": (_u64, [_vp]),
+        "frhip_bam_batch_column": (_vp, [_vp, C.c_int]),
+        "frhip_bam_batch_free": (None, [_vp]),
+        "frhip_bam_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32), _ep]),
+        "frhip_reader_open": (C.c_int, [_vp, _vp, _u64, _u64, C.POINTER(_vp), _ep]),
+        "frhip_reader_next_batch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(_vp),
+                                              C.POINTER(C.POINTER(_u64)), C.POINTER(C.POINTER(_u64)),
+                                              C.POINTER(_u64), C.POINTER(_u64), _ep]),
+        "frhip_reader_carry_from": (C.c_int, [_vp, _u64, _ep]),
+        "frhip_reader_free": (None, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list:
+    """Every function declared in include/frhip.h, for the load/export check."""
+    import re
+    here = os.path.dirname(os.path.abspath(__file__))
+    names = []
+    for h in ("frhip.h", "frhip_pipeline.h"):
+        p = os.path.join(os.path.dirname(here), "include", h)
+        if os.path.exists(p):
+            txt = open(p).read()
+            names += re.findall(r"\b(frhip_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def device_is_available() -> bool:
+    return bool(lib().frhip_device_available())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+def _u8(data) -> np.ndarray:
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    return np.frombuffer(bytes(data), dtype=np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# context, buffers
+# ---------------------------------------------------------------------------
+
+
+class HipContext:
+    """CudaContext (crates/fritillaria-cuda/src/backend.rs:29-436)."""
+
+    def __init__(self, ordinal: int = 0, stream: Optional[int] = None):
+        h = _vp()
+        e = _Err()
+        rc = lib().frhip_ctx_create(ordinal, stream, C.byref(h), C.byref(e))
+        _raise(rc, e)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def ordinal(self) -> int:
+        return lib().frhip_ctx_ordinal(self._h)
+
+    @property
+    def stream(self) -> int:
+        return lib().frhip_ctx_stream(self._h) or 0
+
+    def set_bam_window(self, window: int) -> None:
+        lib().frhip_ctx_set_bam_window(self._h, window)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Owning handle to device memory (DeviceBuffer, device.rs:82-143)."""
+
+    def __init__(self, handle, owned: bool = True, keepalive=None):
+        self._h = handle
+        self._owned = owned
+        self._keep = keepalive
+
+    def byte_len(self) -> int:
+        return int(lib().frhip_buf_len(self._h))
+
+    def is_empty(self) -> bool:
+        return self.byte_len() == 0
+
+    def device_ordinal(self) -> int:
+        return lib().frhip_buf_ordinal(self._h)
+
+    def device_ptr(self) -> int:
+        return lib().frhip_buf_device_ptr(self._h) or 0
+
+    def copy_to_host(self, dst: np.ndarray) -> None:
+        e = _Err()
+        rc = lib().frhip_buf_copy_to_host(self._h, _ptr(dst), dst.nbytes, C.byref(e))
+        _raise(rc, e)
+
+    def to_vec(self) -> np.ndarray:
+        out = np.empty(self.byte_len(), dtype=np.uint8)
+        self.copy_to_host(out)
+        return out
+
+    def free(self):
+        if self._owned and self._h:
+            lib().frhip_buf_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class InflateBatch:
+    """Host output layout (InflateBatch, crates/fritillaria-core/src/codec.rs:53-157)."""
+    data: np.ndarray
+    offsets: np.ndarray
+
+    def __len__(self):
+        return len(self.offsets) - 1
+
+    def block(self, i: int) -> np.ndarray:
+        return self.data[int(self.offsets[i]):int(self.offsets[i + 1])]
+
+
+class DeviceInflateBatch:
+    """Inflated bytes that never left the device; offsets stay on the host."""
+
+    def __init__(self, data: Optional[DeviceBuffer] = None, offsets=None, coffsets=None):
+        self._data = data
+        self._offsets = np.zeros(1, dtype=np.uint64) if offsets is None else np.asarray(offsets, np.uint64)
+        self._coffsets = None if coffsets is None else np.asarray(coffsets, np.uint64)
+
+    def data(self) -> Optional[DeviceBuffer]:
+        return self._data
+
+    def offsets(self) -> np.ndarray:
+        return self._offsets
+
+    def compressed_offsets(self) -> Optional[np.ndarray]:
+        return self._coffsets
+
+    def __len__(self) -> int:
+        return len(self._offsets) - 1
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def byte_len(self) -> int:
+        return 0 if self._data is None else self._data.byte_len()
+
+    def device_ordinal(self) -> Optional[int]:
+        return None if self._data is None else self._data.device_ordinal()
+
+    def block_range(self, i: int):
+        return int(self._offsets[i]), int(self._offsets[i + 1])
+
+    def to_host(self) -> InflateBatch:
+        if self._data is None:
+            return InflateBatch(np.zeros(0, np.uint8), np.zeros(1, np.uint64))
+        return InflateBatch(self._data.to_vec(), self._offsets.copy())
+
+
+def _spans_array(spans) -> np.ndarray:
+    if isinstance(spans, np.ndarray) and spans.dtype == SPAN_DTYPE:
+        return np.ascontiguousarray(spans)
+    arr = np.zeros(len(spans), dtype=SPAN_DTYPE)
+    for i, s in enumerate(spans):
+        if isinstance(s, np.void):
+            arr[i] = s
+        else:
+            arr[i] = (s.compressed_offset, s.payload_start, s.payload_len, s.isize, s.crc32)
+    return arr
+
+
+def discover(data, base: int = 0):
+    """BlockDiscovery over host bytes: (spans, position). Raises InvalidBlock."""
+    a = _u8(data)
+    cap = a.size // 26 + 2
+    spans = np.zeros(cap, dtype=SPAN_DTYPE)
+    n = _u64()
+    pos = _u64()
+    e = _Err()
+    rc = lib().frhip_discover(_ptr(a), a.size, base, _ptr(spans), cap, C.byref(n), C.byref(pos), C.byref(e))
+    _raise(rc, e)
+    return spans[:n.value].copy(), pos.value
+
+
+class HipCodec:
+    """BlockCodec + DeviceBlockCodec over the HIP inflate kernel."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def name(self) -> str:
+        return lib().frhip_codec_name().decode()
+
+    def device_name(self) -> str:
+        return lib().frhip_device_codec_name().decode()
+
+    def device_ordinal(self) -> int:
+        return self.ctx.ordinal
+
+    def inflate_batch_device(self, batch, spans) -> DeviceInflateBatch:
+        a = _u8(batch)
+        sp = _spans_array(spans)
+        offs = np.zeros(sp.size + 1, dtype=np.uint64)
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_inflate_batch_device(self.ctx.handle, _ptr(a), a.size, _ptr(sp), sp.size,
+                                              C.byref(out), _ptr(offs), C.byref(e))
+        _raise(rc, e)
+        return DeviceInflateBatch(DeviceBuffer(out), offs, sp["compressed_offset"].copy())
+
+    def inflate_batch(self, batch, spans) -> InflateBatch:
+        a = _u8(batch)
+        sp = _spans_array(spans)
+        total = int(sp["isize"].astype(np.uint64).sum()) if sp.size else 0
+        out = np.empty(max(total, 1), dtype=np.uint8)
+        offs = np.zeros(sp.size + 1, dtype=np.uint64)
+        e = _Err()
+        rc = lib().frhip_inflate_batch(self.ctx.handle, _ptr(a), a.size, _ptr(sp), sp.size, _ptr(out),
+                                       out.size, _ptr(offs), C.byref(e))
+        _raise(rc, e)
+        return InflateBatch(out[:total], offs)
+
+    def crc32_blocks(self, data: DeviceBuffer, offsets, lengths) -> np.ndarray:
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+        out = np.zeros(o.size, dtype=np.uint32)
+        e = _Err()
+        rc = lib().frhip_crc32_blocks(self.ctx.handle, data._h, _ptr(o), _ptr(ln), o.size, _ptr(out),
+                                      C.byref(e))
+        _raise(rc, e)
+        return out
+
+
+def crc32_combine(crc1: int, crc2: int, len2: int) -> int:
+    return int(lib().frhip_crc32_combine(crc1, crc2, len2))
+
+
+# ---------------------------------------------------------------------------
+# BAM
+# ---------------------------------------------------------------------------
+
+# (name, dtype) in the order of crates/fritillaria-bam/src/columnar/device.rs:95-110
+BAM_COLUMNS = [
+    ("reference_sequence_id", np.int32), ("position", np.int32), ("flags", np.uint16),
+    ("mapping_quality", np.uint8), ("sequence_len", np.uint32),
+    ("mate_reference_sequence_id", np.int32), ("mate_position", np.int32),
+    ("template_length", np.int32), ("record_offsets", np.uint64), ("cigar_start", np.uint32),
+    ("sequence_start", np.uint32), ("quality_start", np.uint32), ("aux_start", np.uint32),
+    ("record_end", np.uint32), ("virtual_offsets", np.uint64),
+]
+_COL_INDEX = {n: i for i, (n, _) in enumerate(BAM_COLUMNS)}
+
+
+class DeviceRecordBatch:
+    """BAM columns in device memory. Columns point into the inflated batch,
+    which is kept alive by this object."""
+
+    def __init__(self, handle, source: DeviceInflateBatch):
+        self._h = handle
+        self._source = source
+
+    def __len__(self) -> int:
+        return int(lib().frhip_bam_batch_len(self._h))
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def tail(self) -> int:
+        return int(lib().frhip_bam_batch_tail(self._h))
+
+    def fallbacks(self) -> int:
+        return int(lib().frhip_bam_batch_fallbacks(self._h))
+
+    def column(self, name: str) -> Optional[DeviceBuffer]:
+        h = lib().frhip_bam_batch_column(self._h, _COL_INDEX[name])
+        return None if not h else DeviceBuffer(h, owned=False, keepalive=self)
+
+    def device_ordinal(self) -> int:
+        return self.column("reference_sequence_id").device_ordinal()
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(BAM_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def to_host(self) -> dict:
+        """Every column (the RecordBatch columns plus field bounds) on the host."""
+        cols = {}
+        for name, _ in BAM_COLUMNS:
+            if self.column(name) is not None:
+                cols[name] = self.column_to_host(name)
+        return cols
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_bam_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BamDecoder:
+    """BamDecoder (crates/fritillaria-cuda/src/bam.rs:103-187)."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def decode(self, batch: DeviceInflateBatch, start: int) -> DeviceRecordBatch:
+        data = batch.data()
+        offs = np.ascontiguousarray(batch.offsets(), dtype=np.uint64)
+        co = batch.compressed_offsets()
+        co = None if co is None else np.ascontiguousarray(co, dtype=np.uint64)
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_bam_decode(self.ctx.handle, None if data is None else data._h, _ptr(offs),
+                                    len(offs) - 1, None if co is None else _ptr(co), start,
+                                    C.byref(out), C.byref(e))
+        _raise(rc, e)
+        return DeviceRecordBatch(out, batch)
+
+
+def bam_parse_header(buf):
+    a = _u8(buf)
+    rs = _u64()
+    nref = C.c_uint32()
+    e = _Err()
+    rc = lib().frhip_bam_parse_header(_ptr(a), a.size, C.byref(rs), C.byref(nref), C.byref(e))
+    _raise(rc, e)
+    return rs.value, nref.value
+
+
+# ---------------------------------------------------------------------------
+# DeviceBgzfReader
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class DeviceBatch:
+    data: DeviceInflateBatch
+    start: int
+
+
+class DeviceBgzfReader:
+    """DeviceBgzfReader over a BGZF stream in host memory."""
+
+    def __init__(self, data, codec: HipCodec, blocks_per_batch: int = 256):
+        self._src = _u8(data)  # borrowed by the native reader: keep it alive
+        self.codec = codec
+        h = _vp()
+        e = _Err()
+        rc = lib().frhip_reader_open(codec.ctx.handle, _ptr(self._src), self._src.size,
+                                     max(1, blocks_per_batch), C.byref(h), C.byref(e))
+        _raise(rc, e)
+        self._h = h
+
+    def codec_name(self) -> str:
+        return self.codec.device_name()
+
+    def next_batch(self) -> Optional[DeviceBatch]:
+        has = C.c_int()
+        out = _vp()
+        offs = C.POINTER(_u64)()
+        co = C.POINTER(_u64)()
+        nb = _u64()
+        start = _u64()
+        e = _Err()
+        rc = lib().frhip_reader_next_batch(self._h, C.byref(has), C.byref(out), C.byref(offs), C.byref(co),
+                                           C.byref(nb), C.byref(start), C.byref(e))
+        _raise(rc, e)
+        if not has.value:
+            return None
+        n = nb.value
+        offsets = np.ctypeslib.as_array(offs, shape=(n + 1,)).copy()
+        coffsets = np.ctypeslib.as_array(co, shape=(n,)).copy()
+        return DeviceBatch(DeviceInflateBatch(DeviceBuffer(out), offsets, coffsets), start.value)
+
+    def carry_from(self, tail: int) -> None:
+        e = _Err()
+        rc = lib().frhip_reader_carry_from(self._h, tail, C.byref(e))
+        _raise(rc, e)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_reader_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# backend selection (crates/fritillaria/src/lib.rs:151-260)
+# ---------------------------------------------------------------------------
+
+
+class Backend(enum.Enum):
+    AUTO = "auto"
+    HIP = "hip"
+
+
+def select_codec(backend: Backend = Backend.AUTO):
+    """Returns (codec, resolved backend). An explicit HIP request is an assertion
+    and raises when no usable device exists; AUTO resolves to HIP when a device is
+    usable. This library ships no CPU codec (that is the host crate's CpuCodec),
+    so AUTO on a machine without a device raises HipUnavailable rather than
+    silently returning a different engine."""
+    if backend in (Backend.AUTO, Backend.HIP):
+        return HipCodec(), Backend.HIP
+    raise ValueError(backend)

@@ -1,0 +1,146 @@
+"""Synthetic BGZF/BAM inputs of the benchmark shapes (noodles framing).
+
+Wraps ``csrc/synth.c`` (built into ``libfrsynth.so``). The generators are
+deterministic for a given seed, whatever the thread count.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _build
+
+_lib = None
+
+
+class _BamParams(C.Structure):
+    _fields_ = [("n_records", C.c_uint64), ("seed", C.c_uint64), ("read_mode", C.c_int32),
+                ("len_fixed", C.c_uint32), ("len_min", C.c_uint32), ("len_max", C.c_uint32),
+                ("ln_median", C.c_double), ("ln_sigma", C.c_double), ("len_cap", C.c_uint32),
+                ("indel_p", C.c_double), ("aux_mode", C.c_int32), ("unaligned", C.c_int32),
+                ("ref_len", C.c_uint32), ("threads", C.c_int32)]
+
+
+class _BgzfParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("level_mode", C.c_int32), ("level", C.c_int32),
+                ("stored_frac", C.c_double), ("payload_cap", C.c_uint32), ("strategy", C.c_int32),
+                ("eof_member", C.c_int32), ("threads", C.c_int32)]
+
+
+def _l():
+    global _lib
+    if _lib is None:
+        path = _build.synth_lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(path)
+        _lib.synth_bam_stream.restype = C.c_int
+        _lib.synth_bam_stream.argtypes = [C.POINTER(_BamParams), C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        _lib.synth_bgzf.restype = C.c_int
+        _lib.synth_bgzf.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(_BgzfParams),
+                                    C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+        _lib.synth_free.restype = None
+        _lib.synth_free.argtypes = [C.c_void_p]
+    return _lib
+
+
+def _take(ptr: C.c_void_p, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint8)
+    if n:
+        C.memmove(out.ctypes.data, ptr.value, n)
+    _l().synth_free(ptr)
+    return out
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def bam_stream(n_records: int, seed: int, *, read_len=150, read_range=None, lognormal=None,
+               indel_p=0.0, long_aux=False, unaligned=False, ref_len=1_000_000,
+               threads=None):
+    """Uncompressed BAM stream (header + records). Returns (bytes array, records_start)."""
+    p = _BamParams()
+    p.n_records = n_records
+    p.seed = seed
+    if lognormal is not None:
+        p.read_mode = 2
+        p.ln_median, p.ln_sigma, p.len_cap = lognormal
+    elif read_range is not None:
+        p.read_mode = 1
+        p.len_min, p.len_max = read_range
+    else:
+        p.read_mode = 0
+        p.len_fixed = read_len
+    p.indel_p = indel_p
+    p.aux_mode = 1 if long_aux else 0
+    p.unaligned = 1 if unaligned else 0
+    p.ref_len = ref_len
+    p.threads = threads or default_threads()
+    ptr = C.c_void_p()
+    n = C.c_uint64()
+    rs = C.c_uint64()
+    rc = _l().synth_bam_stream(C.byref(p), C.byref(ptr), C.byref(n), C.byref(rs))
+    if rc != 0:
+        raise RuntimeError(f"synth_bam_stream failed: {rc}")
+    return _take(ptr, n.value), rs.value
+
+
+def bgzf(data: np.ndarray, *, level=6, mixed_levels=False, stored_frac=0.0, seed=0,
+         payload_cap=65495, eof=True, strategy=0, threads=None) -> np.ndarray:
+    """Frames an uncompressed stream as BGZF members of `payload_cap` bytes each."""
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    b = _BgzfParams()
+    b.seed = seed
+    b.level_mode = 1 if mixed_levels else 0
+    b.level = level
+    b.stored_frac = stored_frac
+    b.payload_cap = payload_cap
+    b.strategy = strategy
+    b.eof_member = 1 if eof else 0
+    b.threads = threads or default_threads()
+    ptr = C.c_void_p()
+    n = C.c_uint64()
+    rc = _l().synth_bgzf(data.ctypes.data_as(C.c_void_p), data.size, C.byref(b), C.byref(ptr),
+                         C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"synth_bgzf failed: {rc}")
+    return _take(ptr, n.value)
+
+
+@dataclass
+class SynthBam:
+    compressed: np.ndarray
+    inflated_len: int
+    records_start: int
+    n_records: int
+
+
+CONFIG_SEED = 0x5EED0000
+
+
+def config_bam(config: int, n_records=None, threads=None) -> SynthBam:
+    """BAM inputs of the benchmark configurations (1: short reads, 2: batched large,
+    3: unaligned long reads); `n_records` scales a configuration down for tests."""
+    seed = CONFIG_SEED + config
+    if config == 1:
+        n = n_records or 10_000
+        u, rs = bam_stream(n, seed, read_len=150, ref_len=1_000_000, threads=threads)
+        c = bgzf(u, level=6, seed=seed, threads=threads)
+    elif config == 2:
+        n = n_records or 20_000_000
+        u, rs = bam_stream(n, seed, read_range=(100, 250), indel_p=0.01, ref_len=250_000_000,
+                           threads=threads)
+        c = bgzf(u, mixed_levels=True, stored_frac=0.01, seed=seed, threads=threads)
+    elif config == 3:
+        n = n_records or 50_000
+        u, rs = bam_stream(n, seed, lognormal=(20_000.0, 1.0, 1_000_000), long_aux=True,
+                           unaligned=True, ref_len=1_000_000, threads=threads)
+        c = bgzf(u, level=6, seed=seed, threads=threads)
+    else:
+        raise ValueError(f"no BAM configuration {config}")
+    return SynthBam(c, int(u.size), rs, n)

@@ -1,0 +1,192 @@
+/*
+ * frhip.h -- C ABI of the MI355X (gfx950) device path of fritillaria.
+ *
+ * This is the boundary a `fritillaria-hip` crate binds in place of
+ * `fritillaria-cuda`. Plain pointers and sizes only; every call is synchronous
+ * on return unless stated otherwise, like the reference (each public call
+ * synchronises before reading status: crates/fritillaria-cuda/src/backend.rs:463-582).
+ *
+ * Ownership follows crates/fritillaria-core/src/device.rs:1-40: a frhip_buf owns
+ * device memory and frhip_buf_free releases it; decoded BAM columns point INTO
+ * the inflated batch, which must outlive them.
+ *
+ * Return values: FRHIP_OK (0) or one of frhip_error_kind; details in *err when
+ * err is not NULL.
+ */
+#ifndef FRHIP_H
+#define FRHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fritillaria_core::Error (crates/fritillaria-core/src/error.rs:14-70). */
+typedef enum {
+    FRHIP_OK = 0,
+    FRHIP_E_INVALID_BLOCK = 1,        /* InvalidBlock{offset, reason} */
+    FRHIP_E_CHECKSUM_MISMATCH = 2,    /* ChecksumMismatch{offset, expected, actual} */
+    FRHIP_E_SIZE_MISMATCH = 3,        /* SizeMismatch{offset, expected, actual} */
+    FRHIP_E_INFLATE = 4,              /* Inflate{offset, reason}; code = FR_ERR_* status */
+    FRHIP_E_MALFORMED = 5,            /* Malformed{format, position(offset), reason} */
+    FRHIP_E_INVALID_DEVICE_BATCH = 6, /* InvalidDeviceBatch{reason} */
+    FRHIP_E_UNAVAILABLE = 7,          /* CudaUnavailable: no usable device */
+    FRHIP_E_DEVICE = 8,               /* Cuda(String): a runtime call failed */
+    FRHIP_E_IO = 9                    /* Io */
+} frhip_error_kind;
+
+typedef struct {
+    int32_t kind;         /* frhip_error_kind */
+    uint32_t code;        /* device status for FRHIP_E_INFLATE / FRHIP_E_MALFORMED */
+    uint64_t offset;      /* compressed offset of the member, or position for Malformed */
+    uint64_t expected;
+    uint64_t actual;
+    char format[16];      /* Malformed: "bam", "bgzf", "bcf", "fastq", "bam header" */
+    char reason[200];
+} frhip_error;
+
+/* BlockSpan (crates/fritillaria-core/src/codec.rs:17-31); usize fields widened to u64. */
+typedef struct {
+    uint64_t compressed_offset;
+    uint64_t payload_start;
+    uint64_t payload_len;
+    uint32_t isize;
+    uint32_t crc32;
+} frhip_block_span;
+
+typedef struct frhip_ctx frhip_ctx;
+typedef struct frhip_buf frhip_buf;
+typedef struct frhip_bam_batch frhip_bam_batch;
+typedef struct frhip_reader frhip_reader;
+
+/* ---- device and context ------------------------------------------------ */
+
+/* device_is_available (crates/fritillaria-cuda/src/lib.rs:136-145): 1 if a gfx950 device can be used. */
+int frhip_device_available(void);
+
+/* CudaContext::new(ordinal) / with_context(ctx, stream) (backend.rs:390-436;
+   lib.rs:194-201). `stream` is a hipStream_t owned by the caller, or NULL for a
+   stream of the context's own. Kernels are precompiled code objects: no JIT. */
+int frhip_ctx_create(int ordinal, void *stream, frhip_ctx **out, frhip_error *err);
+void frhip_ctx_destroy(frhip_ctx *ctx);
+int frhip_ctx_ordinal(const frhip_ctx *ctx);
+/* hipStream_t the context launches on. */
+void *frhip_ctx_stream(const frhip_ctx *ctx);
+/* Bytes of each member whose offsets are tried as record-start candidates by the
+   BAM boundary scan (default 2048). A tuning knob: results do not depend on it.
+   No reference counterpart; the reference speculates on member starts only. */
+int frhip_ctx_set_bam_window(frhip_ctx *ctx, uint32_t window);
+/* Codec names of CudaCodec (lib.rs:152-294): host path and device path. */
+const char *frhip_codec_name(void);        /* "hip" */
+const char *frhip_device_codec_name(void); /* "hip-device" */
+
+/* ---- device buffers: DeviceAlloc (crates/fritillaria-core/src/device.rs:51-75) ---- */
+
+uint64_t frhip_buf_len(const frhip_buf *b);
+int frhip_buf_ordinal(const frhip_buf *b);
+void *frhip_buf_device_ptr(const frhip_buf *b);
+/* copy_to_host: rejects dst_len != byte_len with FRHIP_E_INVALID_DEVICE_BATCH and synchronises. */
+int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t dst_len, frhip_error *err);
+void frhip_buf_free(frhip_buf *b);
+
+/* ---- BGZF member discovery (crates/fritillaria-bgzf/src/discover.rs:51-113) ---- */
+
+/* Host walk of member headers in buf[0..len), base = absolute offset of buf[0].
+   Writes up to cap spans; *n = spans written; *position = resume point.
+   Returns FRHIP_E_INVALID_BLOCK on a corrupt header (then *n spans precede it). */
+int frhip_discover(const uint8_t *buf, uint64_t len, uint64_t base, frhip_block_span *spans,
+                   uint64_t cap, uint64_t *n, uint64_t *position, frhip_error *err);
+
+/* ---- inflate: BlockCodec / DeviceBlockCodec ----------------------------- */
+
+/* DeviceBlockCodec::inflate_batch_device (crates/fritillaria-core/src/device.rs:306-324;
+   crates/fritillaria-cuda/src/backend.rs:536-582). Uploads `batch`, inflates every
+   span, verifies status -> ISIZE -> CRC32 in member order (backend.rs:221-256).
+   On success *out_data owns sum(isize) bytes (NULL when that is 0: an empty batch
+   owns no allocation) and out_offsets[0..n] holds the member boundaries.
+   On failure nothing is returned. */
+int frhip_inflate_batch_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                               const frhip_block_span *spans, uint64_t n, frhip_buf **out_data,
+                               uint64_t *out_offsets, frhip_error *err);
+
+/* BlockCodec::inflate_batch (crates/fritillaria-core/src/codec.rs:159-173): the same,
+   then a download into out[0..out_cap). */
+int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                        const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
+                        uint64_t *out_offsets, frhip_error *err);
+
+/* crc32_blocks (crates/fritillaria-cuda/src/backend.rs:737-810): CRC32 of byte
+   ranges data[off[i] .. off[i]+len[i]) of a device buffer. */
+int frhip_crc32_blocks(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *offsets,
+                       const uint32_t *lengths, uint64_t n, uint32_t *crc_out, frhip_error *err);
+
+/* zlib-compatible crc32_combine on the host, exported for tests of the CRC fold. */
+uint32_t frhip_crc32_combine(uint32_t crc1, uint32_t crc2, uint64_t len2);
+
+/* ---- BAM: BamDecoder::decode (crates/fritillaria-cuda/src/bam.rs:161-187, 324-574) ---- */
+
+/* Column indices of DeviceRecordBatch, in the order of
+   crates/fritillaria-bam/src/columnar/device.rs:95-110, plus the build's own. */
+typedef enum {
+    FRHIP_BAM_REFERENCE_SEQUENCE_ID = 0, /* i32 */
+    FRHIP_BAM_POSITION = 1,              /* i32, 0-based */
+    FRHIP_BAM_FLAGS = 2,                 /* u16 */
+    FRHIP_BAM_MAPPING_QUALITY = 3,       /* u8 */
+    FRHIP_BAM_SEQUENCE_LEN = 4,          /* u32 */
+    FRHIP_BAM_MATE_REFERENCE_SEQUENCE_ID = 5, /* i32 */
+    FRHIP_BAM_MATE_POSITION = 6,         /* i32 */
+    FRHIP_BAM_TEMPLATE_LENGTH = 7,       /* i32 */
+    FRHIP_BAM_RECORD_OFFSETS = 8,        /* u64, into the inflated batch */
+    FRHIP_BAM_CIGAR_START = 9,           /* u32, relative to the record start */
+    FRHIP_BAM_SEQUENCE_START = 10,       /* u32 */
+    FRHIP_BAM_QUALITY_START = 11,        /* u32 */
+    FRHIP_BAM_AUX_START = 12,            /* u32 */
+    FRHIP_BAM_RECORD_END = 13,           /* u32 */
+    FRHIP_BAM_VIRTUAL_OFFSETS = 14,      /* u64 (coffset << 16 | uoffset); only when coffsets were given */
+    FRHIP_BAM_N_COLUMNS = 15
+} frhip_bam_column;
+
+/* Discovers record boundaries from `start` and decodes the columns.
+   `data` is an inflated batch (NULL or empty: no records, tail = start),
+   block_offsets its n_blocks+1 member boundaries. `coffsets` (n_blocks entries,
+   may be NULL) are the members' compressed file offsets for the virtual-offset
+   column. Malformed records yield FRHIP_E_MALFORMED with offset = the position
+   of blocked.rs's reconcile. */
+int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
+                     uint64_t n_blocks, const uint64_t *coffsets, uint64_t start,
+                     frhip_bam_batch **out, frhip_error *err);
+uint64_t frhip_bam_batch_len(const frhip_bam_batch *b);
+uint64_t frhip_bam_batch_tail(const frhip_bam_batch *b);
+/* Members the true chain did not enter at a speculated candidate (a diagnostic,
+   like BlockedScan::fallbacks, blocked.rs:80-88). */
+uint64_t frhip_bam_batch_fallbacks(const frhip_bam_batch *b);
+/* Column i, or NULL if absent. Owned by the batch. */
+const frhip_buf *frhip_bam_batch_column(const frhip_bam_batch *b, int column);
+void frhip_bam_batch_free(frhip_bam_batch *b);
+
+/* BAM header (crates/fritillaria-bam/src/columnar/header.rs:43-103) on host bytes. */
+int frhip_bam_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start,
+                           uint32_t *n_ref, frhip_error *err);
+
+/* ---- DeviceBgzfReader (crates/fritillaria-bgzf/src/device_read.rs:47-280) ---- */
+
+/* A reader over a BGZF stream held in host memory [data, data+len) (borrowed for
+   the reader's lifetime). blocks_per_batch budgets compressed bytes at 64 KiB per
+   member, as with_blocks_per_batch (device_read.rs:113-116); 0 = 256. */
+int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t len, uint64_t blocks_per_batch,
+                      frhip_reader **out, frhip_error *err);
+/* next_batch: out_data, out_offsets and n_blocks describe the inflated batch
+   (ownership of *out_data passes to the caller), *start = DeviceBatch::start,
+   *coffsets points at the members' compressed offsets (valid until the next call).
+   Returns FRHIP_OK with *has_batch = 0 at end of stream. */
+int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_buf **out_data,
+                            const uint64_t **out_offsets, const uint64_t **coffsets,
+                            uint64_t *n_blocks, uint64_t *start, frhip_error *err);
+/* carry_from (device_read.rs:155-182). */
+int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_error *err);
+void frhip_reader_free(frhip_reader *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

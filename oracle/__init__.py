@@ -1,0 +1,231 @@
+"""CPU oracle for fritillaria's device path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker (or the timed CPU
+baseline), never as the thing measured or shipped. The product library
+(``fritillaria_amd``) does not import it and has no CPU fallback.
+
+The C sources (``oracle.c``, ``oracle_mt.c``) restate the reference algorithm;
+each function there cites the reference file:line it follows and names what
+pins it. This module is a thin ctypes/numpy wrapper.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+# status codes (crates/fritillaria-cuda/kernels/inflate.cu:29-39)
+OK, TRUNCATED, BLOCK_TYPE, STORED_LEN, OUTPUT_OVERFLOW, BAD_CODE, BAD_DISTANCE, BAD_TABLE, \
+    BAD_LENGTH_REPEAT, BAD_COUNTS = range(10)
+# error kinds (crates/fritillaria-core/src/error.rs)
+E_OK, E_INVALID_BLOCK, E_CHECKSUM_MISMATCH, E_SIZE_MISMATCH, E_INFLATE, E_MALFORMED, \
+    E_INVALID_DEVICE_BATCH = range(7)
+
+
+class Span(C.Structure):
+    _fields_ = [("compressed_offset", C.c_uint64), ("payload_start", C.c_uint64),
+                ("payload_len", C.c_uint64), ("isize", C.c_uint32), ("crc32", C.c_uint32)]
+
+
+class Err(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("code", C.c_uint32), ("offset", C.c_uint64),
+                ("expected", C.c_uint64), ("actual", C.c_uint64)]
+
+
+SPAN_DTYPE = np.dtype([("compressed_offset", "<u8"), ("payload_start", "<u8"),
+                       ("payload_len", "<u8"), ("isize", "<u4"), ("crc32", "<u4")])
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        u8p = C.POINTER(C.c_uint8)
+        _lib.or_crc32.restype = C.c_uint32
+        _lib.or_crc32.argtypes = [C.c_void_p, C.c_uint64]
+        _lib.or_inflate_member.restype = C.c_int
+        _lib.or_inflate_member.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                           C.POINTER(C.c_uint32)]
+        _lib.or_discover.restype = C.c_uint64
+        _lib.or_discover.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64,
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_uint64)]
+        _lib.or_inflate_batch.restype = C.c_int
+        _lib.or_inflate_batch.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                          C.c_void_p, C.c_void_p, C.POINTER(Err)]
+        _lib.or_bam_parse_header.restype = C.c_int
+        _lib.or_bam_parse_header.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]
+        _lib.or_bam_scan_records.restype = C.c_int
+        _lib.or_bam_scan_records.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                             C.c_uint64, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64)]
+        _lib.or_bam_scan_blocked.restype = C.c_int
+        _lib.or_bam_scan_blocked.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                             C.c_uint64, C.c_void_p, C.c_uint64,
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64)]
+        _lib.or_bam_decode_columns.restype = None
+        _lib.or_bam_decode_columns.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_void_p] * 14
+        _lib.or_virtual_offset.restype = C.c_uint64
+        _lib.or_virtual_offset.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]
+        _lib.or_mt_bam_columns.restype = C.c_int
+        _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
+        del u8p
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _as_u8(data) -> np.ndarray:
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data, dtype=np.uint8)
+    return np.frombuffer(bytes(data), dtype=np.uint8)
+
+
+def crc32(data) -> int:
+    a = _as_u8(data)
+    return int(lib().or_crc32(_ptr(a), a.size))
+
+
+def inflate_member(payload, cap: int):
+    """Returns (status, bytes produced) for one raw DEFLATE stream."""
+    a = _as_u8(payload)
+    out = np.zeros(max(cap, 1), dtype=np.uint8)
+    produced = C.c_uint32(0)
+    st = lib().or_inflate_member(_ptr(a), a.size, _ptr(out), cap, C.byref(produced))
+    return int(st), out[:produced.value].tobytes()
+
+
+@dataclass
+class Discovery:
+    spans: np.ndarray  # SPAN_DTYPE
+    position: int
+    err: int
+    err_offset: int
+
+
+def discover(data, base: int = 0) -> Discovery:
+    a = _as_u8(data)
+    cap = a.size // 26 + 2
+    spans = np.zeros(cap, dtype=SPAN_DTYPE)
+    pos = C.c_uint64(0)
+    err = C.c_int(0)
+    eoff = C.c_uint64(0)
+    n = lib().or_discover(_ptr(a), a.size, base, _ptr(spans), cap, C.byref(pos), C.byref(err),
+                          C.byref(eoff))
+    return Discovery(spans[:n].copy(), pos.value, err.value, eoff.value)
+
+
+@dataclass
+class InflateResult:
+    kind: int
+    code: int
+    offset: int
+    expected: int
+    actual: int
+    data: bytes
+    offsets: np.ndarray
+
+
+def inflate_batch(batch, spans: np.ndarray) -> InflateResult:
+    a = _as_u8(batch)
+    spans = np.ascontiguousarray(spans, dtype=SPAN_DTYPE)
+    total = int(spans["isize"].astype(np.uint64).sum()) if spans.size else 0
+    out = np.zeros(max(total, 1), dtype=np.uint8)
+    offs = np.zeros(spans.size + 1, dtype=np.uint64)
+    e = Err()
+    k = lib().or_inflate_batch(_ptr(a), a.size, _ptr(spans), spans.size, _ptr(out), _ptr(offs),
+                               C.byref(e))
+    if k:
+        return InflateResult(k, e.code, e.offset, e.expected, e.actual, b"", np.zeros(1, np.uint64))
+    return InflateResult(0, 0, 0, 0, 0, out[:total].tobytes(), offs)
+
+
+def bam_parse_header(buf):
+    a = _as_u8(buf)
+    rs = C.c_uint64(0)
+    ep = C.c_uint64(0)
+    nref = C.c_uint32(0)
+    rc = lib().or_bam_parse_header(_ptr(a), a.size, C.byref(rs), C.byref(ep), C.byref(nref))
+    return rc, rs.value, ep.value, nref.value
+
+
+def bam_scan_records(buf, start: int):
+    a = _as_u8(buf)
+    cap = a.size // 36 + 1
+    offs = np.zeros(cap, dtype=np.uint64)
+    n = C.c_uint64(0)
+    tail = C.c_uint64(0)
+    rc = lib().or_bam_scan_records(_ptr(a), a.size, start, _ptr(offs), cap, C.byref(n),
+                                   C.byref(tail))
+    return rc, offs[:n.value].copy(), tail.value
+
+
+def bam_scan_blocked(buf, block_starts, start: int):
+    """Returns (rc, offsets, tail, fallbacks); rc 1 = malformed at `tail`."""
+    a = _as_u8(buf)
+    bs = np.ascontiguousarray(block_starts, dtype=np.uint64)
+    cap = a.size // 36 + 1
+    offs = np.zeros(cap, dtype=np.uint64)
+    n = C.c_uint64(0)
+    tail = C.c_uint64(0)
+    fb = C.c_uint64(0)
+    rc = lib().or_bam_scan_blocked(_ptr(a), a.size, _ptr(bs), bs.size - 1, start, _ptr(offs), cap,
+                                   C.byref(n), C.byref(tail), C.byref(fb))
+    return rc, offs[:n.value].copy(), tail.value, fb.value
+
+
+# Column order and widths of crates/fritillaria-bam/src/columnar/device.rs:95-110.
+BAM_COLUMNS = [
+    ("reference_sequence_id", np.int32), ("position", np.int32), ("flags", np.uint16),
+    ("mapping_quality", np.uint8), ("sequence_len", np.uint32),
+    ("mate_reference_sequence_id", np.int32), ("mate_position", np.int32),
+    ("template_length", np.int32), ("record_offsets", np.uint64), ("cigar_start", np.uint32),
+    ("sequence_start", np.uint32), ("quality_start", np.uint32), ("aux_start", np.uint32),
+    ("record_end", np.uint32),
+]
+
+
+def bam_decode_columns(buf, offsets) -> dict:
+    a = _as_u8(buf)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = offs.size
+    cols = {name: np.zeros(n, dtype=dt) for name, dt in BAM_COLUMNS}
+    lib().or_bam_decode_columns(_ptr(a), _ptr(offs), n, *[_ptr(cols[name]) for name, _ in BAM_COLUMNS])
+    return cols
+
+
+def virtual_offsets(block_offsets, coffsets, positions) -> np.ndarray:
+    bo = np.ascontiguousarray(block_offsets, dtype=np.uint64)
+    co = np.ascontiguousarray(coffsets, dtype=np.uint64)
+    out = np.zeros(len(positions), dtype=np.uint64)
+    f = lib().or_virtual_offset
+    for i, p in enumerate(positions):
+        out[i] = f(_ptr(bo), _ptr(co), co.size, int(p))
+    return out
+
+
+def mt_bam_columns(file_bytes, nthreads: int):
+    a = _as_u8(file_bytes)
+    res = np.zeros(7, dtype=np.uint64)
+    rc = lib().or_mt_bam_columns(_ptr(a), a.size, nthreads, _ptr(res))
+    return rc, {"records": int(res[0]), "inflated": int(res[1]), "sum_pos": int(res[2]),
+                "sum_flag": int(res[3]), "sum_lseq": int(res[4]), "sum_off": int(res[5]),
+                "members": int(res[6])}

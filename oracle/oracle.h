@@ -1,0 +1,93 @@
+/*
+ * oracle.h -- test-only CPU restatement of fritillaria's device path.
+ * Nothing in the product library includes this header or links the oracle.
+ */
+#ifndef FR_ORACLE_H
+#define FR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device status codes of crates/fritillaria-cuda/kernels/inflate.cu:29-39. */
+enum {
+    OR_OK = 0,
+    OR_ERR_TRUNCATED = 1,
+    OR_ERR_BLOCK_TYPE = 2,
+    OR_ERR_STORED_LEN = 3,
+    OR_ERR_OUTPUT_OVERFLOW = 4,
+    OR_ERR_BAD_CODE = 5,
+    OR_ERR_BAD_DISTANCE = 6,
+    OR_ERR_BAD_TABLE = 7,
+    OR_ERR_BAD_LENGTH_REPEAT = 8,
+    OR_ERR_BAD_COUNTS = 9
+};
+
+/* Header parse failures of crates/fritillaria-bgzf/src/block.rs:99-144, in check order. */
+enum {
+    OR_BLK_TRUNCATED_HEADER = 1,
+    OR_BLK_BAD_MAGIC = 2,
+    OR_BLK_BAD_METHOD = 3,
+    OR_BLK_NO_FEXTRA = 4,
+    OR_BLK_TRUNCATED_EXTRA = 5,
+    OR_BLK_NO_BC = 6,
+    OR_BLK_TOO_BIG = 7,
+    OR_BLK_TOO_SMALL = 8,
+    OR_BLK_ISIZE_TOO_BIG = 9
+};
+
+/* fritillaria_core::Error variants (crates/fritillaria-core/src/error.rs:14-70). */
+enum {
+    OR_E_OK = 0,
+    OR_E_INVALID_BLOCK = 1,
+    OR_E_CHECKSUM_MISMATCH = 2,
+    OR_E_SIZE_MISMATCH = 3,
+    OR_E_INFLATE = 4,
+    OR_E_MALFORMED = 5,
+    OR_E_INVALID_DEVICE_BATCH = 6
+};
+
+typedef struct {
+    uint64_t compressed_offset;
+    uint64_t payload_start;
+    uint64_t payload_len;
+    uint32_t isize;
+    uint32_t crc32;
+} or_span_t;
+
+typedef struct {
+    int32_t kind;
+    uint32_t code;
+    uint64_t offset;
+    uint64_t expected;
+    uint64_t actual;
+} or_error_t;
+
+uint32_t or_crc32(const uint8_t *p, uint64_t len);
+int or_inflate_member(const uint8_t *in, uint32_t in_len, uint8_t *out, uint32_t out_cap,
+                      uint32_t *produced);
+int or_parse_header(const uint8_t *buf, uint64_t len, uint32_t *block_size, uint32_t *header_size);
+uint64_t or_discover(const uint8_t *buf, uint64_t len, uint64_t base, or_span_t *spans,
+                     uint64_t cap, uint64_t *position, int *err, uint64_t *err_offset);
+int or_inflate_batch(const uint8_t *batch, uint64_t batch_len, const or_span_t *spans, uint64_t n,
+                     uint8_t *out, uint64_t *offsets, or_error_t *e);
+int or_bam_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start, uint64_t *err_pos,
+                        uint32_t *n_ref);
+int or_bam_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets,
+                        uint64_t cap, uint64_t *n, uint64_t *tail);
+int or_bam_scan_blocked(const uint8_t *buf, uint64_t len, const uint64_t *block_starts, uint64_t nb,
+                        uint64_t start, uint64_t *offsets, uint64_t cap, uint64_t *n,
+                        uint64_t *tail, uint64_t *fallbacks);
+void or_bam_decode_columns(const uint8_t *buf, const uint64_t *offsets, uint64_t n,
+                           int32_t *ref_id, int32_t *pos, uint16_t *flags, uint8_t *mapq,
+                           uint32_t *l_seq, int32_t *next_ref_id, int32_t *next_pos, int32_t *tlen,
+                           uint64_t *rec_off, uint32_t *cigar_start, uint32_t *seq_start,
+                           uint32_t *qual_start, uint32_t *aux_start, uint32_t *rec_end);
+uint64_t or_virtual_offset(const uint64_t *block_offsets, const uint64_t *coffsets, uint64_t nb,
+                           uint64_t p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

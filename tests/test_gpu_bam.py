@@ -1,0 +1,235 @@
+"""GPU parity of BamDecoder::decode and the DeviceBgzfReader carry loop against
+the oracle. Mirrors crates/fritillaria-cuda/tests/{bam_device,bam_reader}.rs."""
+import numpy as np
+import pytest
+
+from conftest import golden_bytes
+from helpers import bam_header, bgzf, build_record, random_records
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["htslib.bam", "htslib_multiblock.bam", "ont_ultralong.bam", "pacbio_hifi.bam"]
+REF_COLS = ["reference_sequence_id", "position", "flags", "mapping_quality", "sequence_len",
+            "mate_reference_sequence_id", "mate_position", "template_length", "record_offsets",
+            "cigar_start", "sequence_start", "quality_start", "aux_start", "record_end"]
+
+
+def oracle_decode(oracle_mod, u, offsets, start):
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, offsets, start)
+    return rc, rec, tail, oracle_mod.bam_decode_columns(u, rec)
+
+
+def check_batch(oracle_mod, records, u, offsets, coffsets, start):
+    rc, rec, tail, cols = oracle_decode(oracle_mod, u, offsets, start)
+    assert rc == 0
+    assert len(records) == len(rec) and records.tail() == tail
+    got = records.to_host()
+    for name in REF_COLS:
+        assert np.array_equal(got[name], cols[name]), name
+    if coffsets is not None:
+        want_v = oracle_mod.virtual_offsets(offsets, coffsets, rec)
+        assert np.array_equal(got["virtual_offsets"], want_v)
+    return got
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_columns_match_oracle(codec, decoder, oracle_mod, name):
+    print(f"[gpu] bam fixture {name}", flush=True)
+    import fritillaria_amd as fa
+    buf = golden_bytes(name)
+    spans, _ = fa.discover(buf)
+    batch = codec.inflate_batch_device(buf, spans)
+    u = np.frombuffer(oracle_mod.inflate_batch(buf, spans).data, np.uint8)
+    start, _ = fa.bam_parse_header(u)
+    records = decoder.decode(batch, start)
+    check_batch(oracle_mod, records, u, batch.offsets(), batch.compressed_offsets(), start)
+
+
+@pytest.mark.parametrize("config,n", [(1, 10000), (2, 30000), (3, 60)])
+def test_synthetic_noodles_framed(codec, decoder, oracle_mod, config, n):
+    print(f"[gpu] bam synthetic config {config} n={n}", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(config, n_records=n)
+    spans, _ = fa.discover(s.compressed)
+    batch = codec.inflate_batch_device(s.compressed, spans)
+    u = batch.data().to_vec()
+    records = decoder.decode(batch, s.records_start)
+    got = check_batch(oracle_mod, records, u, batch.offsets(), batch.compressed_offsets(), s.records_start)
+    assert len(got["position"]) == n
+    if config != 3:
+        # noodles framing puts member starts mid-record; the window speculation
+        # must still resolve nearly every member without a fallback walk
+        assert records.fallbacks() <= 2 + len(spans) // 50
+
+
+@pytest.mark.parametrize("window", [64, 300, 2048, 65536])
+def test_speculation_window_does_not_change_results(hip_ctx, codec, oracle_mod, window):
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(2, n_records=8000)
+    spans, _ = fa.discover(s.compressed)
+    batch = codec.inflate_batch_device(s.compressed, spans)
+    u = batch.data().to_vec()
+    hip_ctx.set_bam_window(window)
+    try:
+        records = fa.BamDecoder(hip_ctx).decode(batch, s.records_start)
+        check_batch(oracle_mod, records, u, batch.offsets(), None, s.records_start)
+    finally:
+        hip_ctx.set_bam_window(2048)
+
+
+def test_partial_trailing_record_reports_the_tail(codec, decoder, oracle_mod):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(21)
+    data = bam_header() + b"".join(random_records(rng, 600))
+    for cut in (len(data) - 1, len(data) - 37, len(data) // 2 + 3):
+        buf = bgzf(data[:cut], cap=5000)
+        spans, _ = fa.discover(buf)
+        batch = codec.inflate_batch_device(buf, spans)
+        u = batch.data().to_vec()
+        start, _ = fa.bam_parse_header(u)
+        records = decoder.decode(batch, start)
+        check_batch(oracle_mod, records, u, batch.offsets(), batch.compressed_offsets(), start)
+        assert records.tail() < len(u)
+
+
+def test_record_spanning_members_and_every_small_blocking(codec, decoder, oracle_mod):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(22)
+    recs = random_records(rng, 40, max_len=2000)
+    data = b"".join(recs)
+    for cap in (37, 100, 333, 1000, 4096):
+        buf = bgzf(data, cap=cap, eof=False)
+        spans, _ = fa.discover(buf)
+        batch = codec.inflate_batch_device(buf, spans)
+        u = np.frombuffer(data, np.uint8)
+        records = decoder.decode(batch, 0)
+        check_batch(oracle_mod, records, u, batch.offsets(), batch.compressed_offsets(), 0)
+
+
+def test_malformed_record_position_matches_oracle(codec, decoder, oracle_mod):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(23)
+    recs = random_records(rng, 300)
+    bad = bytearray(recs[170])
+    bad[20:24] = (1 << 22).to_bytes(4, "little")
+    data = b"".join(recs[:170]) + bytes(bad) + b"".join(recs[171:])
+    buf = bgzf(data, cap=7000)
+    spans, _ = fa.discover(buf)
+    batch = codec.inflate_batch_device(buf, spans)
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(np.frombuffer(data, np.uint8), batch.offsets(), 0)
+    assert rc == 1
+    with pytest.raises(fa.Malformed) as ei:
+        decoder.decode(batch, 0)
+    assert ei.value.position == tail and ei.value.format == "bam"
+    tiny = bytearray(recs[5])
+    tiny[0:4] = (20).to_bytes(4, "little")  # block_size below the 32-byte core
+    data = b"".join(recs[:5]) + bytes(tiny) + b"".join(recs[6:])
+    buf = bgzf(data, cap=7000)
+    spans, _ = fa.discover(buf)
+    batch = codec.inflate_batch_device(buf, spans)
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(np.frombuffer(data, np.uint8), batch.offsets(), 0)
+    with pytest.raises(fa.Malformed) as ei:
+        decoder.decode(batch, 0)
+    assert rc == 1 and ei.value.position == tail
+
+
+def test_empty_batch_decodes_to_no_records(codec, decoder):
+    import fritillaria_amd as fa
+    b = codec.inflate_batch_device(b"", np.zeros(0, dtype=fa.SPAN_DTYPE))
+    r = decoder.decode(b, 0)
+    assert len(r) == 0 and r.tail() == 0
+    r = decoder.decode(fa.DeviceInflateBatch(), 17)
+    assert len(r) == 0 and r.tail() == 17
+
+
+def _reader_columns(codec, decoder, data, bpb):
+    import fritillaria_amd as fa
+    reader = fa.DeviceBgzfReader(data, codec, blocks_per_batch=bpb)
+    first = True
+    vo, pos, lseq, n_batches = [], [], [], 0
+    while True:
+        batch = reader.next_batch()
+        if batch is None:
+            break
+        n_batches += 1
+        start = batch.start
+        if first:
+            start, _ = fa.bam_parse_header(batch.data.data().to_vec()[:1 << 20])
+            first = False
+        records = decoder.decode(batch.data, start)
+        cols = records.to_host()
+        vo.append(cols["virtual_offsets"])
+        pos.append(cols["position"])
+        lseq.append(cols["sequence_len"])
+        tail = records.tail()
+        del records
+        reader.carry_from(tail)
+    return np.concatenate(vo), np.concatenate(pos), np.concatenate(lseq), n_batches
+
+
+@pytest.mark.parametrize("bpb", [1, 2, 3, 4, 5, 8, 256])
+def test_reader_loop_matches_whole_file(codec, decoder, oracle_mod, bpb):
+    print(f"[gpu] reader loop blocks_per_batch={bpb}", flush=True)
+    from fritillaria_amd import synth
+    for config, n in ((1, 4000), (3, 12)):
+        s = synth.config_bam(config, n_records=n)
+        d = oracle_mod.discover(s.compressed)
+        r = oracle_mod.inflate_batch(s.compressed, d.spans)
+        u = np.frombuffer(r.data, np.uint8)
+        rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, s.records_start)
+        cols = oracle_mod.bam_decode_columns(u, rec)
+        want_v = oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec)
+        vo, pos, lseq, nb = _reader_columns(codec, decoder, s.compressed, bpb)
+        assert np.array_equal(vo, want_v)
+        assert np.array_equal(pos, cols["position"]) and np.array_equal(lseq, cols["sequence_len"])
+
+
+def test_reader_stream_ending_inside_a_record(codec, decoder):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(30)
+    data = bam_header() + b"".join(random_records(rng, 300))
+    buf = bgzf(data[:-10], cap=9000)
+    reader = fa.DeviceBgzfReader(buf, codec, blocks_per_batch=2)
+    first = True
+    with pytest.raises(fa.Malformed) as ei:
+        while True:
+            b = reader.next_batch()
+            if b is None:
+                break
+            start = b.start
+            if first:
+                start, _ = fa.bam_parse_header(b.data.data().to_vec())
+                first = False
+            rec = decoder.decode(b.data, start)
+            reader.carry_from(rec.tail())
+    assert ei.value.format == "bgzf"
+
+
+def test_reader_truncated_bgzf(codec):
+    import fritillaria_amd as fa
+    buf = bgzf(b"z" * 100000, cap=20000)
+    reader = fa.DeviceBgzfReader(buf[:-5], codec, blocks_per_batch=256)
+    with pytest.raises(fa.Malformed) as ei:
+        reader.next_batch()
+    assert "truncated BGZF" in ei.value.reason
+
+
+def test_carry_past_end_is_rejected(codec):
+    import fritillaria_amd as fa
+    buf = bgzf(b"z" * 1000)
+    reader = fa.DeviceBgzfReader(buf, codec)
+    b = reader.next_batch()
+    with pytest.raises(fa.InvalidDeviceBatch):
+        reader.carry_from(b.data.byte_len() + 1)
+
+
+def test_columns_report_their_device(codec, decoder):
+    import fritillaria_amd as fa
+    buf = golden_bytes("htslib.bam")
+    batch = codec.inflate_batch_device(buf, fa.discover(buf)[0])
+    u = batch.data().to_vec()
+    start, _ = fa.bam_parse_header(u)
+    r = decoder.decode(batch, start)
+    assert r.device_ordinal() == batch.device_ordinal() == 0

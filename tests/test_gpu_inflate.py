@@ -1,0 +1,171 @@
+"""GPU parity of the inflate path (DeviceBlockCodec / BlockCodec) against the
+oracle, through the C ABI. Mirrors crates/fritillaria-cuda/tests/inflate.rs:37-233."""
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_bytes
+from helpers import bgzf_member, corrupt_streams, deflate_raw, random_records
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = json.load(open(os.path.join(GOLDEN, "members.json")))
+
+
+def assert_matches_oracle(codec, oracle_mod, buf):
+    d = oracle_mod.discover(buf)
+    want = oracle_mod.inflate_batch(buf, d.spans)
+    assert want.kind == 0
+    got = codec.inflate_batch_device(buf, d.spans)
+    host = got.to_host()
+    assert host.offsets.tolist() == want.offsets.tolist()
+    assert host.data.tobytes() == want.data
+    return got
+
+
+@pytest.mark.parametrize("name", sorted(MEMBERS))
+def test_fixtures_match_oracle(codec, oracle_mod, name):
+    print(f"[gpu] inflate fixture {name}", flush=True)
+    got = assert_matches_oracle(codec, oracle_mod, golden_bytes(name))
+    assert got.byte_len() == MEMBERS[name]["inflated_len"]
+
+
+def _payload_kinds(rng):
+    return {
+        "records": b"".join(random_records(rng, 250))[:65495],
+        "random": rng.integers(0, 256, 65495, dtype=np.uint8).tobytes(),
+        "runs": b"\0" * 30000 + b"AB" * 10000 + b"ACGTACGTT" * 1000,
+        "lowent": bytes(rng.choice(list(b"ACGTN"), 65000).tolist()),
+        "tiny": b"x",
+        "empty": b"",
+    }
+
+
+@pytest.mark.parametrize("level", list(range(10)))
+def test_every_level_every_payload(codec, oracle_mod, level):
+    print(f"[gpu] inflate level {level}", flush=True)
+    rng = np.random.default_rng(100 + level)
+    buf = b"".join(bgzf_member(p, level=level) for p in _payload_kinds(rng).values())
+    assert_matches_oracle(codec, oracle_mod, buf)
+
+
+@pytest.mark.parametrize("strategy", [zlib.Z_FILTERED, zlib.Z_HUFFMAN_ONLY, zlib.Z_RLE, zlib.Z_FIXED])
+def test_every_strategy(codec, oracle_mod, strategy):
+    print(f"[gpu] inflate strategy {strategy}", flush=True)
+    rng = np.random.default_rng(strategy)
+    members = [bgzf_member(p, cdata=deflate_raw(p, 6, strategy)) for p in _payload_kinds(rng).values()]
+    assert_matches_oracle(codec, oracle_mod, b"".join(members))
+
+
+def test_multi_block_members_and_overlapping_copies(codec, oracle_mod):
+    print("[gpu] inflate multi-block", flush=True)
+    rng = np.random.default_rng(9)
+    data = b"".join(random_records(rng, 200))[:65000]
+    ms = [bgzf_member(data, cdata=deflate_raw(data, 6, flushes=(1, 2, 9000, 33000, 64999)))]
+    # distance 1..64 runs of every length class (overlapping back-references)
+    runs = b"".join(bytes(rng.integers(0, 256, d, dtype=np.uint8)) * (300 // d + 2) for d in range(1, 65))
+    ms.append(bgzf_member(runs[:65000], level=9))
+    assert_matches_oracle(codec, oracle_mod, b"".join(ms))
+
+
+def test_many_members_of_uneven_size(codec, oracle_mod):
+    print("[gpu] inflate many members", flush=True)
+    rng = np.random.default_rng(4)
+    data = b"".join(random_records(rng, 4000))
+    ms = []
+    pos = 0
+    while pos < len(data):
+        n = int(rng.integers(1, 65495))
+        ms.append(bgzf_member(data[pos:pos + n], level=int(rng.integers(0, 10))))
+        pos += n
+    for k in range(0, 997 * 40, 997):  # the reference's payload-997 case
+        ms.append(bgzf_member(data[k:k + 997]))
+    assert_matches_oracle(codec, oracle_mod, b"".join(ms))
+
+
+def test_host_path_matches(codec, oracle_mod):
+    buf = golden_bytes("htslib_multiblock.bam")
+    d = oracle_mod.discover(buf)
+    want = oracle_mod.inflate_batch(buf, d.spans)
+    got = codec.inflate_batch(buf, d.spans)
+    assert got.data.tobytes() == want.data and got.offsets.tolist() == want.offsets.tolist()
+
+
+def test_empty_and_eof_only_batches(codec, oracle_mod):
+    import fritillaria_amd as fa
+    b = codec.inflate_batch_device(b"", np.zeros(0, dtype=fa.SPAN_DTYPE))
+    assert b.byte_len() == 0 and len(b) == 0
+    eof = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+    b = codec.inflate_batch_device(eof, fa.discover(eof)[0])
+    assert b.byte_len() == 0 and b.offsets().tolist() == [0, 0]
+
+
+def test_corrupt_streams_report_the_oracle_status(codec, oracle_mod):
+    import fritillaria_amd as fa
+    for name, payload, cap, want in corrupt_streams():
+        print(f"[gpu] corrupt stream: {name}", flush=True)
+        st, out = oracle_mod.inflate_member(payload, cap)
+        assert st == want
+        member = bgzf_member(b"", cdata=payload, crc=0, isize=cap)
+        with pytest.raises(fa.InflateError) as ei:
+            codec.inflate_batch_device(member, fa.discover(member)[0])
+        assert ei.value.code == want, name
+
+
+def test_tampered_trailers(codec, oracle_mod):
+    import fritillaria_amd as fa
+    good = bgzf_member(b"A" * 5000)
+    bad_crc = bgzf_member(b"B" * 5000, crc=0xDEADBEEF)
+    with pytest.raises(fa.ChecksumMismatch) as ei:
+        buf = good + bad_crc
+        codec.inflate_batch_device(buf, fa.discover(buf)[0])
+    assert ei.value.offset == len(good) and ei.value.expected == 0xDEADBEEF
+    assert ei.value.actual == zlib.crc32(b"B" * 5000)
+    big = bgzf_member(b"C" * 5000, isize=5001)
+    with pytest.raises(fa.SizeMismatch) as ei:
+        codec.inflate_batch_device(big, fa.discover(big)[0])
+    assert ei.value.expected == 5001 and ei.value.actual == 5000
+    small = bgzf_member(b"C" * 5000, isize=4999)
+    with pytest.raises(fa.InflateError) as ei:
+        codec.inflate_batch_device(small, fa.discover(small)[0])
+    assert ei.value.code == 4
+
+
+def test_corrupted_payload_bytes_fail_like_the_oracle(codec, oracle_mod):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(77)
+    data = b"".join(random_records(rng, 200))[:60000]
+    for trial in range(24):
+        cd = bytearray(deflate_raw(data, int(rng.integers(1, 10))))
+        k = int(rng.integers(0, len(cd)))
+        cd[k] ^= 1 << int(rng.integers(0, 8))
+        member = bgzf_member(data, cdata=bytes(cd))
+        spans = fa.discover(member)[0]
+        want = oracle_mod.inflate_batch(member, spans)
+        print(f"[gpu] corrupted payload trial {trial}: oracle kind {want.kind}", flush=True)
+        if want.kind == 0:
+            got = codec.inflate_batch_device(member, spans).to_host()
+            assert got.data.tobytes() == want.data
+            continue
+        with pytest.raises(fa.FrError) as ei:
+            codec.inflate_batch_device(member, spans)
+        assert ei.value.kind == want.kind
+        if want.kind == 4:
+            assert ei.value.code == want.code
+        else:
+            assert (ei.value.expected, ei.value.actual) == (want.expected, want.actual)
+
+
+def test_crc32_blocks(codec):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(6)
+    data = rng.integers(0, 256, 300000, dtype=np.uint8).tobytes()
+    buf = bgzf_member(data[:65000])
+    b = codec.inflate_batch_device(buf, fa.discover(buf)[0])
+    offs = [0, 1, 100, 64000, 0]
+    lens = [65000, 0, 7, 1000, 64]
+    got = codec.crc32_blocks(b.data(), offs, lens)
+    assert got.tolist() == [zlib.crc32(data[o:o + n]) for o, n in zip(offs, lens)]
