@@ -8,5 +8,5 @@ from .hip import (  # noqa: F401
     BAM_COLUMNS, Backend, BamDecoder, ChecksumMismatch, DeviceBatch, DeviceBgzfReader, DeviceBuffer,
     DeviceInflateBatch, DeviceRecordBatch, FrError, HipCodec, HipContext, HipError, HipUnavailable,
     InflateBatch, InflateError, InvalidBlock, InvalidDeviceBatch, Malformed, SizeMismatch, SPAN_DTYPE,
-    bam_parse_header, crc32_combine, device_is_available, discover, select_codec,
+    BatchedPipeline, bam_parse_header, crc32_combine, device_is_available, discover, select_codec,
 )

@@ -208,9 +208,13 @@ extern "C" int frhip_ctx_create(int ordinal, void *stream, frhip_ctx **out, frhi
     return FRHIP_OK;
 }
 
-extern "C" void frhip_ctx_destroy(frhip_ctx *c)
+namespace frhip {
+
+void ctx_retain(frhip_ctx *c) { __atomic_add_fetch(&c->refs, 1, __ATOMIC_ACQ_REL); }
+
+void ctx_release(frhip_ctx *c)
 {
-    if (!c) return;
+    if (!c || __atomic_sub_fetch(&c->refs, 1, __ATOMIC_ACQ_REL) != 0) return;
     (void)hipSetDevice(c->ordinal);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->in_scratch.release();
@@ -223,8 +227,34 @@ extern "C" void frhip_ctx_destroy(frhip_ctx *c)
     delete c;
 }
 
+} // namespace frhip
+
+extern "C" void frhip_ctx_destroy(frhip_ctx *c) { ctx_release(c); }
+
 extern "C" int frhip_ctx_ordinal(const frhip_ctx *c) { return c->ordinal; }
 extern "C" void *frhip_ctx_stream(const frhip_ctx *c) { return (void *)c->stream; }
+extern "C" int frhip_ctx_synchronize(frhip_ctx *c, frhip_error *err)
+{
+    (void)hipSetDevice(c->ordinal);
+    hipError_t r = hipDeviceSynchronize();
+    return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "hipDeviceSynchronize");
+}
+extern "C" int frhip_ctx_set_inflate_schedule(frhip_ctx *c, int schedule)
+{
+    if (schedule < 0 || schedule >= N_INFLATE_SCHEDULES) return FRHIP_E_INVALID_DEVICE_BATCH;
+    c->inflate_schedule = schedule;
+    return FRHIP_OK;
+}
+extern "C" int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, int cap)
+{
+    int k = cap < N_INFLATE_SCHEDULES ? cap : N_INFLATE_SCHEDULES;
+    for (int i = 0; i < k; ++i) {
+        if (ring) ring[i] = INFLATE_SCHEDULES[i].ring;
+        if (waves_per_group) waves_per_group[i] = INFLATE_SCHEDULES[i].waves_per_group;
+        if (waves_per_cu) waves_per_cu[i] = inflate_waves_per_cu(i);
+    }
+    return N_INFLATE_SCHEDULES;
+}
 extern "C" int frhip_ctx_set_bam_window(frhip_ctx *c, uint32_t window)
 {
     c->bam_window = window < 1 ? 1 : window;
@@ -437,7 +467,7 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     a.queue = nullptr;
     a.order = nullptr;
     a.x2n = ctx->x2n;
-    r = launch_inflate(a, (uint32_t)n, ctx->stream);
+    r = launch_inflate(a, (uint32_t)n, ctx->stream, ctx->inflate_schedule);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
     if (t) (void)hipEventRecord(e2, ctx->stream);
     uint32_t *h_st = (uint32_t *)ctx->pinned.get(n * 12 > meta_bytes ? n * 12 : meta_bytes, err, &rc);

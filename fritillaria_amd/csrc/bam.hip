@@ -63,6 +63,36 @@ __device__ __forceinline__ Step step(const uint8_t *buf, uint64_t len, uint64_t 
     return {end, 0};
 }
 
+// Candidate filter for the speculation only (never for the true chain): a
+// record that cannot be whole inside the buffer is dropped rather than treated
+// as a tail, and the fixed fields must look like a record -- IDs and positions
+// at least -1, a name of at least one printable SAM QNAME character followed by
+// its NUL. A true record that fails this only costs a fallback walk.
+__device__ __forceinline__ Step cand_step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until)
+{
+    if (cur >= until) return {cur, 1};
+    if (cur + 36 > len) return {cur, 3};
+    uint32_t bs = ld32(buf + cur);
+    if (bs < 32) return {cur, 3};
+    uint64_t end = cur + 4 + (uint64_t)bs;
+    if (end > len) return {cur, 3};
+    uint32_t name_len = buf[cur + 12];
+    uint64_t n_cigar = ld16(buf + cur + 16);
+    uint64_t l_seq = ld32(buf + cur + 20);
+    uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
+    if (name_len == 0 || cur + need > end) return {cur, 3};
+    if ((int32_t)ld32(buf + cur + 4) < -1 || (int32_t)ld32(buf + cur + 8) < -1 ||
+        (int32_t)ld32(buf + cur + 24) < -1 || (int32_t)ld32(buf + cur + 28) < -1)
+        return {cur, 3};
+    // a QNAME is at least one printable character, then the NUL
+    if (name_len < 2 || buf[cur + 35 + name_len] != 0) return {cur, 3};
+    for (uint32_t i = 0; i + 1 < name_len; ++i) {
+        uint8_t c = buf[cur + 36 + i];
+        if (c < 33 || c > 126) return {cur, 3};
+    }
+    return {end, 0};
+}
+
 struct Walk {
     uint64_t land;
     uint32_t count;
@@ -113,9 +143,8 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
         if (p < T) {
             alive = 1;
             for (;;) {
-                Step s = step(a.buf, a.len, cur, T);
+                Step s = cand_step(a.buf, a.len, cur, T);
                 if (s.kind == 1) break;
-                if (s.kind == 2) { complete = 0; break; }
                 if (s.kind == 3) { alive = 0; break; }
                 ++cnt;
                 cur = s.next;
@@ -198,7 +227,8 @@ __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t j)
 __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
 {
     const uint32_t lane = threadIdx.x;
-    uint64_t cur = a.start, emitted = 0, tail = a.len, fallbacks = 0;
+    if (a.len_dev) a.len = *a.len_dev;
+    uint64_t cur = a.start_dev ? *a.start_dev : a.start, emitted = 0, tail = a.len, fallbacks = 0;
     int stopped = 0;
     uint64_t status = BAM_OK;
     for (uint32_t base = 0; base < a.nb; base += 64) {
@@ -234,11 +264,27 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
             }
             uint32_t cnt, comp_j, valid = 1, mode;
             uint64_t land;
+            uint32_t found = 0, fcnt = 0;
             if (rl32(match, j) && rl64(expect, j) == cur) {
-                cnt = rl32(mcnt, j);
+                found = rl32(match, j);
+                fcnt = rl32(mcnt, j);
+            } else if (rl32(ok, j)) {
+                // the chain did not arrive where the previous member's
+                // speculation said; look for it among this member's survivors
+                const uint32_t bj = base + j;
+                const uint32_t ns = a.n_surv[bj];
+                for (uint32_t k = 0; k < ns && !found; ++k) {
+                    if (a.surv_pos[(uint64_t)bj * BAM_K + k] == cur) {
+                        found = k + 1;
+                        fcnt = a.surv_cnt[(uint64_t)bj * BAM_K + k] + rl32(ctail, j);
+                    }
+                }
+            }
+            if (found) {
+                cnt = fcnt;
                 land = rl64(X, j);
                 comp_j = rl32(comp, j);
-                mode = rl32(match, j);
+                mode = found;
             } else {
                 Walk w = walk_uniform(a.buf, a.len, cur, js1);
                 cnt = w.count;

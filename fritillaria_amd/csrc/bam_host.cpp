@@ -17,6 +17,7 @@
 #include "../../include/frhip.h"
 #include "frhip_internal.h"
 #include "host_internal.h"
+#include "bam_layout.h"
 
 using namespace frhip;
 
@@ -30,39 +31,6 @@ struct frhip_bam_batch {
 };
 
 namespace {
-
-struct BamScratchLayout {
-    uint64_t bytes;
-    uint64_t o_bs, o_coff, o_ok, o_ns, o_spos, o_scnt, o_land, o_exit, o_ctail, o_comp, o_rel, o_entry, o_count,
-        o_first, o_mode, o_totals;
-};
-
-uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
-
-BamScratchLayout layout(uint64_t nb)
-{
-    BamScratchLayout L{};
-    uint64_t p = 0;
-    auto take = [&](uint64_t bytes) { uint64_t o = p; p = align256(p + bytes); return o; };
-    L.o_bs = take((nb + 1) * 8);
-    L.o_coff = take(nb * 8);
-    L.o_ok = take(nb);
-    L.o_ns = take(nb);
-    L.o_spos = take(nb * BAM_K * 8);
-    L.o_scnt = take(nb * BAM_K * 4);
-    L.o_land = take(nb * 8);
-    L.o_exit = take(nb * 8);
-    L.o_ctail = take(nb * 4);
-    L.o_comp = take(nb);
-    L.o_rel = take(nb * (uint64_t)BAM_CAPB * 2);
-    L.o_entry = take(nb * 8);
-    L.o_count = take(nb * 4);
-    L.o_first = take(nb * 8);
-    L.o_mode = take(nb);
-    L.o_totals = take(4 * 8);
-    L.bytes = p;
-    return L;
-}
 
 int empty_batch(frhip_ctx *ctx, uint64_t start, bool with_voff, frhip_bam_batch **out, frhip_error *err)
 {
@@ -100,7 +68,7 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
             return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "member %llu has an invalid extent",
                            (unsigned long long)i);
     (void)hipSetDevice(ctx->ordinal);
-    BamScratchLayout L = layout(nb);
+    BamScratchLayout L = bam_layout(nb);
     int rc;
     uint8_t *s = (uint8_t *)ctx->bam_scratch.get(L.bytes, err, &rc);
     if (!s) return rc;
@@ -296,6 +264,7 @@ extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t l
     frhip_reader *r = new (std::nothrow) frhip_reader;
     if (!r) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
     r->ctx = ctx;
+    ctx_retain(ctx);
     r->src = data;
     r->src_len = len;
     r->blocks_per_batch = blocks_per_batch ? blocks_per_batch : 256;
@@ -303,7 +272,12 @@ extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t l
     return FRHIP_OK;
 }
 
-extern "C" void frhip_reader_free(frhip_reader *r) { delete r; }
+extern "C" void frhip_reader_free(frhip_reader *r)
+{
+    if (!r) return;
+    ctx_release(r->ctx);
+    delete r;
+}
 
 extern "C" int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_error *err)
 {

@@ -23,7 +23,19 @@ struct InflateArgs {
     const uint32_t *x2n;     // x^(2^k) mod P, k = 0..31
 };
 
-hipError_t launch_inflate(const InflateArgs &a, uint32_t grid, hipStream_t stream);
+// Inflate kernel variants: output ring bytes per member and waves per workgroup.
+struct InflateSchedule {
+    uint32_t ring;
+    uint32_t waves_per_group;
+};
+extern const InflateSchedule INFLATE_SCHEDULES[];
+extern const int N_INFLATE_SCHEDULES;
+constexpr int DEFAULT_INFLATE_SCHEDULE = 5;
+
+// Launches `waves` waves (one member each at a time) of schedule `schedule`.
+hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stream, int schedule);
+uint32_t inflate_lds_bytes(int schedule);
+uint32_t inflate_waves_per_cu(int schedule);
 
 // Record-boundary discovery for BAM over an inflated batch.
 constexpr uint32_t BAM_K = 8;                 // survivors kept per block
@@ -54,6 +66,8 @@ struct BamReconcileArgs {
     const uint64_t *block_starts;
     uint32_t nb;
     uint64_t start;
+    const uint64_t *start_dev; // device-side start (batched form), or null
+    const uint64_t *len_dev;   // device-side buffer length (batched form), or null
     const uint8_t *spec_ok;
     const uint8_t *n_surv;
     const uint64_t *surv_pos;

@@ -47,18 +47,24 @@ struct frhip_buf {
 };
 
 struct frhip_ctx {
+    // Pipelines and readers keep a reference; the context is freed when the
+    // last holder releases it (frhip_ctx_destroy releases the creator's).
+    int refs = 1;
     int ordinal = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     uint32_t *x2n = nullptr;
     uint32_t bam_window = 2048;
+    int inflate_schedule = 5; // index into INFLATE_SCHEDULES
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch;
     frhip::PinnedScratch pinned;
 };
 
 namespace frhip {
 frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err);
+void ctx_retain(frhip_ctx *ctx);
+void ctx_release(frhip_ctx *ctx);
 void buf_free(frhip_buf *b);
 } // namespace frhip
 

@@ -9,18 +9,20 @@
 //    state, so it lives in SGPRs and runs on the scalar unit, while the vector
 //    lanes do the parallel parts -- table construction (ballot ranks, LUT fill
 //    by a canonical walk per entry), LZ77 copies (one byte per lane per step),
-//    the HBM write-back and the CRC.
-//  * Output is staged in a 32 KiB LDS ring (DEFLATE's window is 32 KiB), so
-//    back-references never touch HBM; it is flushed to HBM every ~8 KB.
-//  * CRC is folded in during each flush: lane l takes a 132-byte segment
-//    (stride 33 dwords: conflict-free), and the segments are combined with a
-//    per-lane GF(2) factor x^(8*132*(63-l)) and a wave XOR reduction.
-//  * Compressed input is staged through a 2 KiB LDS window fed by coalesced
-//    16-byte loads, so the scalar refill never waits on HBM per symbol.
-//  * Persistent form: a grid of resident waves pulls member indices from a
-//    device work-queue counter (optionally through a size-sorted order).
-//
-// LDS per wave: 39.8 KB -> 4 waves per CU.
+//    the HBM write-back and the CRC. Length and distance bases are computed,
+//    not looked up, so the symbol loop touches only LDS.
+//  * Output is staged in an LDS ring holding the last RING bytes of the member
+//    (RING = 4..32 KiB by schedule). Ring slots follow the absolute output
+//    offset, so write-back moves aligned 16-byte words to HBM. A back-reference
+//    that reaches past the ring is read back from HBM, where this wave already
+//    wrote it (after a workgroup-scope fence).
+//  * CRC is folded in during each write-back: lane l takes a SEG-byte segment
+//    and the segments are combined with a per-lane GF(2) factor
+//    x^(8*SEG*(63-l)) and a wave XOR reduction.
+//  * Compressed input is staged through a 1 KiB LDS window fed by coalesced
+//    8-byte loads, so the scalar refill never waits on HBM per symbol.
+//  * Persistent form: resident waves pull member indices from a device
+//    work-queue counter (optionally through a size-sorted order).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -31,10 +33,7 @@
 namespace frhip {
 namespace {
 
-constexpr uint32_t RING = 32768, RMASK = RING - 1;
-constexpr uint32_t STAGE_BYTES = 2048, STAGE_WORDS = STAGE_BYTES / 4, CHUNK = 1024;
-constexpr uint32_t FLUSH_AT = 8000;
-constexpr uint32_t SEG = 132; // 64 * 132 = 8448 >= FLUSH_AT + 258
+constexpr uint32_t STAGE_BYTES = 1024, STAGE_WORDS = STAGE_BYTES / 4, CHUNK = 512;
 constexpr int LBITS = 10, DBITS = 8, CBITS = 7;
 
 enum : uint32_t {
@@ -50,18 +49,39 @@ enum : uint32_t {
     ST_BAD_COUNTS = 9,
 };
 
-__constant__ uint16_t LEN_BASE[29] = {3,  4,  5,  6,  7,  8,  9,  10, 11,  13,  15,  17,  19,  23, 27,
-                                      31, 35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
-__constant__ uint8_t LEN_EXTRA[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2,
-                                      2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
-__constant__ uint16_t DIST_BASE[30] = {1,    2,    3,    4,    5,    7,     9,     13,    17,    25,
-                                       33,   49,   65,   97,   129,  193,   257,   385,   513,   769,
-                                       1025, 1537, 2049, 3073, 4097, 6145,  8193,  12289, 16385, 24577};
-__constant__ uint8_t DIST_EXTRA[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6,
-                                       6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
-__constant__ uint8_t CLEN_ORDER[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+// RFC 1951 3.2.5 in closed form. Length code c = sym - 257 (0..28): codes 0-7
+// carry no extra bits (lengths 3-10), then each group of four doubles the step;
+// code 28 is length 258. Distance code d (0..29): codes 0-3 are distances 1-4,
+// then each pair doubles the step.
+__device__ __forceinline__ uint32_t len_extra(uint32_t c) { return c < 8 || c == 28 ? 0 : (c - 4) >> 2; }
+__device__ __forceinline__ uint32_t len_base(uint32_t c)
+{
+    if (c < 8) return c + 3;
+    if (c == 28) return 258;
+    uint32_t e = (c - 4) >> 2;
+    return ((4u | (c & 3)) << e) + 3;
+}
+__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 4 ? 0 : (d - 2) >> 1; }
+__device__ __forceinline__ uint32_t dist_base(uint32_t d)
+{
+    if (d < 4) return d + 1;
+    uint32_t e = (d - 2) >> 1;
+    return ((2u | (d & 1)) << e) + 1;
+}
+// Order of the code-length code lengths (RFC 1951 3.2.7), 5 bits per entry.
+__device__ __forceinline__ uint32_t clen_order(uint32_t i)
+{
+    const uint64_t lo = (16ull << 0) | (17ull << 5) | (18ull << 10) | (0ull << 15) | (8ull << 20) | (7ull << 25) |
+                        (9ull << 30) | (6ull << 35) | (10ull << 40) | (5ull << 45) | (11ull << 50) | (4ull << 55);
+    const uint64_t hi = (12ull << 0) | (3ull << 5) | (13ull << 10) | (2ull << 15) | (14ull << 20) | (1ull << 25) |
+                        (15ull << 30);
+    return i < 12 ? (uint32_t)(lo >> (5 * i)) & 31 : (uint32_t)(hi >> (5 * (i - 12))) & 31;
+}
 
-struct Smem {
+// Per-wave LDS: the output ring, the staged compressed input and the Huffman
+// tables of the current block.
+template <uint32_t RING>
+struct alignas(16) WaveMem {
     uint8_t ring[RING];
     uint32_t stage[STAGE_WORDS];
     uint16_t llut[1 << LBITS];
@@ -74,67 +94,62 @@ struct Smem {
     uint16_t dcount[16];
     uint16_t ccount[16];
     uint8_t lens[320];
-    uint32_t crctab[256];
 };
 
 #define WFENCE() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront")
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint64_t uni64(uint64_t v)
-{
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
-// Wave-uniform LSB-first bit reader over one member's payload.
+// Wave-uniform LSB-first bit reader over one member's payload. Offsets are
+// relative to `sbase`, the payload start rounded down to CHUNK.
 struct Bits {
     uint64_t buf;
     uint32_t cnt;       // valid bits in buf (never counts bytes past the payload)
-    uint32_t pos;       // payload bytes moved into buf
+    uint32_t pos;       // payload bytes moved into buf (relative to the payload start)
     uint32_t len;       // payload length
-    uint64_t base;      // absolute offset of the payload in the batch buffer
-    uint64_t stage_end; // absolute end of the staged window (multiple of CHUNK)
+    uint32_t lead;      // payload start - sbase
+    uint32_t stage_end; // end of the staged window, relative to sbase (multiple of CHUNK)
+    const uint8_t *sbase;
 };
 
-__device__ __forceinline__ void load_chunk(Smem &sm, const uint8_t *gin, uint64_t chunk_abs, uint32_t lane)
+__device__ __forceinline__ void load_chunk(uint32_t *stage, const uint8_t *sbase, uint32_t rel, uint32_t lane)
 {
-    const uint4 v = *reinterpret_cast<const uint4 *>(gin + chunk_abs + 16u * lane);
-    reinterpret_cast<uint4 *>(sm.stage)[((chunk_abs & (STAGE_BYTES - 1)) >> 4) + lane] = v;
+    const uint2 v = *reinterpret_cast<const uint2 *>(sbase + rel + 8u * lane);
+    reinterpret_cast<uint2 *>(stage)[((rel & (STAGE_BYTES - 1)) >> 3) + lane] = v;
 }
 
-// Makes bytes [abs & ~3, (abs & ~3) + 12) resident in the stage.
-__device__ __forceinline__ void ensure_stage(Smem &sm, Bits &b, const uint8_t *gin, uint64_t abs, uint32_t lane)
+// Makes bytes [r & ~3, (r & ~3) + 12) resident in the stage (r relative to sbase).
+__device__ __forceinline__ void ensure_stage(uint32_t *stage, Bits &b, uint32_t r, uint32_t lane)
 {
-    uint64_t need_lo = abs & ~3ull, need_hi = need_lo + 12;
-    if (need_hi <= b.stage_end && need_lo + STAGE_BYTES >= b.stage_end) return;
-    if (need_lo + CHUNK >= b.stage_end && need_hi <= b.stage_end + CHUNK && b.stage_end != 0) {
-        load_chunk(sm, gin, b.stage_end, lane);
+    uint32_t lo = r & ~3u, hi = lo + 12;
+    if (hi <= b.stage_end && lo + STAGE_BYTES >= b.stage_end) return;
+    if (b.stage_end != 0 && lo + CHUNK >= b.stage_end && hi <= b.stage_end + CHUNK) {
+        load_chunk(stage, b.sbase, b.stage_end, lane);
         b.stage_end += CHUNK;
     } else {
-        uint64_t c0 = abs & ~(uint64_t)(CHUNK - 1);
-        load_chunk(sm, gin, c0, lane);
-        load_chunk(sm, gin, c0 + CHUNK, lane);
+        uint32_t c0 = r & ~(CHUNK - 1);
+        load_chunk(stage, b.sbase, c0, lane);
+        load_chunk(stage, b.sbase, c0 + CHUNK, lane);
         b.stage_end = c0 + 2 * CHUNK;
     }
     WFENCE();
 }
 
-__device__ __forceinline__ void refill(Smem &sm, Bits &b, const uint8_t *gin, uint32_t lane)
+// Tops the bit buffer up to at least 56 bits (while payload remains).
+__device__ __forceinline__ void refill(uint32_t *stage, Bits &b, uint32_t lane)
 {
-    if (b.cnt > 56) return;
     uint32_t avail = b.len - b.pos;
     if (avail == 0) return;
     uint32_t nbytes = (64u - b.cnt) >> 3;
     if (nbytes > avail) nbytes = avail;
-    uint64_t abs = b.base + b.pos;
-    ensure_stage(sm, b, gin, abs, lane);
-    uint32_t w = (uint32_t)(abs >> 2) & (STAGE_WORDS - 1);
-    uint32_t w0 = uni(sm.stage[w]);
-    uint32_t w1 = uni(sm.stage[(w + 1) & (STAGE_WORDS - 1)]);
-    uint32_t w2 = uni(sm.stage[(w + 2) & (STAGE_WORDS - 1)]);
+    uint32_t r = b.lead + b.pos;
+    ensure_stage(stage, b, r, lane);
+    uint32_t w = (r >> 2) & (STAGE_WORDS - 1);
+    uint32_t w0 = uni(stage[w]);
+    uint32_t w1 = uni(stage[(w + 1) & (STAGE_WORDS - 1)]);
+    uint32_t w2 = uni(stage[(w + 2) & (STAGE_WORDS - 1)]);
     uint64_t lo = ((uint64_t)w1 << 32) | w0;
-    uint32_t sh = (uint32_t)(abs & 3) * 8;
+    uint32_t sh = (r & 3) * 8;
     uint64_t v = sh ? ((lo >> sh) | ((uint64_t)w2 << (64 - sh))) : lo;
     if (nbytes < 8) v &= (1ull << (8 * nbytes)) - 1;
     b.buf |= v << b.cnt;
@@ -154,7 +169,7 @@ __device__ __forceinline__ uint32_t getbits(Bits &b, uint32_t n, uint32_t &err)
     return v;
 }
 
-// One symbol through a LUT of `lbits`, falling back to the canonical walk for
+// One symbol through a LUT of LB bits, falling back to the canonical walk for
 // codes longer than the LUT and for invalid codes (puff semantics: an invalid
 // code is reported after 15 bits, a stream that ends first is truncated).
 template <int LB>
@@ -193,27 +208,27 @@ __device__ __forceinline__ int decode_sym(Bits &b, const uint16_t *lut, const ui
     return -1;
 }
 
-// Canonical code construction from lengths (puff's construct, lane-parallel):
-// counts by ballot, symbol order by ballot ranks. Returns 0 complete, >0
-// incomplete, <0 over-subscribed; *count0 = symbols of length 0.
+// Canonical code construction from lengths (puff's construct), lane-parallel.
+// Lane l keeps the count of length l in a VGPR; symbols are placed by ballot
+// ranks, one distinct length at a time. Returns 0 complete, >0 incomplete, <0
+// over-subscribed; *count0 = symbols of length 0.
 __device__ int construct(const uint8_t *lens, int n, uint16_t *count_out, uint16_t *sym_out, uint32_t lane,
                          int *count0)
 {
-    int cnt[16];
-#pragma unroll
-    for (int l = 0; l < 16; ++l) cnt[l] = 0;
+    uint32_t cntv = 0; // lane l: symbols of length l
     for (int r = 0; r < n; r += 64) {
         int s = r + (int)lane;
-        int L = s < n ? (int)lens[s] : -1;
+        int L = s < n ? (int)lens[s] : 16;
 #pragma unroll
-        for (int l = 0; l < 16; ++l) cnt[l] += __popcll(__ballot(L == l));
+        for (int l = 0; l < 16; ++l) {
+            uint32_t c = (uint32_t)__popcll(__ballot(L == l));
+            if ((int)lane == l) cntv += c;
+        }
     }
-    if (lane == 0) {
-#pragma unroll
-        for (int l = 0; l < 16; ++l) count_out[l] = (uint16_t)cnt[l];
-    }
-    *count0 = cnt[0];
-    if (cnt[0] == n) {
+    if (lane < 16) count_out[lane] = (uint16_t)cntv;
+    const int c0 = (int)__builtin_amdgcn_readlane(cntv, 0);
+    *count0 = c0;
+    if (c0 == n) {
         WFENCE();
         return 0;
     }
@@ -221,25 +236,37 @@ __device__ int construct(const uint8_t *lens, int n, uint16_t *count_out, uint16
 #pragma unroll
     for (int l = 1; l < 16; ++l) {
         left <<= 1;
-        left -= cnt[l];
+        left -= (int)__builtin_amdgcn_readlane(cntv, l);
         if (left < 0) {
             WFENCE();
             return left;
         }
     }
-    int offs[16];
-    offs[1] = 0;
+    // lane l: first index of length l in the sorted symbol table
+    uint32_t offv;
+    {
+        uint32_t x = lane >= 1 && lane < 16 ? cntv : 0;
 #pragma unroll
-    for (int l = 1; l < 15; ++l) offs[l + 1] = offs[l] + cnt[l];
+        for (int d = 1; d < 16; d <<= 1) {
+            uint32_t y = __shfl_up(x, d, 64);
+            if ((int)lane >= d) x += y;
+        }
+        offv = __shfl_up(x, 1, 64);
+        if (lane <= 1) offv = 0;
+    }
     const uint64_t lt = (1ull << lane) - 1;
     for (int r = 0; r < n; r += 64) {
         int s = r + (int)lane;
         int L = s < n ? (int)lens[s] : 0;
-#pragma unroll
-        for (int l = 1; l < 16; ++l) {
-            uint64_t m = __ballot(L == l);
-            if (L == l) sym_out[offs[l] + __popcll(m & lt)] = (uint16_t)s;
-            offs[l] += __popcll(m);
+        uint64_t todo = __ballot(L != 0);
+        while (todo) {
+            uint32_t lead = (uint32_t)__ffsll((unsigned long long)todo) - 1;
+            uint32_t Ll = __builtin_amdgcn_readlane((uint32_t)L, lead);
+            uint64_t m = __ballot(L == (int)Ll);
+            uint32_t base = __builtin_amdgcn_readlane(offv, Ll);
+            if (L == (int)Ll) sym_out[base + __popcll(m & lt)] = (uint16_t)s;
+            if (lane == Ll) offv += (uint32_t)__popcll(m);
+            todo &= ~m;
         }
     }
     WFENCE();
@@ -250,23 +277,22 @@ __device__ int construct(const uint8_t *lens, int n, uint16_t *count_out, uint16
 template <int LB>
 __device__ void build_lut(uint16_t *lut, const uint16_t *count, const uint16_t *sym, uint32_t lane)
 {
-    int cnt[LB + 1];
-#pragma unroll
-    for (int l = 1; l <= LB; ++l) cnt[l] = (int)uni(count[l]);
+    uint32_t cv = lane < 16 ? count[lane] : 0;
     for (int e = (int)lane; e < (1 << LB); e += 64) {
         int code = 0, first = 0, index = 0;
         uint16_t out = 0;
         bool found = false;
 #pragma unroll
         for (int len = 1; len <= LB; ++len) {
+            int c = (int)__builtin_amdgcn_readlane(cv, len);
             if (!found) {
                 code |= (e >> (len - 1)) & 1;
-                if (code - first < cnt[len]) {
+                if (code - first < c) {
                     out = (uint16_t)((sym[index + code - first] << 4) | len);
                     found = true;
                 } else {
-                    index += cnt[len];
-                    first = (first + cnt[len]) << 1;
+                    index += c;
+                    first = (first + c) << 1;
                     code <<= 1;
                 }
             }
@@ -280,28 +306,60 @@ struct Out {
     uint8_t *g;         // member's output in HBM
     uint32_t pos;       // bytes produced
     uint32_t flushed;   // bytes written back to HBM (and folded into the CRC)
+    uint32_t visible;   // bytes whose HBM stores are known complete
     uint32_t cap;       // ISIZE
     uint32_t raw;       // zero-init CRC register of bytes [0, flushed)
+    uint32_t skew;      // ring slot of byte q is (skew + q) & RMASK; skew = output address mod 16 KiB
 };
 
-// Writes ring bytes [flushed, pos) to HBM and folds them into the CRC.
-__device__ void flush(Smem &sm, Out &o, uint32_t lane, uint32_t lane_factor, const uint32_t *x2n)
+template <uint32_t RING>
+struct Geometry {
+    static constexpr uint32_t RMASK = RING - 1;
+    static constexpr uint32_t FLUSH_AT = RING / 2;       // unflushed bytes that trigger a write-back
+    static constexpr uint32_t CHUNK_MAX = FLUSH_AT + 258; // most bytes one write-back can carry
+    static constexpr uint32_t SEG = (CHUNK_MAX + 63) / 64; // CRC segment per lane
+};
+
+// Writes ring bytes [flushed, pos) to HBM and folds them into the CRC: lane l
+// takes the l-th of 64 SEG-byte segments counted back from the end (leading
+// zeros do not change a zero-initialised CRC register), then the segments are
+// combined with x^(8*SEG*(63-l)) and an XOR reduction. The HBM write uses
+// 16-byte stores for every aligned 16-byte word inside the range and bytes at
+// its edges (neighbouring members own the rest of those words).
+template <uint32_t RING>
+__device__ void flush(const uint8_t *ring, const uint32_t *crctab, Out &o, uint32_t lane, uint32_t lane_factor,
+                      const uint32_t *x2n)
 {
-    uint32_t c = o.pos - o.flushed;
+    using G = Geometry<RING>;
+    const uint32_t c = o.pos - o.flushed;
     if (c == 0) return;
     WFENCE();
-    for (uint32_t i = lane; i < c; i += 64) o.g[o.flushed + i] = sm.ring[(o.flushed + i) & RMASK];
-    // segment of lane l: [c - SEG*(64-l), c - SEG*(63-l)) clipped at 0
-    int32_t hi = (int32_t)c - (int32_t)(SEG * (63 - lane));
-    int32_t lo = hi - (int32_t)SEG;
+    uint8_t *dst = o.g + o.flushed;
+    const uint32_t s0 = o.skew + o.flushed; // slot (unmasked) of the first byte
+    const uint32_t head = (16u - (uint32_t)((uintptr_t)dst & 15)) & 15;
+    if (c < head + 16) {
+        for (uint32_t i = lane; i < c; i += 64) dst[i] = ring[(s0 + i) & G::RMASK];
+    } else {
+        const uint32_t words = (c - head) >> 4;
+        const uint32_t tail0 = head + (words << 4);
+        if (lane < head) dst[lane] = ring[(s0 + lane) & G::RMASK];
+        for (uint32_t w = lane; w < words; w += 64) {
+            const uint32_t off = head + (w << 4);
+            // an aligned output word sits at a 16-byte aligned ring slot
+            const uint4 v = *reinterpret_cast<const uint4 *>(ring + ((s0 + off) & G::RMASK));
+            *reinterpret_cast<uint4 *>(dst + off) = v;
+        }
+        for (uint32_t i = tail0 + lane; i < c; i += 64) dst[i] = ring[(s0 + i) & G::RMASK];
+    }
+    int32_t hi = (int32_t)c - (int32_t)(G::SEG * (63 - lane));
+    int32_t lo = hi - (int32_t)G::SEG;
     if (lo < 0) lo = 0;
     uint32_t r = 0;
     for (int32_t k = lo; k < hi; ++k) {
-        uint32_t byte = sm.ring[(o.flushed + (uint32_t)k) & RMASK];
-        r = sm.crctab[(r ^ byte) & 0xFF] ^ (r >> 8);
+        uint32_t byte = ring[(s0 + (uint32_t)k) & G::RMASK];
+        r = crctab[(r ^ byte) & 0xFF] ^ (r >> 8);
     }
     r = fr_mulmod(r, lane_factor);
-    // XOR reduction across the wave
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) r ^= __shfl_xor(r, off, 64);
     uint32_t chunk = uni(r);
@@ -309,28 +367,33 @@ __device__ void flush(Smem &sm, Out &o, uint32_t lane, uint32_t lane_factor, con
     o.flushed = o.pos;
 }
 
-__device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, uint32_t in_len, uint8_t *gout,
-                               uint32_t cap, uint32_t lane, uint32_t lane_factor, const uint32_t *x2n,
-                               uint32_t &status_out, uint32_t &produced_out, uint32_t &crc_out)
+template <uint32_t RING>
+__device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const uint8_t *gin, uint64_t in_off,
+                               uint32_t in_len, uint8_t *gout, uint32_t cap, uint32_t lane, uint32_t lane_factor,
+                               const uint32_t *x2n, uint32_t &status_out, uint32_t &produced_out, uint32_t &crc_out)
 {
+    using G = Geometry<RING>;
     Bits b;
     b.buf = 0;
     b.cnt = 0;
     b.pos = 0;
     b.len = in_len;
-    b.base = in_off;
+    b.sbase = gin + (in_off & ~(uint64_t)(CHUNK - 1));
+    b.lead = (uint32_t)(in_off & (CHUNK - 1));
     b.stage_end = 0;
     Out o;
     o.g = gout;
     o.pos = 0;
     o.flushed = 0;
+    o.visible = 0;
     o.cap = cap;
     o.raw = 0;
+    o.skew = (uint32_t)((uintptr_t)gout & 16383);
     uint32_t err = ST_OK;
     int tables = 0; // 0 none, 1 fixed, 2 dynamic
     uint32_t last;
     do {
-        refill(sm, b, gin, lane);
+        if (b.cnt < 32) refill(sm.stage, b, lane);
         last = getbits(b, 1, err);
         uint32_t type = getbits(b, 2, err);
         if (err) break;
@@ -343,7 +406,7 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
                 err = ST_TRUNCATED;
                 break;
             }
-            const uint8_t *p = gin + b.base + b.pos;
+            const uint8_t *p = b.sbase + b.lead + b.pos;
             uint32_t len = uni((uint32_t)p[0] | ((uint32_t)p[1] << 8));
             uint32_t nlen = uni((uint32_t)p[2] | ((uint32_t)p[3] << 8));
             b.pos += 4;
@@ -359,11 +422,12 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
                 err = ST_OUTPUT_OVERFLOW;
                 break;
             }
-            const uint8_t *src = gin + b.base + b.pos;
+            const uint8_t *src = b.sbase + b.lead + b.pos;
             for (uint32_t done = 0; done < len; done += 256) {
-                if (o.pos - o.flushed >= FLUSH_AT) flush(sm, o, lane, lane_factor, x2n);
+                if (o.pos - o.flushed >= G::FLUSH_AT) flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
                 uint32_t piece = len - done < 256 ? len - done : 256;
-                for (uint32_t i = lane; i < piece; i += 64) sm.ring[(o.pos + i) & RMASK] = src[done + i];
+                for (uint32_t i = lane; i < piece; i += 64)
+                    sm.ring[(o.skew + o.pos + i) & G::RMASK] = src[done + i];
                 o.pos += piece;
             }
             b.pos += len;
@@ -400,10 +464,10 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
             if (lane < 19) sm.lens[lane] = 0;
             WFENCE();
             for (uint32_t i = 0; i < ncode; ++i) {
-                if (b.cnt < 3) refill(sm, b, gin, lane);
+                if (b.cnt < 3) refill(sm.stage, b, lane);
                 uint32_t v = getbits(b, 3, err);
                 if (err) break;
-                if (lane == 0) sm.lens[CLEN_ORDER[i]] = (uint8_t)v;
+                if (lane == 0) sm.lens[clen_order(i)] = (uint8_t)v;
             }
             if (err) break;
             WFENCE();
@@ -415,7 +479,7 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
             build_lut<CBITS>(sm.clut, sm.ccount, sm.csym, lane);
             uint32_t index = 0, total = nlen + ndist;
             while (index < total) {
-                refill(sm, b, gin, lane);
+                if (b.cnt < 32) refill(sm.stage, b, lane);
                 int sym = decode_sym<CBITS>(b, sm.clut, sm.ccount, sm.csym, ST_BAD_CODE, err);
                 if (err) break;
                 if (sym < 16) {
@@ -467,8 +531,8 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
         }
         // literal / length / distance codes until end-of-block
         for (;;) {
-            if (o.pos - o.flushed >= FLUSH_AT) flush(sm, o, lane, lane_factor, x2n);
-            refill(sm, b, gin, lane);
+            if (o.pos - o.flushed >= G::FLUSH_AT) flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
+            if (b.cnt < 32) refill(sm.stage, b, lane);
             int sym = decode_sym<LBITS>(b, sm.llut, sm.lcount, sm.lsym, ST_BAD_CODE, err);
             if (err) break;
             if (sym < 256) {
@@ -476,25 +540,26 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
                     err = ST_OUTPUT_OVERFLOW;
                     break;
                 }
-                if (lane == 0) sm.ring[o.pos & RMASK] = (uint8_t)sym;
+                if (lane == 0) sm.ring[(o.skew + o.pos) & G::RMASK] = (uint8_t)sym;
                 ++o.pos;
                 continue;
             }
             if (sym == 256) break;
-            sym -= 257;
-            if (sym >= 29) {
+            const uint32_t lc = (uint32_t)sym - 257;
+            if (lc >= 29) {
                 err = ST_BAD_CODE;
                 break;
             }
-            uint32_t len = LEN_BASE[sym] + getbits(b, LEN_EXTRA[sym], err);
+            uint32_t len = len_base(lc) + getbits(b, len_extra(lc), err);
             if (err) break;
+            if (b.cnt < 28) refill(sm.stage, b, lane);
             int ds = decode_sym<DBITS>(b, sm.dlut, sm.dcount, sm.dsym, ST_BAD_DISTANCE, err);
             if (err) break;
             if (ds >= 30) {
                 err = ST_BAD_DISTANCE;
                 break;
             }
-            uint32_t dist = DIST_BASE[ds] + getbits(b, DIST_EXTRA[ds], err);
+            uint32_t dist = dist_base((uint32_t)ds) + getbits(b, dist_extra((uint32_t)ds), err);
             if (err) break;
             if (dist > o.pos) {
                 err = ST_BAD_DISTANCE;
@@ -505,19 +570,34 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
                 break;
             }
             WFENCE();
-            uint32_t src0 = o.pos - dist;
-            if (dist >= len) {
-                for (uint32_t off = lane; off < len; off += 64)
-                    sm.ring[(o.pos + off) & RMASK] = sm.ring[(src0 + off) & RMASK];
-            } else {
-                // period-`dist` pattern: every source byte precedes o.pos
-                uint32_t r = lane % dist;
-                uint32_t step = 64u % dist;
-                for (uint32_t off = lane; off < len; off += 64) {
-                    sm.ring[(o.pos + off) & RMASK] = sm.ring[(src0 + r) & RMASK];
-                    r += step;
-                    if (r >= dist) r -= dist;
+            const uint32_t d0 = o.skew + o.pos;
+            if (dist + len <= RING) {
+                const uint32_t s0 = d0 - dist;
+                if (dist >= len) {
+                    for (uint32_t off = lane; off < len; off += 64)
+                        sm.ring[(d0 + off) & G::RMASK] = sm.ring[(s0 + off) & G::RMASK];
+                } else {
+                    // period-`dist` pattern: every source byte precedes o.pos
+                    uint32_t r = lane % dist;
+                    uint32_t step = 64u % dist;
+                    for (uint32_t off = lane; off < len; off += 64) {
+                        sm.ring[(d0 + off) & G::RMASK] = sm.ring[(s0 + r) & G::RMASK];
+                        r += step;
+                        if (r >= dist) r -= dist;
+                    }
                 }
+            } else {
+                // Beyond the ring: these bytes were written back to HBM by this
+                // wave (dist > RING - 258 puts them below `flushed`, and there
+                // is no overlap since dist > len). Make those stores complete
+                // before reading them back.
+                if (o.visible < o.flushed) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    o.visible = o.flushed;
+                }
+                const uint8_t *src = o.g + (o.pos - dist);
+                for (uint32_t off = lane; off < len; off += 64) sm.ring[(d0 + off) & G::RMASK] = src[off];
             }
             WFENCE();
             o.pos += len;
@@ -526,7 +606,7 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
     } while (!last);
 
     if (err == ST_OK) {
-        flush(sm, o, lane, lane_factor, x2n);
+        flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
         crc_out = o.raw ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(x2n, o.pos)) ^ 0xFFFFFFFFu;
     } else {
         crc_out = 0;
@@ -535,17 +615,24 @@ __device__ void inflate_member(Smem &sm, const uint8_t *gin, uint64_t in_off, ui
     produced_out = o.pos;
 }
 
-__global__ __launch_bounds__(64) void fr_inflate_kernel(InflateArgs a)
+// NW waves per workgroup, each inflating its own members; the CRC table is
+// shared by the workgroup. Members come from the work-queue counter when one is
+// given, else member (block, wave) of a grid-stride walk.
+template <uint32_t RING, uint32_t NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void fr_inflate_kernel(InflateArgs a)
 {
-    __shared__ Smem sm;
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < 256; i += 64) {
+    static_assert(RING >= 4096 && RING <= 32768, "ring size");
+    __shared__ uint32_t crctab[256];
+    __shared__ __attribute__((aligned(16))) WaveMem<RING> waves[NW];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
         uint32_t c = i;
         for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (FR_CRC_POLY & (0u - (c & 1u)));
-        sm.crctab[i] = c;
+        crctab[i] = c;
     }
-    const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)SEG * (63 - lane));
-    WFENCE();
+    __syncthreads();
+    WaveMem<RING> &sm = waves[wave];
+    const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)Geometry<RING>::SEG * (63 - lane));
     uint32_t iter = 0;
     for (;;) {
         uint32_t m;
@@ -554,14 +641,14 @@ __global__ __launch_bounds__(64) void fr_inflate_kernel(InflateArgs a)
             if (lane == 0) t = atomicAdd(a.queue, 1u);
             m = uni(t);
         } else {
-            m = blockIdx.x + iter * gridDim.x;
+            m = (blockIdx.x + iter * gridDim.x) * NW + wave;
         }
         ++iter;
         if (m >= a.n) break;
         if (a.order) m = a.order[m];
         uint32_t st, produced, crc;
-        inflate_member(sm, a.in, a.in_off[m], a.in_len[m], a.out + a.out_off[m], a.out_cap[m], lane,
-                       lane_factor, a.x2n, st, produced, crc);
+        inflate_member<RING>(sm, crctab, a.in, a.in_off[m], a.in_len[m], a.out + a.out_off[m], a.out_cap[m], lane,
+                             lane_factor, a.x2n, st, produced, crc);
         if (lane == 0) {
             a.status[m] = st;
             a.produced[m] = produced;
@@ -611,11 +698,56 @@ hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const u
     return hipGetLastError();
 }
 
-hipError_t launch_inflate(const InflateArgs &a, uint32_t grid, hipStream_t stream)
+template <uint32_t RING, uint32_t NW>
+static hipError_t launch_variant(const InflateArgs &a, uint32_t waves, hipStream_t stream)
+{
+    uint32_t grid = (waves + NW - 1) / NW;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL((fr_inflate_kernel<RING, NW>), dim3(grid), dim3(64 * NW), 0, stream, a);
+    return hipGetLastError();
+}
+
+const InflateSchedule INFLATE_SCHEDULES[] = {
+    {32768, 1}, {16384, 1}, {16384, 2}, {8192, 1}, {8192, 2}, {8192, 4}, {4096, 2}, {4096, 4},
+};
+const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHEDULES[0]);
+
+uint32_t inflate_lds_bytes(int schedule)
+{
+    const InflateSchedule &s = INFLATE_SCHEDULES[schedule];
+    uint32_t per_wave = 0;
+    switch (s.ring) {
+    case 32768: per_wave = sizeof(WaveMem<32768>); break;
+    case 16384: per_wave = sizeof(WaveMem<16384>); break;
+    case 8192: per_wave = sizeof(WaveMem<8192>); break;
+    default: per_wave = sizeof(WaveMem<4096>); break;
+    }
+    return 1024 + s.waves_per_group * per_wave;
+}
+
+// Resident waves per CU of a schedule: LDS-limited (160 KiB per CU), and at most
+// 4 waves per SIMD for the kernel's register budget.
+uint32_t inflate_waves_per_cu(int schedule)
+{
+    uint32_t groups = (160u * 1024u) / inflate_lds_bytes(schedule);
+    uint32_t w = groups * INFLATE_SCHEDULES[schedule].waves_per_group;
+    return w > 16 ? 16 : (w ? w : 1);
+}
+
+hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stream, int schedule)
 {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fr_inflate_kernel, dim3(grid), dim3(64), 0, stream, a);
-    return hipGetLastError();
+    switch (schedule) {
+    case 0: return launch_variant<32768, 1>(a, waves, stream);
+    case 1: return launch_variant<16384, 1>(a, waves, stream);
+    case 2: return launch_variant<16384, 2>(a, waves, stream);
+    case 3: return launch_variant<8192, 1>(a, waves, stream);
+    case 4: return launch_variant<8192, 2>(a, waves, stream);
+    case 5: return launch_variant<8192, 4>(a, waves, stream);
+    case 6: return launch_variant<4096, 2>(a, waves, stream);
+    case 7: return launch_variant<4096, 4>(a, waves, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 } // namespace frhip

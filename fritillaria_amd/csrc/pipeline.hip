@@ -1,0 +1,302 @@
+// pipeline.hip -- kernels of the batched, device-resident path: BGZF member
+// discovery on the device, batch verification, the BAM header walk, a
+// single-workgroup exclusive scan and a fill kernel (graph-safe clearing).
+//
+// Member discovery restates crates/fritillaria-bgzf/src/discover.rs:51-113 and
+// block.rs:99-164 as a sieve: every byte offset whose bytes parse as a BGZF
+// header (magic, CM=8, FEXTRA, a BC subfield, a block size that fits its own
+// framing) is a candidate, compacted in file order. The candidates are the
+// true member chain exactly when the first is at 0, each one's block size lands
+// on the next, and the last lands on the end of the file; the host checks that
+// proof once and otherwise falls back to its serial walk.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frhip_internal.h"
+#include "pipeline_internal.h"
+
+namespace frhip {
+namespace {
+
+__device__ __forceinline__ uint32_t rd16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
+__device__ __forceinline__ uint32_t rd32(const uint8_t *p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+// block.rs:99-164 on the bytes at p; returns block size (0 if not a header).
+__device__ __forceinline__ uint32_t header_at(const uint8_t *f, uint64_t len, uint64_t p, uint32_t *hsize)
+{
+    if (p + 12 > len) return 0;
+    const uint8_t *b = f + p;
+    if (b[0] != 0x1f || b[1] != 0x8b || b[2] != 8 || (b[3] & 4) == 0) return 0;
+    uint32_t xlen = rd16(b + 10);
+    uint64_t hs = 12 + (uint64_t)xlen;
+    if (p + hs > len) return 0;
+    uint32_t off = 0;
+    for (int k = 0; k < 64 && off + 4 <= xlen; ++k) {
+        uint32_t slen = rd16(b + 12 + off + 2);
+        if (off + 4 + slen > xlen) return 0;
+        if (b[12 + off] == 'B' && b[12 + off + 1] == 'C') {
+            if (slen != 2) return 0;
+            uint32_t bs = rd16(b + 12 + off + 4) + 1;
+            if (bs < hs + 8) return 0;
+            if (p + bs > len) return 0;
+            *hsize = (uint32_t)hs;
+            return bs;
+        }
+        off += 4 + slen;
+    }
+    return 0;
+}
+
+constexpr uint32_t SIEVE_TILE = 16384;
+constexpr uint32_t SIEVE_THREADS = 256;
+
+// Pass 1: candidates per tile.
+__global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_t *f, uint64_t len, uint32_t *counts,
+                                                                    uint64_t n_tiles)
+{
+    __shared__ uint32_t wsum[SIEVE_THREADS / 64];
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        uint64_t base = t * SIEVE_TILE;
+        uint32_t c = 0;
+        for (uint32_t k = threadIdx.x; k < SIEVE_TILE; k += SIEVE_THREADS) {
+            uint64_t p = base + k;
+            uint32_t hs;
+            if (p < len && header_at(f, len, p, &hs)) ++c;
+        }
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+            for (uint32_t w = 0; w < SIEVE_THREADS / 64; ++w) s += wsum[w];
+            counts[t] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// Pass 2: candidates written in file order at the tile's scanned base.
+__global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_t *f, uint64_t len,
+                                                                    const uint64_t *tile_base, uint64_t n_tiles,
+                                                                    uint64_t *cand, uint32_t *cand_bs,
+                                                                    uint32_t *cand_hs, uint64_t cap)
+{
+    __shared__ uint32_t wcnt[SIEVE_THREADS / 64];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        uint64_t base = t * SIEVE_TILE;
+        uint64_t out = tile_base[t];
+        for (uint32_t k = 0; k < SIEVE_TILE; k += SIEVE_THREADS) {
+            uint64_t p = base + k + threadIdx.x;
+            uint32_t hs = 0, bs = 0;
+            if (p < len) bs = header_at(f, len, p, &hs);
+            uint64_t m = __ballot(bs != 0);
+            if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (uint32_t v = 0; v < SIEVE_THREADS / 64; ++v) {
+                if (v < w) before += wcnt[v];
+                total += wcnt[v];
+            }
+            if (bs) {
+                uint64_t i = out + before + (uint64_t)__popcll(m & lt);
+                if (i < cap) {
+                    cand[i] = p;
+                    cand_bs[i] = bs;
+                    cand_hs[i] = hs;
+                }
+            }
+            out += total;
+            __syncthreads();
+        }
+    }
+}
+
+// The tiling proof and the spans: member i is valid when it starts where
+// member i-1 ends (0 for the first) and the last ends at the file end.
+__global__ void spans_kernel(const uint8_t *f, uint64_t len, const uint64_t *cand, const uint32_t *cand_bs,
+                             const uint32_t *cand_hs, const uint64_t *n_cand, uint64_t cap, PipeMembers m,
+                             uint64_t *bad)
+{
+    uint64_t n = *n_cand;
+    if (n > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long *)bad, 1ull);
+        return;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t p = cand[i];
+        uint64_t end = p + cand_bs[i];
+        bool ok = (i == 0 ? p == 0 : true) && (i + 1 < n ? end == cand[i + 1] : end == len);
+        if (!ok) atomicAdd((unsigned long long *)bad, 1ull);
+        const uint8_t *b = f + p;
+        uint32_t bs = cand_bs[i], hs = cand_hs[i];
+        uint32_t isz = rd32(b + bs - 4);
+        if (isz > 65536) atomicAdd((unsigned long long *)bad, 1ull);
+        m.coff[i] = p;
+        m.in_off[i] = p + hs;
+        m.in_len[i] = bs - hs - 8;
+        m.isize[i] = isz;
+        m.crc_expect[i] = rd32(b + bs - 8);
+    }
+}
+
+// Exclusive scan by one workgroup of 1024 threads, 8 elements per thread per
+// step; out[n] receives the total. in and out may alias.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(1024) void scan_kernel(const TI *in, TO *out, const uint64_t *n_dev, uint64_t n_host)
+{
+    __shared__ TO sums[1024];
+    __shared__ TO carry_s;
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < n; base += 1024 * 8) {
+        TO v[8];
+        TO s = 0;
+        uint64_t i0 = base + (uint64_t)threadIdx.x * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = i0 + k < n ? (TO)in[i0 + k] : (TO)0;
+            s += v[k];
+        }
+        sums[threadIdx.x] = s;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1) {
+            TO add = threadIdx.x >= off ? sums[threadIdx.x - off] : (TO)0;
+            __syncthreads();
+            sums[threadIdx.x] += add;
+            __syncthreads();
+        }
+        TO run = carry_s + sums[threadIdx.x] - s;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (i0 + k < n) out[i0 + k] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s += sums[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry_s;
+}
+
+// First failing member in file order, with the check that failed
+// (status, then size, then CRC: crates/fritillaria-cuda/src/backend.rs:221-256).
+__global__ void verify_kernel(const uint32_t *status, const uint32_t *produced, const uint32_t *crc, PipeMembers m,
+                              const uint64_t *n_dev, unsigned long long *first_bad)
+{
+    uint64_t n = *n_dev;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        bool bad = status[i] != 0 || produced[i] != m.isize[i] || crc[i] != m.crc_expect[i];
+        if (bad) atomicMin(first_bad, (unsigned long long)i);
+    }
+}
+
+// crates/fritillaria-bam/src/columnar/header.rs:43-103 on the inflated buffer.
+__global__ void bam_header_kernel(const uint8_t *u, const uint64_t *u_len, uint64_t *records_start, uint64_t *status)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t len = *u_len;
+    uint64_t st = 0, pos = 0;
+    if (len < 12 || u[0] != 'B' || u[1] != 'A' || u[2] != 'M' || u[3] != 1) {
+        st = 1;
+    } else {
+        int32_t l_text = (int32_t)rd32(u + 4);
+        uint64_t te = 8 + (uint64_t)(l_text < 0 ? 0 : l_text);
+        if (l_text < 0 || te + 4 > len) {
+            st = 1;
+        } else {
+            int32_t nref = (int32_t)rd32(u + te);
+            pos = te + 4;
+            if (nref < 0) st = 1;
+            for (int32_t i = 0; i < nref && !st; ++i) {
+                if (pos + 4 > len) { st = 1; break; }
+                int32_t l_name = (int32_t)rd32(u + pos);
+                if (l_name < 0 || pos + 4 + (uint64_t)l_name + 4 > len) { st = 1; break; }
+                pos += 4 + (uint64_t)l_name + 4;
+            }
+        }
+    }
+    *records_start = st ? 0 : pos;
+    *status = st;
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t v, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+} // namespace
+
+uint64_t sieve_tiles(uint64_t len) { return (len + SIEVE_TILE - 1) / SIEVE_TILE; }
+
+hipError_t launch_sieve_count(const uint8_t *f, uint64_t len, uint32_t *counts, hipStream_t s)
+{
+    uint64_t nt = sieve_tiles(len);
+    if (nt == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)(nt < 65536 ? nt : 65536);
+    hipLaunchKernelGGL(sieve_count_kernel, dim3(grid), dim3(SIEVE_THREADS), 0, s, f, len, counts, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_sieve_write(const uint8_t *f, uint64_t len, const uint64_t *tile_base, uint64_t *cand,
+                              uint32_t *cand_bs, uint32_t *cand_hs, uint64_t cap, hipStream_t s)
+{
+    uint64_t nt = sieve_tiles(len);
+    if (nt == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)(nt < 65536 ? nt : 65536);
+    hipLaunchKernelGGL(sieve_write_kernel, dim3(grid), dim3(SIEVE_THREADS), 0, s, f, len, tile_base, nt, cand, cand_bs,
+                       cand_hs, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_spans(const uint8_t *f, uint64_t len, const uint64_t *cand, const uint32_t *cand_bs,
+                        const uint32_t *cand_hs, const uint64_t *n_cand, uint64_t cap, const PipeMembers &m,
+                        uint64_t *bad, hipStream_t s)
+{
+    uint32_t grid = (uint32_t)((cap + 255) / 256);
+    if (grid == 0) grid = 1;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(spans_kernel, dim3(grid), dim3(256), 0, s, f, len, cand, cand_bs, cand_hs, n_cand, cap, m, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_u32_u64(const uint32_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL((scan_kernel<uint32_t, uint64_t>), dim3(1), dim3(1024), 0, s, in, out, n_dev, n_host);
+    return hipGetLastError();
+}
+
+hipError_t launch_verify(const uint32_t *status, const uint32_t *produced, const uint32_t *crc, const PipeMembers &m,
+                         const uint64_t *n_dev, uint64_t cap, unsigned long long *first_bad, hipStream_t s)
+{
+    uint32_t grid = (uint32_t)((cap + 255) / 256);
+    if (grid == 0) grid = 1;
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(verify_kernel, dim3(grid), dim3(256), 0, s, status, produced, crc, m, n_dev, first_bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_header(const uint8_t *u, const uint64_t *u_len, uint64_t *records_start, uint64_t *status,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(bam_header_kernel, dim3(1), dim3(64), 0, s, u, u_len, records_start, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint32_t grid = (uint32_t)((n + 255) / 256);
+    if (grid > 1024) grid = 1024;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+} // namespace frhip

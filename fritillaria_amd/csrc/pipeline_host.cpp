@@ -1,0 +1,527 @@
+// pipeline_host.cpp -- the batched, device-resident BAM path (frhip_pipeline.h).
+//
+// One run enqueues, on the context's stream: member discovery (sieve, scan,
+// ordered compaction, chain proof, spans, ISIZE prefix sum), the persistent
+// work-queue inflate, batch verification, the BAM header walk, the boundary
+// speculation / reconcile / emit, and the column decode. Nothing is read back
+// until the end, so the whole run can be captured once as a hipGraph and
+// replayed. Buffers are sized at create from one discovery pass: every run over
+// the same resident file reproduces the same member table.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/frhip_pipeline.h"
+#include "bam_layout.h"
+#include "frhip_internal.h"
+#include "host_internal.h"
+#include "pipeline_internal.h"
+
+using namespace frhip;
+
+namespace {
+
+const int WIDTHS[FRHIP_BAM_N_COLUMNS] = {4, 4, 2, 1, 4, 4, 4, 4, 8, 4, 4, 4, 4, 4, 8};
+
+// Small block read back once per run.
+struct ResultBlock {
+    uint64_t chain_bad;      // member-chain proof failures (device discovery)
+    uint64_t first_bad;      // first member failing verification, or ~0
+    uint64_t hdr_status;     // BAM header walk
+    uint64_t records_start;
+    uint64_t totals[4];      // records, tail, status, fallbacks
+};
+
+enum { EV_START, EV_DISC, EV_INFL, EV_VERIFY, EV_SPEC, EV_REC, EV_END, N_EV };
+
+} // namespace
+
+struct frhip_pipeline {
+    frhip_ctx *ctx = nullptr;
+    frhip_pipeline_config cfg{};
+    uint64_t file_len = 0, n = 0, U = 0, cap_rec = 0, n_tiles = 0, cand_cap = 0;
+    bool device_discovery = false;
+    uint8_t *d_file = nullptr;
+    // discovery
+    uint32_t *d_tile_counts = nullptr;
+    uint64_t *d_tile_base = nullptr;
+    uint64_t *d_cand = nullptr;
+    uint32_t *d_cand_bs = nullptr, *d_cand_hs = nullptr;
+    PipeMembers m{};
+    uint64_t *d_n = nullptr;     // member count (device)
+    uint64_t *d_ulen = nullptr;  // inflated length (device)
+    // inflate
+    uint8_t *d_u = nullptr;
+    uint32_t *d_st = nullptr;    // status, produced, crc: 3n
+    uint64_t *d_queue = nullptr; // low 32 bits are the work-queue counter
+    uint32_t *d_order = nullptr;
+    // bam
+    uint8_t *d_bam = nullptr;
+    BamScratchLayout L{};
+    void *d_cols[FRHIP_BAM_N_COLUMNS] = {};
+    ResultBlock *d_res = nullptr;
+    ResultBlock res{};
+    ResultBlock *h_res = nullptr; // pinned
+    // views
+    frhip_buf u_view{};
+    frhip_buf col_view[FRHIP_BAM_N_COLUMNS] = {};
+    // graph and timing
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[N_EV] = {};
+    bool timed = false;
+    double stage_ms[7] = {};
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int dalloc(frhip_pipeline *p, void **out, uint64_t bytes, frhip_error *err, const char *what)
+{
+    hipError_t r = hipMalloc(out, bytes ? bytes : 1);
+    if (r != hipSuccess) {
+        *out = nullptr;
+        return hip_err(err, r, what);
+    }
+    p->allocs.push_back(*out);
+    return FRHIP_OK;
+}
+
+// Enqueues discovery: sieve count, tile scan, ordered write, chain proof and
+// spans, ISIZE scan. The member arrays are written for `cap` candidates at most.
+hipError_t enqueue_discovery(frhip_pipeline *p, hipStream_t s)
+{
+    hipError_t r = launch_fill_u64((uint64_t *)&p->d_res->chain_bad, 0, 1, s);
+    if (r == hipSuccess) r = launch_sieve_count(p->d_file, p->file_len, p->d_tile_counts, s);
+    if (r == hipSuccess) r = launch_scan_u32_u64(p->d_tile_counts, p->d_tile_base, nullptr, p->n_tiles, s);
+    if (r == hipSuccess)
+        r = launch_sieve_write(p->d_file, p->file_len, p->d_tile_base, p->d_cand, p->d_cand_bs, p->d_cand_hs,
+                               p->cand_cap, s);
+    if (r == hipSuccess)
+        r = launch_spans(p->d_file, p->file_len, p->d_cand, p->d_cand_bs, p->d_cand_hs, p->d_tile_base + p->n_tiles,
+                         p->cand_cap, p->m, (uint64_t *)&p->d_res->chain_bad, s);
+    if (r == hipSuccess)
+        r = launch_scan_u32_u64(p->m.isize, p->m.out_off, p->d_tile_base + p->n_tiles, 0, s);
+    return r;
+}
+
+hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
+{
+    hipError_t r = hipSuccess;
+    auto rec = [&](int e) {
+        if (timed && r == hipSuccess) r = hipEventRecord(p->ev[e], s);
+    };
+    rec(EV_START);
+    if (p->device_discovery) r = enqueue_discovery(p, s);
+    if (r == hipSuccess) r = launch_fill_u64(p->d_queue, 0, 1, s);
+    if (r == hipSuccess) r = launch_fill_u64((uint64_t *)&p->d_res->first_bad, ~0ull, 1, s);
+    rec(EV_DISC);
+    if (r == hipSuccess) {
+        InflateArgs a{};
+        a.in = p->d_file;
+        a.in_off = p->m.in_off;
+        a.in_len = p->m.in_len;
+        a.out = p->d_u;
+        a.out_off = p->m.out_off;
+        a.out_cap = p->m.isize;
+        a.status = p->d_st;
+        a.produced = p->d_st + p->n;
+        a.crc = p->d_st + 2 * p->n;
+        a.n = (uint32_t)p->n;
+        a.queue = (uint32_t *)p->d_queue;
+        a.order = p->cfg.order ? p->d_order : nullptr;
+        a.x2n = p->ctx->x2n;
+        int sched = p->cfg.inflate_schedule >= 0 && p->cfg.inflate_schedule < N_INFLATE_SCHEDULES
+                        ? p->cfg.inflate_schedule
+                        : p->ctx->inflate_schedule;
+        uint32_t wpc = p->cfg.waves_per_cu ? p->cfg.waves_per_cu : inflate_waves_per_cu(sched);
+        uint64_t waves = (uint64_t)p->ctx->n_cu * wpc;
+        if (waves > p->n) waves = p->n;
+        if (waves == 0) waves = 1;
+        r = launch_inflate(a, (uint32_t)waves, s, sched);
+    }
+    rec(EV_INFL);
+    if (r == hipSuccess)
+        r = launch_verify(p->d_st, p->d_st + p->n, p->d_st + 2 * p->n, p->m, p->d_n, p->n,
+                          (unsigned long long *)&p->d_res->first_bad, s);
+    if (r == hipSuccess) r = launch_bam_header(p->d_u, p->d_ulen, &p->d_res->records_start, &p->d_res->hdr_status, s);
+    rec(EV_VERIFY);
+    uint8_t *b = p->d_bam;
+    const BamScratchLayout &L = p->L;
+    BamSpecArgs sa{};
+    sa.buf = p->d_u;
+    sa.len = p->U;
+    sa.block_starts = p->m.out_off;
+    sa.nb = (uint32_t)p->n;
+    sa.window = p->cfg.bam_window ? p->cfg.bam_window : p->ctx->bam_window;
+    sa.spec_ok = b + L.o_ok;
+    sa.n_surv = b + L.o_ns;
+    sa.surv_pos = (uint64_t *)(b + L.o_spos);
+    sa.surv_cnt = (uint32_t *)(b + L.o_scnt);
+    sa.land = (uint64_t *)(b + L.o_land);
+    sa.exitp = (uint64_t *)(b + L.o_exit);
+    sa.cnt_tail = (uint32_t *)(b + L.o_ctail);
+    sa.complete = b + L.o_comp;
+    sa.rel_offs = (uint16_t *)(b + L.o_rel);
+    if (r == hipSuccess) r = launch_bam_spec(sa, s);
+    rec(EV_SPEC);
+    BamReconcileArgs ra{};
+    ra.buf = p->d_u;
+    ra.len = p->U;
+    ra.block_starts = p->m.out_off;
+    ra.nb = (uint32_t)p->n;
+    ra.start = 0;
+    ra.start_dev = &p->d_res->records_start;
+    ra.len_dev = nullptr;
+    ra.spec_ok = sa.spec_ok;
+    ra.n_surv = sa.n_surv;
+    ra.surv_pos = sa.surv_pos;
+    ra.surv_cnt = sa.surv_cnt;
+    ra.exitp = sa.exitp;
+    ra.cnt_tail = sa.cnt_tail;
+    ra.complete = sa.complete;
+    ra.entry = (uint64_t *)(b + L.o_entry);
+    ra.count = (uint32_t *)(b + L.o_count);
+    ra.first_index = (uint64_t *)(b + L.o_first);
+    ra.mode = b + L.o_mode;
+    ra.totals = p->d_res->totals;
+    if (r == hipSuccess) r = launch_bam_reconcile(ra, s);
+    rec(EV_REC);
+    BamEmitArgs ea{};
+    ea.buf = p->d_u;
+    ea.block_starts = p->m.out_off;
+    ea.coffsets = p->m.coff;
+    ea.nb = (uint32_t)p->n;
+    ea.entry = ra.entry;
+    ea.count = ra.count;
+    ea.first_index = ra.first_index;
+    ea.mode = ra.mode;
+    ea.surv_cnt = sa.surv_cnt;
+    ea.rel_offs = sa.rel_offs;
+    ea.record_offsets = (uint64_t *)p->d_cols[FRHIP_BAM_RECORD_OFFSETS];
+    ea.virtual_offsets = (uint64_t *)p->d_cols[FRHIP_BAM_VIRTUAL_OFFSETS];
+    if (r == hipSuccess) r = launch_bam_emit(ea, s);
+    BamDecodeArgs da{};
+    da.buf = p->d_u;
+    da.n = 0;
+    da.n_dev = &p->d_res->totals[0];
+    da.c.ref_id = (int32_t *)p->d_cols[0];
+    da.c.pos = (int32_t *)p->d_cols[1];
+    da.c.flags = (uint16_t *)p->d_cols[2];
+    da.c.mapq = (uint8_t *)p->d_cols[3];
+    da.c.l_seq = (uint32_t *)p->d_cols[4];
+    da.c.next_ref_id = (int32_t *)p->d_cols[5];
+    da.c.next_pos = (int32_t *)p->d_cols[6];
+    da.c.tlen = (int32_t *)p->d_cols[7];
+    da.c.record_offsets = (uint64_t *)p->d_cols[8];
+    da.c.cigar_start = (uint32_t *)p->d_cols[9];
+    da.c.seq_start = (uint32_t *)p->d_cols[10];
+    da.c.qual_start = (uint32_t *)p->d_cols[11];
+    da.c.aux_start = (uint32_t *)p->d_cols[12];
+    da.c.record_end = (uint32_t *)p->d_cols[13];
+    if (r == hipSuccess) r = launch_bam_decode(da, p->cap_rec, s);
+    rec(EV_END);
+    return r;
+}
+
+int check_result(frhip_pipeline *p, frhip_error *err)
+{
+    const ResultBlock &R = p->res;
+    if (p->device_discovery && R.chain_bad)
+        return set_err(err, FRHIP_E_INVALID_BLOCK, 0, "device member discovery lost the member chain");
+    if (R.first_bad != ~0ull) {
+        uint64_t i = R.first_bad;
+        uint32_t st[3], isz, crc;
+        uint64_t coff;
+        hipError_t r = hipMemcpy(&st[0], p->d_st + i, 4, hipMemcpyDeviceToHost);
+        if (r == hipSuccess) r = hipMemcpy(&st[1], p->d_st + p->n + i, 4, hipMemcpyDeviceToHost);
+        if (r == hipSuccess) r = hipMemcpy(&st[2], p->d_st + 2 * p->n + i, 4, hipMemcpyDeviceToHost);
+        if (r == hipSuccess) r = hipMemcpy(&isz, p->m.isize + i, 4, hipMemcpyDeviceToHost);
+        if (r == hipSuccess) r = hipMemcpy(&crc, p->m.crc_expect + i, 4, hipMemcpyDeviceToHost);
+        if (r == hipSuccess) r = hipMemcpy(&coff, p->m.coff + i, 8, hipMemcpyDeviceToHost);
+        if (r != hipSuccess) return hip_err(err, r, "reading the failing member");
+        if (st[0] != 0) {
+            int k = set_err(err, FRHIP_E_INFLATE, coff, "%s", inflate_status_message(st[0]));
+            if (err) err->code = st[0];
+            return k;
+        }
+        if (st[1] != isz) {
+            int k = set_err(err, FRHIP_E_SIZE_MISMATCH, coff, "ISIZE says %u bytes, inflated %u", isz, st[1]);
+            if (err) { err->expected = isz; err->actual = st[1]; }
+            return k;
+        }
+        int k = set_err(err, FRHIP_E_CHECKSUM_MISMATCH, coff, "expected crc32 %#010x, got %#010x", crc, st[2]);
+        if (err) { err->expected = crc; err->actual = st[2]; }
+        return k;
+    }
+    if (R.hdr_status) {
+        int k = set_err(err, FRHIP_E_MALFORMED, 0, "BAM header does not parse");
+        if (err) std::strcpy(err->format, "bam header");
+        return k;
+    }
+    if (R.totals[2]) {
+        int k = set_err(err, FRHIP_E_MALFORMED, R.totals[1], "malformed record reached from a confirmed boundary");
+        if (err) {
+            err->code = (uint32_t)R.totals[2];
+            std::strcpy(err->format, "bam");
+        }
+        return k;
+    }
+    for (int c = 0; c < FRHIP_BAM_N_COLUMNS; ++c) p->col_view[c].len = R.totals[0] * WIDTHS[c];
+    return FRHIP_OK;
+}
+
+} // namespace
+
+extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64_t len,
+                                     const frhip_pipeline_config *cfg, frhip_pipeline **out, frhip_error *err)
+{
+    *out = nullptr;
+    (void)hipSetDevice(ctx->ordinal);
+    frhip_pipeline *p = new (std::nothrow) frhip_pipeline;
+    if (!p) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    p->ctx = ctx;
+    ctx_retain(ctx);
+    p->cfg.inflate_schedule = -1;
+    if (cfg) p->cfg = *cfg;
+    p->file_len = len;
+    hipStream_t s = ctx->stream;
+    int rc;
+    auto fail = [&](int k) {
+        frhip_pipeline_free(p);
+        return k;
+    };
+    if ((rc = dalloc(p, (void **)&p->d_file, len + 4096, err, "file buffer"))) return fail(rc);
+    hipError_t r = hipMemcpy(p->d_file, file, len, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemset(p->d_file + len, 0, 4096);
+    if (r != hipSuccess) return fail(hip_err(err, r, "uploading the file"));
+    if ((rc = dalloc(p, (void **)&p->d_res, sizeof(ResultBlock), err, "result block"))) return fail(rc);
+    r = hipHostMalloc((void **)&p->h_res, sizeof(ResultBlock), hipHostMallocDefault);
+    if (r != hipSuccess) return fail(hip_err(err, r, "hipHostMalloc"));
+
+    // discovery once, to size everything
+    std::vector<frhip_block_span> host_spans;
+    p->n_tiles = sieve_tiles(len);
+    bool try_device = !p->cfg.host_discovery && len > 0;
+    if (try_device) {
+        p->cand_cap = len / 28 + 2;
+        if ((rc = dalloc(p, (void **)&p->d_tile_counts, p->n_tiles * 4 + 4, err, "tile counts"))) return fail(rc);
+        if ((rc = dalloc(p, (void **)&p->d_tile_base, (p->n_tiles + 1) * 8, err, "tile bases"))) return fail(rc);
+        r = launch_sieve_count(p->d_file, len, p->d_tile_counts, s);
+        if (r == hipSuccess) r = launch_scan_u32_u64(p->d_tile_counts, p->d_tile_base, nullptr, p->n_tiles, s);
+        uint64_t n_cand = 0;
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(&n_cand, p->d_tile_base + p->n_tiles, 8, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipStreamSynchronize(s);
+        if (r != hipSuccess) return fail(hip_err(err, r, "member sieve"));
+        p->n = n_cand;
+        p->cand_cap = n_cand;
+    }
+    if (!try_device) {
+        uint64_t pos = 0;
+        rc = discover(file, len, 0, host_spans, &pos, err);
+        if (rc) return fail(rc);
+        if (pos != len)
+            return fail(set_err(err, FRHIP_E_INVALID_BLOCK, pos, "truncated BGZF: trailing bytes are not a member"));
+        p->n = host_spans.size();
+    }
+    const uint64_t n = p->n;
+    uint64_t nn = n ? n : 1;
+    if ((rc = dalloc(p, (void **)&p->m.coff, nn * 8, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->m.in_off, nn * 8, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->m.in_len, nn * 4, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->m.isize, nn * 4, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->m.crc_expect, nn * 4, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->m.out_off, (nn + 1) * 8, err, "members"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->d_n, 8, err, "count"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->d_ulen, 8, err, "count"))) return fail(rc);
+    if (try_device) {
+        if ((rc = dalloc(p, (void **)&p->d_cand, nn * 8, err, "candidates"))) return fail(rc);
+        if ((rc = dalloc(p, (void **)&p->d_cand_bs, nn * 4, err, "candidates"))) return fail(rc);
+        if ((rc = dalloc(p, (void **)&p->d_cand_hs, nn * 4, err, "candidates"))) return fail(rc);
+        r = enqueue_discovery(p, s);
+        uint64_t bad = 0;
+        if (r == hipSuccess) r = hipMemcpyAsync(&bad, &p->d_res->chain_bad, 8, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipStreamSynchronize(s);
+        if (r != hipSuccess) return fail(hip_err(err, r, "member discovery"));
+        p->device_discovery = bad == 0;
+        if (!p->device_discovery) {
+            // the sieve's candidates are not exactly the member chain: walk it
+            uint64_t pos = 0;
+            host_spans.clear();
+            rc = discover(file, len, 0, host_spans, &pos, err);
+            if (rc) return fail(rc);
+            if (pos != len)
+                return fail(set_err(err, FRHIP_E_INVALID_BLOCK, pos, "truncated BGZF: trailing bytes are not a member"));
+            if (host_spans.size() > p->cand_cap)
+                return fail(set_err(err, FRHIP_E_DEVICE, 0, "member table larger than sized"));
+            p->n = host_spans.size();
+        }
+    }
+    if (!p->device_discovery) {
+        const uint64_t k = p->n;
+        std::vector<uint64_t> coff(k), in_off(k), out_off(k + 1);
+        std::vector<uint32_t> in_len(k), isz(k), crc(k);
+        out_off[0] = 0;
+        for (uint64_t i = 0; i < k; ++i) {
+            coff[i] = host_spans[i].compressed_offset;
+            in_off[i] = host_spans[i].payload_start;
+            in_len[i] = (uint32_t)host_spans[i].payload_len;
+            isz[i] = host_spans[i].isize;
+            crc[i] = host_spans[i].crc32;
+            out_off[i + 1] = out_off[i] + isz[i];
+        }
+        r = hipMemcpy(p->m.coff, coff.data(), k * 8, hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = hipMemcpy(p->m.in_off, in_off.data(), k * 8, hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = hipMemcpy(p->m.in_len, in_len.data(), k * 4, hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = hipMemcpy(p->m.isize, isz.data(), k * 4, hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = hipMemcpy(p->m.crc_expect, crc.data(), k * 4, hipMemcpyHostToDevice);
+        if (r == hipSuccess) r = hipMemcpy(p->m.out_off, out_off.data(), (k + 1) * 8, hipMemcpyHostToDevice);
+        if (r != hipSuccess) return fail(hip_err(err, r, "uploading members"));
+    }
+    r = hipMemcpy(&p->U, p->m.out_off + p->n, 8, hipMemcpyDeviceToHost);
+    if (r == hipSuccess) r = hipMemcpy(p->d_n, &p->n, 8, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemcpy(p->d_ulen, &p->U, 8, hipMemcpyHostToDevice);
+    if (r != hipSuccess) return fail(hip_err(err, r, "member table"));
+    if (p->n > 0xFFFFFFFFull) return fail(set_err(err, FRHIP_E_DEVICE, 0, "too many members for one pipeline"));
+
+    // inflate buffers
+    if ((rc = dalloc(p, (void **)&p->d_u, p->U, err, "inflated output"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->d_st, nn * 12, err, "member status"))) return fail(rc);
+    if ((rc = dalloc(p, (void **)&p->d_queue, 8, err, "work queue"))) return fail(rc);
+    if (p->cfg.order) {
+        // largest compressed payload first: the long members start early
+        std::vector<uint32_t> in_len(p->n);
+        r = hipMemcpy(in_len.data(), p->m.in_len, p->n * 4, hipMemcpyDeviceToHost);
+        if (r != hipSuccess) return fail(hip_err(err, r, "member lengths"));
+        std::vector<uint32_t> order(p->n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+        if ((rc = dalloc(p, (void **)&p->d_order, nn * 4, err, "member order"))) return fail(rc);
+        r = hipMemcpy(p->d_order, order.data(), p->n * 4, hipMemcpyHostToDevice);
+        if (r != hipSuccess) return fail(hip_err(err, r, "member order"));
+    }
+    // BAM buffers: at most one record per 36 inflated bytes
+    p->L = bam_layout(nn);
+    if ((rc = dalloc(p, (void **)&p->d_bam, p->L.bytes, err, "boundary scan"))) return fail(rc);
+    p->cap_rec = p->U / 36 + 1;
+    for (int c = 0; c < FRHIP_BAM_N_COLUMNS; ++c) {
+        if ((rc = dalloc(p, &p->d_cols[c], p->cap_rec * WIDTHS[c], err, "columns"))) return fail(rc);
+        p->col_view[c] = frhip_buf{p->d_cols[c], 0, ctx->ordinal};
+    }
+    p->u_view = frhip_buf{p->d_u, p->U, ctx->ordinal};
+    for (auto &e : p->ev) {
+        r = hipEventCreate(&e);
+        if (r != hipSuccess) return fail(hip_err(err, r, "hipEventCreate"));
+    }
+    if (p->cfg.use_graph && p->n > 0) {
+        r = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (r == hipSuccess) {
+            hipError_t r2 = enqueue_run(p, s, false);
+            r = hipStreamEndCapture(s, &p->graph);
+            if (r2 != hipSuccess) r = r2;
+        }
+        if (r == hipSuccess) r = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (r != hipSuccess) return fail(hip_err(err, r, "capturing the pipeline graph"));
+    }
+    *out = p;
+    return FRHIP_OK;
+}
+
+extern "C" void frhip_pipeline_free(frhip_pipeline *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->ordinal);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *a : p->allocs) (void)hipFree(a);
+    if (p->h_res) (void)hipHostFree(p->h_res);
+    ctx_release(p->ctx);
+    delete p;
+}
+
+extern "C" int frhip_pipeline_launch(frhip_pipeline *p, frhip_error *err)
+{
+    (void)hipSetDevice(p->ctx->ordinal);
+    hipStream_t s = p->ctx->stream;
+    if (p->n == 0) return FRHIP_OK;
+    hipError_t r;
+    if (p->exec) {
+        r = hipGraphLaunch(p->exec, s);
+        p->timed = false;
+    } else {
+        r = enqueue_run(p, s, true);
+        p->timed = true;
+    }
+    if (r == hipSuccess) r = hipMemcpyAsync(p->h_res, p->d_res, sizeof(ResultBlock), hipMemcpyDeviceToHost, s);
+    if (r != hipSuccess) return hip_err(err, r, "launching the pipeline");
+    return FRHIP_OK;
+}
+
+extern "C" int frhip_pipeline_finish(frhip_pipeline *p, frhip_error *err)
+{
+    if (p->n == 0) {
+        p->res = ResultBlock{};
+        p->res.first_bad = ~0ull;
+        return set_err(err, FRHIP_E_MALFORMED, 0, "no BGZF members");
+    }
+    hipError_t r = hipStreamSynchronize(p->ctx->stream);
+    if (r != hipSuccess) return hip_err(err, r, "pipeline");
+    p->res = *p->h_res;
+    if (p->timed) {
+        float ms = 0;
+        for (int i = 0; i < 6; ++i) {
+            (void)hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]);
+            p->stage_ms[i] = ms;
+        }
+        (void)hipEventElapsedTime(&ms, p->ev[EV_START], p->ev[EV_END]);
+        p->stage_ms[6] = ms;
+    }
+    return check_result(p, err);
+}
+
+extern "C" int frhip_pipeline_run(frhip_pipeline *p, frhip_error *err)
+{
+    int rc = frhip_pipeline_launch(p, err);
+    if (rc) return rc;
+    return frhip_pipeline_finish(p, err);
+}
+
+extern "C" uint64_t frhip_pipeline_members(const frhip_pipeline *p) { return p->n; }
+extern "C" uint64_t frhip_pipeline_inflated_len(const frhip_pipeline *p) { return p->U; }
+extern "C" uint64_t frhip_pipeline_compressed_len(const frhip_pipeline *p) { return p->file_len; }
+extern "C" uint64_t frhip_pipeline_records(const frhip_pipeline *p) { return p->res.totals[0]; }
+extern "C" uint64_t frhip_pipeline_records_start(const frhip_pipeline *p) { return p->res.records_start; }
+extern "C" uint64_t frhip_pipeline_tail(const frhip_pipeline *p) { return p->res.totals[1]; }
+extern "C" uint64_t frhip_pipeline_fallbacks(const frhip_pipeline *p) { return p->res.totals[3]; }
+extern "C" int frhip_pipeline_device_discovery(const frhip_pipeline *p) { return p->device_discovery ? 1 : 0; }
+extern "C" const frhip_buf *frhip_pipeline_inflated(const frhip_pipeline *p) { return &p->u_view; }
+extern "C" const frhip_buf *frhip_pipeline_column(const frhip_pipeline *p, int column)
+{
+    if (column < 0 || column >= FRHIP_BAM_N_COLUMNS) return nullptr;
+    return &p->col_view[column];
+}
+
+extern "C" int frhip_pipeline_member_offsets(const frhip_pipeline *p, uint64_t *offsets, uint64_t *coffsets,
+                                             frhip_error *err)
+{
+    hipError_t r = hipSuccess;
+    if (offsets) r = hipMemcpy(offsets, p->m.out_off, (p->n + 1) * 8, hipMemcpyDeviceToHost);
+    if (r == hipSuccess && coffsets && p->n) r = hipMemcpy(coffsets, p->m.coff, p->n * 8, hipMemcpyDeviceToHost);
+    if (r != hipSuccess) return hip_err(err, r, "member offsets");
+    return FRHIP_OK;
+}
+
+extern "C" int frhip_pipeline_stage_ms(const frhip_pipeline *p, double *ms, int n)
+{
+    int k = n < 7 ? n : 7;
+    for (int i = 0; i < k; ++i) ms[i] = p->timed ? p->stage_ms[i] : -1.0;
+    return k;
+}

@@ -167,7 +167,7 @@ static const char RG_VALUE[] = "rg1";
 
 static uint64_t record_size(const synth_bam_params *P, const shape_t *s)
 {
-    uint64_t body = 32 + 14 /* name "syn%09llu\0" */ + 4ull * s->n_cigar + (s->l_seq + 1) / 2 + s->l_seq;
+    uint64_t body = 32 + 14 /* name "syn%010llu\0" */ + 4ull * s->n_cigar + (s->l_seq + 1) / 2 + s->l_seq;
     if (P->aux_mode == 0) {
         body += 4 + 3 + sizeof RG_VALUE; /* NM:C:x, RG:Z:rg1\0 */
     } else {
@@ -197,7 +197,7 @@ static void write_record(const synth_bam_params *P, uint64_t i, const shape_t *s
     w32(o + 32, 0);
     uint8_t *p = o + 36;
     char name[16];
-    snprintf(name, sizeof name, "syn%09llu", (unsigned long long)(i % 1000000000ull));
+    snprintf(name, sizeof name, "syn%010llu", (unsigned long long)(i % 10000000000ull));
     memcpy(p, name, 13);
     p[13] = 0;
     p += 14;

@@ -133,6 +133,9 @@ def lib():
         "frhip_ctx_ordinal": (C.c_int, [_vp]),
         "frhip_ctx_stream": (_vp, [_vp]),
         "frhip_ctx_set_bam_window": (C.c_int, [_vp, C.c_uint32]),
+        "frhip_ctx_synchronize": (C.c_int, [_vp, _ep]),
+        "frhip_ctx_set_inflate_schedule": (C.c_int, [_vp, C.c_int]),
+        "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, C.c_int]),
         "frhip_codec_name": (C.c_char_p, []),
         "frhip_device_codec_name": (C.c_char_p, []),
         "frhip_buf_len": (_u64, [_vp]),
@@ -158,6 +161,23 @@ def lib():
                                               C.POINTER(_u64), C.POINTER(_u64), _ep]),
         "frhip_reader_carry_from": (C.c_int, [_vp, _u64, _ep]),
         "frhip_reader_free": (None, [_vp]),
+        "frhip_pipeline_create": (C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(_vp), _ep]),
+        "frhip_pipeline_free": (None, [_vp]),
+        "frhip_pipeline_run": (C.c_int, [_vp, _ep]),
+        "frhip_pipeline_launch": (C.c_int, [_vp, _ep]),
+        "frhip_pipeline_finish": (C.c_int, [_vp, _ep]),
+        "frhip_pipeline_members": (_u64, [_vp]),
+        "frhip_pipeline_inflated_len": (_u64, [_vp]),
+        "frhip_pipeline_compressed_len": (_u64, [_vp]),
+        "frhip_pipeline_records": (_u64, [_vp]),
+        "frhip_pipeline_records_start": (_u64, [_vp]),
+        "frhip_pipeline_tail": (_u64, [_vp]),
+        "frhip_pipeline_fallbacks": (_u64, [_vp]),
+        "frhip_pipeline_device_discovery": (C.c_int, [_vp]),
+        "frhip_pipeline_inflated": (_vp, [_vp]),
+        "frhip_pipeline_column": (_vp, [_vp, C.c_int]),
+        "frhip_pipeline_member_offsets": (C.c_int, [_vp, _vp, _vp, _ep]),
+        "frhip_pipeline_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -178,6 +198,15 @@ def exported_symbols() -> list:
             txt = open(p).read()
             names += re.findall(r"\b(frhip_[a-z0-9_]+)\s*\(", txt)
     return sorted(set(names))
+
+
+def inflate_schedules() -> list:
+    """(ring bytes, waves per workgroup, resident waves per CU) per inflate variant."""
+    ring = np.zeros(32, np.uint32)
+    nw = np.zeros(32, np.uint32)
+    wpc = np.zeros(32, np.uint32)
+    k = lib().frhip_inflate_schedules(_ptr(ring), _ptr(nw), _ptr(wpc), 32)
+    return [(int(ring[i]), int(nw[i]), int(wpc[i])) for i in range(k)]
 
 
 def device_is_available() -> bool:
@@ -220,6 +249,14 @@ class HipContext:
     @property
     def stream(self) -> int:
         return lib().frhip_ctx_stream(self._h) or 0
+
+    def synchronize(self) -> None:
+        e = _Err()
+        _raise(lib().frhip_ctx_synchronize(self._h, C.byref(e)), e)
+
+    def set_inflate_schedule(self, schedule: int) -> None:
+        if lib().frhip_ctx_set_inflate_schedule(self._h, schedule) != OK:
+            raise ValueError(f"no inflate schedule {schedule}")
 
     def set_bam_window(self, window: int) -> None:
         lib().frhip_ctx_set_bam_window(self._h, window)
@@ -588,3 +625,109 @@ def select_codec(backend: Backend = Backend.AUTO):
     if backend in (Backend.AUTO, Backend.HIP):
         return HipCodec(), Backend.HIP
     raise ValueError(backend)
+
+
+# ---------------------------------------------------------------------------
+# batched, device-resident pipeline (include/frhip_pipeline.h)
+# ---------------------------------------------------------------------------
+
+
+class PipelineConfig(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_uint32), ("order", C.c_uint32), ("bam_window", C.c_uint32),
+                ("use_graph", C.c_uint32), ("host_discovery", C.c_uint32), ("inflate_schedule", C.c_int32)]
+
+
+STAGES = ["discovery", "inflate", "verify", "speculate", "reconcile", "columns", "total"]
+
+
+class BatchedPipeline:
+    """A whole BGZF BAM resident in HBM, decoded to columns in one enqueue."""
+
+    def __init__(self, context: HipContext, file_bytes, *, waves_per_cu=0, order=0, bam_window=0,
+                 use_graph=False, host_discovery=False, inflate_schedule=-1):
+        self.ctx = context
+        self._file = _u8(file_bytes)
+        cfg = PipelineConfig(waves_per_cu, order, bam_window, 1 if use_graph else 0, 1 if host_discovery else 0,
+                             inflate_schedule)
+        h = _vp()
+        e = _Err()
+        rc = lib().frhip_pipeline_create(context.handle, _ptr(self._file), self._file.size, C.byref(cfg),
+                                         C.byref(h), C.byref(e))
+        _raise(rc, e)
+        self._h = h
+
+    def run(self) -> None:
+        e = _Err()
+        _raise(lib().frhip_pipeline_run(self._h, C.byref(e)), e)
+
+    def launch(self) -> None:
+        e = _Err()
+        _raise(lib().frhip_pipeline_launch(self._h, C.byref(e)), e)
+
+    def finish(self) -> None:
+        e = _Err()
+        _raise(lib().frhip_pipeline_finish(self._h, C.byref(e)), e)
+
+    def members(self) -> int:
+        return int(lib().frhip_pipeline_members(self._h))
+
+    def inflated_len(self) -> int:
+        return int(lib().frhip_pipeline_inflated_len(self._h))
+
+    def compressed_len(self) -> int:
+        return int(lib().frhip_pipeline_compressed_len(self._h))
+
+    def records(self) -> int:
+        return int(lib().frhip_pipeline_records(self._h))
+
+    def records_start(self) -> int:
+        return int(lib().frhip_pipeline_records_start(self._h))
+
+    def tail(self) -> int:
+        return int(lib().frhip_pipeline_tail(self._h))
+
+    def fallbacks(self) -> int:
+        return int(lib().frhip_pipeline_fallbacks(self._h))
+
+    def device_discovery(self) -> bool:
+        return bool(lib().frhip_pipeline_device_discovery(self._h))
+
+    def inflated(self) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_pipeline_inflated(self._h), owned=False, keepalive=self)
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_pipeline_column(self._h, _COL_INDEX[name]), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(BAM_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def columns_to_host(self) -> dict:
+        return {name: self.column_to_host(name) for name, _ in BAM_COLUMNS}
+
+    def member_offsets(self):
+        n = self.members()
+        offs = np.zeros(n + 1, np.uint64)
+        co = np.zeros(max(n, 1), np.uint64)
+        e = _Err()
+        _raise(lib().frhip_pipeline_member_offsets(self._h, _ptr(offs), _ptr(co), C.byref(e)), e)
+        return offs, co[:n]
+
+    def stage_ms(self) -> dict:
+        out = (C.c_double * 7)()
+        k = lib().frhip_pipeline_stage_ms(self._h, out, 7)
+        return {STAGES[i]: out[i] for i in range(k)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_pipeline_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -68,14 +68,26 @@ int frhip_device_available(void);
    lib.rs:194-201). `stream` is a hipStream_t owned by the caller, or NULL for a
    stream of the context's own. Kernels are precompiled code objects: no JIT. */
 int frhip_ctx_create(int ordinal, void *stream, frhip_ctx **out, frhip_error *err);
+/* Releases the creator's reference. Readers and pipelines hold their own, so the
+   context outlives them whatever order they are freed in (the reference shares
+   its context as Arc<CudaContext>). */
 void frhip_ctx_destroy(frhip_ctx *ctx);
 int frhip_ctx_ordinal(const frhip_ctx *ctx);
 /* hipStream_t the context launches on. */
 void *frhip_ctx_stream(const frhip_ctx *ctx);
+/* Waits for all work on the context's device. */
+int frhip_ctx_synchronize(frhip_ctx *ctx, frhip_error *err);
 /* Bytes of each member whose offsets are tried as record-start candidates by the
    BAM boundary scan (default 2048). A tuning knob: results do not depend on it.
    No reference counterpart; the reference speculates on member starts only. */
 int frhip_ctx_set_bam_window(frhip_ctx *ctx, uint32_t window);
+/* Inflate kernel variant (output ring size per member x waves per workgroup),
+   an index into the table frhip_inflate_schedules describes. Results do not
+   depend on it. No reference counterpart. */
+int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
+/* Writes up to cap entries of (ring bytes, waves per workgroup, resident waves
+   per CU); returns the number of schedules. */
+int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, int cap);
 /* Codec names of CudaCodec (lib.rs:152-294): host path and device path. */
 const char *frhip_codec_name(void);        /* "hip" */
 const char *frhip_device_codec_name(void); /* "hip-device" */

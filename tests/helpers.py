@@ -26,6 +26,8 @@ def deflate_raw(data: bytes, level=6, strategy=zlib.Z_DEFAULT_STRATEGY, flushes=
 def bgzf_member(payload: bytes, cdata: bytes = None, level=6, crc=None, isize=None) -> bytes:
     if cdata is None:
         cdata = deflate_raw(payload, level)
+        if len(cdata) > 65495 + 15:  # noodles re-encodes at level 0 (deflate.rs:58-95)
+            cdata = deflate_raw(payload, 0)
     bsize = 18 + len(cdata) + 8
     assert bsize <= 65536
     hdr = bytes([0x1F, 0x8B, 8, 4, 0, 0, 0, 0, 0, 0xFF]) + struct.pack("<HBBHH", 6, 66, 67, 2, bsize - 1)

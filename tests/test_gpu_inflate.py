@@ -56,7 +56,10 @@ def test_every_level_every_payload(codec, oracle_mod, level):
 def test_every_strategy(codec, oracle_mod, strategy):
     print(f"[gpu] inflate strategy {strategy}", flush=True)
     rng = np.random.default_rng(strategy)
-    members = [bgzf_member(p, cdata=deflate_raw(p, 6, strategy)) for p in _payload_kinds(rng).values()]
+    members = []
+    for p in _payload_kinds(rng).values():
+        cd = deflate_raw(p, 6, strategy)
+        members.append(bgzf_member(p, cdata=cd if len(cd) <= 65510 else deflate_raw(p, 0)))
     assert_matches_oracle(codec, oracle_mod, b"".join(members))
 
 

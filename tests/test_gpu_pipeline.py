@@ -1,0 +1,108 @@
+"""GPU parity of the batched, device-resident pipeline: the whole file decoded
+in one enqueue (and as one hipGraph replay) equals the oracle's whole-file
+decode, bit for bit."""
+import numpy as np
+import pytest
+
+from conftest import golden_bytes
+from helpers import bam_header, bgzf, bgzf_member, random_records
+
+pytestmark = pytest.mark.gpu
+
+REF_COLS = ["reference_sequence_id", "position", "flags", "mapping_quality", "sequence_len",
+            "mate_reference_sequence_id", "mate_position", "template_length", "record_offsets",
+            "cigar_start", "sequence_start", "quality_start", "aux_start", "record_end"]
+
+
+def oracle_whole_file(oracle_mod, data):
+    d = oracle_mod.discover(data)
+    r = oracle_mod.inflate_batch(data, d.spans)
+    assert r.kind == 0
+    u = np.frombuffer(r.data, np.uint8)
+    rc, start, _, _ = oracle_mod.bam_parse_header(u)
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, start)
+    assert rc == 0
+    cols = oracle_mod.bam_decode_columns(u, rec)
+    cols["virtual_offsets"] = oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec)
+    return r, u, start, tail, cols
+
+
+def check_pipeline(pipe, oracle_mod, data, check_bytes=True):
+    r, u, start, tail, cols = oracle_whole_file(oracle_mod, data)
+    assert pipe.records() == len(cols["position"]) and pipe.tail() == tail
+    assert pipe.records_start() == start
+    offs, co = pipe.member_offsets()
+    assert offs.tolist() == r.offsets.tolist()
+    if check_bytes:
+        assert pipe.inflated().to_vec().tobytes() == r.data
+    got = pipe.columns_to_host()
+    for name in REF_COLS + ["virtual_offsets"]:
+        assert np.array_equal(got[name], cols[name]), name
+
+
+@pytest.mark.parametrize("config,n", [(1, 10000), (2, 60000), (3, 40)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_pipeline_matches_oracle(hip_ctx, oracle_mod, config, n, graph):
+    print(f"[gpu] pipeline config {config} n={n} graph={graph}", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(config, n_records=n)
+    pipe = fa.BatchedPipeline(hip_ctx, s.compressed, use_graph=graph)
+    assert pipe.device_discovery()
+    for rep in range(3 if graph else 1):
+        pipe.run()
+        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0))
+        # eager work between replays
+        hip_ctx_tmp = fa.HipCodec(hip_ctx).inflate_batch_device(s.compressed[:65536 * 2], fa.discover(
+            s.compressed[:200000])[0][:1])
+        del hip_ctx_tmp
+    if not graph:
+        ms = pipe.stage_ms()
+        assert ms["total"] > 0 and ms["inflate"] > 0
+
+
+@pytest.mark.parametrize("name", ["htslib.bam", "htslib_multiblock.bam", "ont_ultralong.bam", "pacbio_hifi.bam"])
+def test_pipeline_fixtures(hip_ctx, oracle_mod, name):
+    import fritillaria_amd as fa
+    data = golden_bytes(name)
+    pipe = fa.BatchedPipeline(hip_ctx, data)
+    pipe.run()
+    check_pipeline(pipe, oracle_mod, data)
+
+
+@pytest.mark.parametrize("opts", [dict(order=1), dict(waves_per_cu=1), dict(waves_per_cu=8), dict(bam_window=256),
+                                  dict(host_discovery=True)])
+def test_schedules_do_not_change_results(hip_ctx, oracle_mod, opts):
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(2, n_records=40000)
+    pipe = fa.BatchedPipeline(hip_ctx, s.compressed, **opts)
+    pipe.run()
+    check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=False)
+
+
+def test_false_header_in_payload_falls_back_to_host_walk(hip_ctx, oracle_mod):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(41)
+    # a stored member whose payload contains a whole BGZF member: the sieve sees
+    # a header that is not on the member chain
+    inner = bgzf_member(b"hello" * 100)
+    data = bam_header() + b"".join(random_records(rng, 200))
+    blob = bgzf(data, cap=9000, eof=False) + bgzf_member(inner, level=0) + bgzf(b"", eof=True)
+    pipe = fa.BatchedPipeline(hip_ctx, blob)
+    assert not pipe.device_discovery()
+    pipe.run()
+    assert pipe.members() == len(oracle_mod.discover(blob).spans)
+
+
+def test_pipeline_reports_a_bad_crc_like_the_per_call_path(hip_ctx):
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(42)
+    data = bam_header() + b"".join(random_records(rng, 300))
+    good = bgzf(data, cap=20000, eof=False)
+    bad = bgzf_member(b"x" * 100, crc=1)
+    blob = good + bad + bgzf(b"", eof=True)
+    pipe = fa.BatchedPipeline(hip_ctx, blob)
+    with pytest.raises(fa.ChecksumMismatch) as ei:
+        pipe.run()
+    assert ei.value.offset == len(good) and ei.value.expected == 1
