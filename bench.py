@@ -32,6 +32,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(records, schedule):
+    """HBM bytes per inflate launch for this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/fr_inflate_kernel_traffic_*.json),
+    each scaled by the same counter's reading for a plain copy of known size.
+    None when no pass was taken on this record count and inflate variant."""
+    path = os.path.join(ROOT, "profiles", f"fr_inflate_kernel_traffic_config2_{records}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("inflate_schedule") != schedule:
+        return None
+    return int(t["traffic_bytes_per_launch"])
+
+
 def per_call_pass(fa, ctx, compressed, blocks_per_batch=256, max_batches=None):
     """The drop-in loop of crates/fritillaria-cuda/tests/bam_reader.rs:76-115."""
     codec = fa.HipCodec(ctx)
@@ -80,6 +95,8 @@ def main():
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--schedule", type=int, default=-1, help="inflate variant; -1 = autotune")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,12 +143,17 @@ def main():
     gbps = U / (dt / args.steps) / 1e9
     n_rec = pipe.records()
     log(f"batched: {ms_per_step:.2f} ms/step, {gbps:.2f} GB/s inflated, {n_rec / (dt / args.steps) / 1e6:.1f} M records/s")
-    cols = pipe.columns_to_host() if False else None
-    # checksums of the batched columns, compared with the per-call pass below
+    # the bytes the batched path left in HBM, re-checked member by member
+    # against the trailer CRC32s (the kernel's own CRC is taken on chip)
+    offs, _ = pipe.member_offsets()
+    spans, _ = fa.discover(s.compressed)
+    crc_written = fa.HipCodec(ctx).crc32_blocks(pipe.inflated(), offs[:-1], np.diff(offs).astype(np.uint32))
+    bad_members = int(np.count_nonzero(crc_written != spans["crc32"][: len(crc_written)]))
+    log(f"inflated bytes in HBM re-checked: {bad_members} member CRC mismatches")
     pos = pipe.column_to_host("position")
     lseq = pipe.column_to_host("sequence_len")
     batched_sig = (n_rec, int(pos.astype(np.int64).sum()), int(lseq.astype(np.int64).sum()))
-    del pos, lseq, cols
+    del pos, lseq, spans, crc_written
 
     # inflate-kernel roofline: one run without a graph, HIP events around the launch
     timed = fa.BatchedPipeline(ctx, s.compressed, use_graph=False, waves_per_cu=args.waves_per_cu,
@@ -179,6 +201,21 @@ def main():
                "records_match": r["records"] == n_rec}
         log(f"cpu baseline: {cpu['value']:.2f} GB/s on {cores} threads ({cdt:.2f} s)")
 
+    traffic = measured_traffic(args.records, args.schedule)
+    if traffic is not None:
+        log(f"inflate HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic} bytes per launch")
+
+    if args.copy_calibration:
+        import torch
+        # an elementwise kernel that reads and writes every byte once (a plain
+        # device-to-device copy may go to a DMA engine, which the counters miss)
+        src = torch.zeros(args.copy_calibration, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.bitwise_not(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"calibration kernel over {args.copy_calibration} bytes done")
+        del src, dst
+
     out = {
         "metric": METRIC,
         "value": round(gbps, 3),
@@ -207,7 +244,7 @@ def main():
         "batched_over_per_call": round(gbps / per_call["gbps"], 3) if per_call else None,
         "stage_ms": {k: round(v, 3) for k, v in stages.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
                      "kernel": "fr_inflate_kernel", "algorithmic_bytes_per_launch": alg_bytes,
                      "launch_ms": round(inflate_ms, 3)},
         "pipeline_roofline": {"algorithmic_bytes": pipe_bytes,
@@ -216,6 +253,7 @@ def main():
         "cpu_baseline": cpu,
         "p99_latency_ms": None,
         "batched_signature": list(batched_sig),
+        "inflated_crc_recheck_mismatches": bad_members,
     }
     print(json.dumps(out), flush=True)
 

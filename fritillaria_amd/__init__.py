@@ -1,6 +1,7 @@
 """fritillaria_amd -- MI355X (gfx950) device path of fritillaria.
 
 BGZF inflate + CRC32/ISIZE verification, BAM record boundaries and SoA columns,
+FASTQ record boundaries and line-bound columns,
 as hand-written HIP kernels behind the C ABI of ``include/frhip.h``. This
 package is the binding; the native library is ``libfrhip.so``.
 """
@@ -8,5 +9,6 @@ from .hip import (  # noqa: F401
     BAM_COLUMNS, Backend, BamDecoder, ChecksumMismatch, DeviceBatch, DeviceBgzfReader, DeviceBuffer,
     DeviceInflateBatch, DeviceRecordBatch, FrError, HipCodec, HipContext, HipError, HipUnavailable,
     InflateBatch, InflateError, InvalidBlock, InvalidDeviceBatch, Malformed, SizeMismatch, SPAN_DTYPE,
-    BatchedPipeline, bam_parse_header, crc32_combine, device_is_available, discover, select_codec,
+    BatchedPipeline, FASTQ_COLUMNS, FastqRecordBatch, FastqScanner, SpeculativeScan, bam_parse_header,
+    crc32_combine, device_is_available, discover, select_codec,
 )

@@ -221,6 +221,8 @@ void ctx_release(frhip_ctx *c)
     c->meta_scratch.release();
     c->status_scratch.release();
     c->bam_scratch.release();
+    c->fq_small.release();
+    c->fq_big.release();
     c->pinned.release();
     if (c->x2n) (void)hipFree(c->x2n);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

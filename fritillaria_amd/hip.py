@@ -155,6 +155,12 @@ def lib():
         "frhip_bam_batch_column": (_vp, [_vp, C.c_int]),
         "frhip_bam_batch_free": (None, [_vp]),
         "frhip_bam_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32), _ep]),
+        "frhip_fastq_decode": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_fastq_batch_len": (_u64, [_vp]),
+        "frhip_fastq_batch_tail": (_u64, [_vp]),
+        "frhip_fastq_batch_survivors": (_u64, [_vp]),
+        "frhip_fastq_batch_column": (_vp, [_vp, C.c_int]),
+        "frhip_fastq_batch_free": (None, [_vp]),
         "frhip_reader_open": (C.c_int, [_vp, _vp, _u64, _u64, C.POINTER(_vp), _ep]),
         "frhip_reader_next_batch": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(_vp),
                                               C.POINTER(C.POINTER(_u64)), C.POINTER(C.POINTER(_u64)),
@@ -532,6 +538,98 @@ class BamDecoder:
                                     C.byref(out), C.byref(e))
         _raise(rc, e)
         return DeviceRecordBatch(out, batch)
+
+
+# ---------------------------------------------------------------------------
+# FASTQ
+# ---------------------------------------------------------------------------
+
+# (name, dtype) in the order of crates/fritillaria-fastq/src/columnar/device.rs:44-68
+FASTQ_COLUMNS = [
+    ("record_offsets", np.uint64), ("sequence_start", np.uint32), ("plus_start", np.uint32),
+    ("quality_start", np.uint32), ("sequence_len", np.uint32), ("record_end", np.uint32),
+]
+_FQ_INDEX = {n: i for i, (n, _) in enumerate(FASTQ_COLUMNS)}
+
+
+class FastqRecordBatch:
+    """FASTQ columns in device memory (crates/fritillaria-fastq/src/columnar/device.rs:84-160).
+    Bounds index the inflated batch, which this object keeps alive."""
+
+    def __init__(self, handle, source: DeviceInflateBatch):
+        self._h = handle
+        self._source = source
+
+    def __len__(self) -> int:
+        return int(lib().frhip_fastq_batch_len(self._h))
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def tail(self) -> int:
+        return int(lib().frhip_fastq_batch_tail(self._h))
+
+    def survivors(self) -> int:
+        return int(lib().frhip_fastq_batch_survivors(self._h))
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_fastq_batch_column(self._h, _FQ_INDEX[name]), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(FASTQ_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def to_host(self) -> dict:
+        return {name: self.column_to_host(name) for name, _ in FASTQ_COLUMNS}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_fastq_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class SpeculativeScan:
+    """crates/fritillaria-fastq/src/columnar/speculative.rs:81-93. `proof` is
+    "tiled" when every line start that begins a record is a record of the walk
+    (survivors == records), else "walked"; the offsets are the walk's either way."""
+
+    def __init__(self, offsets, tail, survivors, proof):
+        self.offsets = offsets
+        self.tail = tail
+        self.candidates = 0
+        self.survivors = survivors
+        self.proof = proof
+
+
+class FastqScanner:
+    """FastqScanner (crates/fritillaria-cuda/src/fastq.rs:48-135)."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def decode(self, batch: DeviceInflateBatch, start: int) -> FastqRecordBatch:
+        data = batch.data()
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_fastq_decode(self.ctx.handle, None if data is None else data._h, start, C.byref(out),
+                                      C.byref(e))
+        _raise(rc, e)
+        return FastqRecordBatch(out, batch)
+
+    def scan(self, batch: DeviceInflateBatch, start: int) -> SpeculativeScan:
+        d = self.decode(batch, start)
+        offs = d.column_to_host("record_offsets")
+        n, surv = len(d), d.survivors()
+        return SpeculativeScan(offs, d.tail(), surv, "tiled" if surv == n else "walked")
 
 
 def bam_parse_header(buf):

@@ -144,3 +144,46 @@ def config_bam(config: int, n_records=None, threads=None) -> SynthBam:
     else:
         raise ValueError(f"no BAM configuration {config}")
     return SynthBam(c, int(u.size), rs, n)
+
+
+@dataclass
+class SynthFastq:
+    compressed: np.ndarray
+    inflated: np.ndarray
+    n_records: int
+
+
+def fastq_stream(n_records: int, seed: int, read_len: int = 150) -> np.ndarray:
+    """Uncompressed FASTQ: names syn%010d, bases uniform ACGT, qualities
+    round(N(35, 5)) clipped to [2, 41], Phred+33, an empty '+' line."""
+    rng = np.random.default_rng(seed)
+    name_len = 1 + 3 + 10 + 1
+    width = name_len + (read_len + 1) + 2 + (read_len + 1)
+    out = np.empty((n_records, width), dtype=np.uint8)
+    out[:, 0] = ord("@")
+    out[:, 1:4] = np.frombuffer(b"syn", dtype=np.uint8)
+    idx = np.arange(n_records, dtype=np.int64)
+    for d in range(10):
+        out[:, 13 - d] = ord("0") + (idx // 10 ** d) % 10
+    out[:, 14] = ord("\n")
+    s0 = name_len
+    out[:, s0:s0 + read_len] = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, (n_records, read_len))]
+    out[:, s0 + read_len] = ord("\n")
+    p0 = s0 + read_len + 1
+    out[:, p0] = ord("+")
+    out[:, p0 + 1] = ord("\n")
+    q0 = p0 + 2
+    q = np.clip(np.rint(rng.normal(35.0, 5.0, (n_records, read_len))), 2, 41).astype(np.uint8) + 33
+    out[:, q0:q0 + read_len] = q
+    out[:, q0 + read_len] = ord("\n")
+    return out.reshape(-1)
+
+
+def config_fastq(n_records=None, threads=None) -> SynthFastq:
+    """FASTQ half of the mixed BCF + FASTQ configuration: 5,000,000 x 150 bp
+    records, bgzip framing (65,280-byte payloads, level 6); `n_records` scales it
+    down for tests."""
+    n = n_records or 5_000_000
+    u = fastq_stream(n, CONFIG_SEED + 4)
+    c = bgzf(u, level=6, payload_cap=65280, seed=CONFIG_SEED + 4, threads=threads)
+    return SynthFastq(c, u, n)

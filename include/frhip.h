@@ -57,6 +57,7 @@ typedef struct {
 typedef struct frhip_ctx frhip_ctx;
 typedef struct frhip_buf frhip_buf;
 typedef struct frhip_bam_batch frhip_bam_batch;
+typedef struct frhip_fastq_batch frhip_fastq_batch;
 typedef struct frhip_reader frhip_reader;
 
 /* ---- device and context ------------------------------------------------ */
@@ -179,6 +180,40 @@ void frhip_bam_batch_free(frhip_bam_batch *b);
 /* BAM header (crates/fritillaria-bam/src/columnar/header.rs:43-103) on host bytes. */
 int frhip_bam_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start,
                            uint32_t *n_ref, frhip_error *err);
+
+/* ---- FASTQ: FastqScanner::{scan, decode} (crates/fritillaria-cuda/src/fastq.rs:94-135, 469-560) ---- */
+
+/* Column indices of the FASTQ DeviceRecordBatch, in the order of
+   crates/fritillaria-fastq/src/columnar/device.rs:44-68. Bounds are relative
+   to the record start; the definition line is record_offsets + 1 ..
+   sequence_start - 1. */
+typedef enum {
+    FRHIP_FASTQ_RECORD_OFFSETS = 0, /* u64, into the inflated batch */
+    FRHIP_FASTQ_SEQUENCE_START = 1, /* u32 */
+    FRHIP_FASTQ_PLUS_START = 2,     /* u32 */
+    FRHIP_FASTQ_QUALITY_START = 3,  /* u32 */
+    FRHIP_FASTQ_SEQUENCE_LEN = 4,   /* u32, also the quality length */
+    FRHIP_FASTQ_RECORD_END = 5,     /* u32 */
+    FRHIP_FASTQ_N_COLUMNS = 6
+} frhip_fastq_column;
+
+/* Record boundaries from `start` and the line-bound columns of an inflated
+   batch (NULL or empty, or start >= its length: no records, tail =
+   min(start, length), as fastq.rs:469-479). The records are those of the
+   serial walk of crates/fritillaria-fastq/src/columnar/record.rs:145-176:
+   newlines between records are skipped and the first position that is not a
+   whole record is the tail when it holds '@'; any other byte there is
+   FRHIP_E_MALFORMED (format "fastq", offset = that position). */
+int frhip_fastq_decode(frhip_ctx *ctx, const frhip_buf *data, uint64_t start, frhip_fastq_batch **out,
+                       frhip_error *err);
+uint64_t frhip_fastq_batch_len(const frhip_fastq_batch *b);
+uint64_t frhip_fastq_batch_tail(const frhip_fastq_batch *b);
+/* Line starts that begin a whole record (SpeculativeScan::survivors,
+   crates/fritillaria-fastq/src/columnar/speculative.rs:86-92); equal to the
+   record count exactly when the records tile the batch (Proof::Tiled). */
+uint64_t frhip_fastq_batch_survivors(const frhip_fastq_batch *b);
+const frhip_buf *frhip_fastq_batch_column(const frhip_fastq_batch *b, int column);
+void frhip_fastq_batch_free(frhip_fastq_batch *b);
 
 /* ---- DeviceBgzfReader (crates/fritillaria-bgzf/src/device_read.rs:47-280) ---- */
 

@@ -83,6 +83,12 @@ def lib():
         _lib.or_bam_decode_columns.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_void_p] * 14
         _lib.or_virtual_offset.restype = C.c_uint64
         _lib.or_virtual_offset.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]
+        _lib.or_fastq_bounds.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]
+        _lib.or_fastq_scan_records.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+                                               C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_uint64)]
+        _lib.or_fastq_survivors.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        _lib.or_fastq_survivors.restype = C.c_uint64
         _lib.or_mt_bam_columns.restype = C.c_int
         _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
         del u8p
@@ -176,6 +182,38 @@ def bam_scan_records(buf, start: int):
     rc = lib().or_bam_scan_records(_ptr(a), a.size, start, _ptr(offs), cap, C.byref(n),
                                    C.byref(tail))
     return rc, offs[:n.value].copy(), tail.value
+
+
+FASTQ_BOUNDS = ("sequence_start", "plus_start", "quality_start", "sequence_len", "record_end")
+
+
+def fastq_bounds(buf, pos: int):
+    """Line bounds of a record at pos (crates/fritillaria-fastq/src/columnar/record.rs:85-131),
+    as a 5-tuple in FASTQ_BOUNDS order, or None."""
+    a = _as_u8(buf)
+    b = np.zeros(5, dtype=np.uint32)
+    if not lib().or_fastq_bounds(_ptr(a), a.size, pos, _ptr(b)):
+        return None
+    return tuple(int(x) for x in b)
+
+
+def fastq_scan_records(buf, start: int):
+    """Serial walk of record.rs:145-176. Returns (rc, offsets u64[n], bounds u32[n, 5], tail, err_pos);
+    rc is 0 or E_MALFORMED (a non-'@', non-newline byte where a record must start)."""
+    a = _as_u8(buf)
+    cap = max(1, a.size // 6 + 1)
+    offs = np.zeros(cap, dtype=np.uint64)
+    bounds = np.zeros((cap, 5), dtype=np.uint32)
+    n, tail, err = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+    rc = lib().or_fastq_scan_records(_ptr(a), a.size, start, _ptr(offs), _ptr(bounds), cap, C.byref(n),
+                                     C.byref(tail), C.byref(err))
+    k = n.value
+    return rc, offs[:k].copy(), bounds[:k].copy(), tail.value, err.value
+
+
+def fastq_survivors(buf, start: int) -> int:
+    a = _as_u8(buf)
+    return int(lib().or_fastq_survivors(_ptr(a), a.size, start))
 
 
 def bam_scan_blocked(buf, block_starts, start: int):

@@ -21,6 +21,14 @@
  *                       blocked scan with the plain chain walk over every
  *                       blocking (crates/fritillaria-bam/src/columnar/blocked.rs:306-319).
  *   or_virtual_offset   noodles vectors (crates/fritillaria-bgzf/src/io/block.rs:61-77).
+ *   or_fastq_*          fixture record counts and read-length ranges
+ *                       (testdata/README.md:319-322); the FASTQ fixtures are
+ *                       `samtools fastq` of the BAM fixtures, so each record's
+ *                       name, bases and qualities must equal the BAM record's
+ *                       (reverse-complemented for flag 0x10), decoded by the
+ *                       independent BAM oracle; the reference's hand-built
+ *                       decoy, seam and short-quality cases
+ *                       (crates/fritillaria-fastq/src/columnar/record.rs:291-468).
  */
 #include "oracle.h"
 
@@ -593,4 +601,97 @@ uint64_t or_virtual_offset(const uint64_t *block_offsets, const uint64_t *coffse
     uint64_t b = lo - 1;
     if (b >= nb) b = nb - 1; /* p == total: end of the last member */
     return (coffsets[b] << 16) | (p - block_offsets[b]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FASTQ record boundaries and line bounds.                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Index of the next '\n' at or after pos, or len when there is none. */
+static uint64_t fq_line_end(const uint8_t *buf, uint64_t len, uint64_t pos)
+{
+    if (pos >= len) return len;
+    const uint8_t *p = (const uint8_t *)memchr(buf + pos, '\n', (size_t)(len - pos));
+    return p ? (uint64_t)(p - buf) : len;
+}
+
+/*
+ * The four line bounds of a record at pos, relative to pos, or 0 when pos does
+ * not begin a whole record. crates/fritillaria-fastq/src/columnar/record.rs:85-131:
+ * '@' at pos; newlines end the definition, sequence and '+' lines; '+' opens
+ * line 3; the quality line runs to the next newline or the buffer end and must
+ * be as long as the sequence line; the record ends past the quality newline
+ * when there is one.
+ */
+int or_fastq_bounds(const uint8_t *buf, uint64_t len, uint64_t pos, uint32_t b[5])
+{
+    if (pos >= len || buf[pos] != '@') return 0;
+    uint64_t def_end = fq_line_end(buf, len, pos);
+    if (def_end >= len) return 0;
+    uint64_t seq_start = def_end + 1;
+    uint64_t seq_end = fq_line_end(buf, len, seq_start);
+    if (seq_end >= len) return 0;
+    uint64_t plus_start = seq_end + 1;
+    if (plus_start >= len || buf[plus_start] != '+') return 0;
+    uint64_t plus_end = fq_line_end(buf, len, plus_start);
+    if (plus_end >= len) return 0;
+    uint64_t qual_start = plus_end + 1;
+    uint64_t seq_len = seq_end - seq_start;
+    uint64_t qual_end = fq_line_end(buf, len, qual_start);
+    if (qual_end - qual_start != seq_len) return 0;
+    uint64_t end = qual_end < len ? qual_end + 1 : qual_end;
+    b[0] = (uint32_t)(seq_start - pos);
+    b[1] = (uint32_t)(plus_start - pos);
+    b[2] = (uint32_t)(qual_start - pos);
+    b[3] = (uint32_t)seq_len;
+    b[4] = (uint32_t)(end - pos);
+    return 1;
+}
+
+/*
+ * Serial record walk, crates/fritillaria-fastq/src/columnar/record.rs:145-176:
+ * newlines between records are skipped; a position that is not a whole record
+ * ends the walk (the tail) when it holds '@', and is a Malformed error
+ * (err_pos = that position) otherwise. Returns 0 or OR_E_MALFORMED.
+ * bounds (5 per record) may be NULL.
+ */
+int or_fastq_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets,
+                          uint32_t *bounds, uint64_t cap, uint64_t *n, uint64_t *tail, uint64_t *err_pos)
+{
+    uint64_t pos = start, k = 0;
+    *n = 0;
+    while (pos < len) {
+        if (buf[pos] == '\n') {
+            ++pos;
+            continue;
+        }
+        uint32_t b[5];
+        if (!or_fastq_bounds(buf, len, pos, b)) {
+            if (buf[pos] == '@') break;
+            *err_pos = pos;
+            *tail = pos;
+            return OR_E_MALFORMED;
+        }
+        if (k < cap) {
+            offsets[k] = pos;
+            if (bounds) memcpy(bounds + 5 * k, b, sizeof b);
+        }
+        ++k;
+        pos += b[4];
+    }
+    *n = k;
+    *tail = pos;
+    return 0;
+}
+
+/* Number of line starts at or after start (start itself and every byte after
+   a '\n' below len) that begin a whole record: the survivors of the sieve of
+   crates/fritillaria-fastq/src/columnar/speculative.rs:96-126. */
+uint64_t or_fastq_survivors(const uint8_t *buf, uint64_t len, uint64_t start)
+{
+    uint64_t s = 0;
+    uint32_t b[5];
+    for (uint64_t i = start; i < len; ++i)
+        if ((i == start || buf[i - 1] == '\n') && or_fastq_bounds(buf, len, i, b)) ++s;
+    return s;
 }

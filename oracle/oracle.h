@@ -86,6 +86,10 @@ void or_bam_decode_columns(const uint8_t *buf, const uint64_t *offsets, uint64_t
                            uint32_t *qual_start, uint32_t *aux_start, uint32_t *rec_end);
 uint64_t or_virtual_offset(const uint64_t *block_offsets, const uint64_t *coffsets, uint64_t nb,
                            uint64_t p);
+int or_fastq_bounds(const uint8_t *buf, uint64_t len, uint64_t pos, uint32_t b[5]);
+int or_fastq_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets,
+                          uint32_t *bounds, uint64_t cap, uint64_t *n, uint64_t *tail, uint64_t *err_pos);
+uint64_t or_fastq_survivors(const uint8_t *buf, uint64_t len, uint64_t start);
 
 #ifdef __cplusplus
 }

@@ -233,3 +233,20 @@ def test_columns_report_their_device(codec, decoder):
     start, _ = fa.bam_parse_header(u)
     r = decoder.decode(batch, start)
     assert r.device_ordinal() == batch.device_ordinal() == 0
+
+
+def test_speculation_holds_past_65536_members(codec, decoder, oracle_mod):
+    """One batch of ~100k small members: the speculation must resolve members
+    past the 65,536th like the first ones (no fallback walks)."""
+    print("[gpu] bam many small members", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    u, rs = synth.bam_stream(300_000, synth.CONFIG_SEED + 2, read_range=(100, 250), indel_p=0.01,
+                             ref_len=250_000_000)
+    c = synth.bgzf(u, level=1, payload_cap=1024)
+    spans, _ = fa.discover(c)
+    assert len(spans) > 70_000
+    batch = codec.inflate_batch_device(c, spans)
+    records = decoder.decode(batch, rs)
+    check_batch(oracle_mod, records, batch.data().to_vec(), batch.offsets(), batch.compressed_offsets(), rs)
+    assert records.fallbacks() <= 2 + len(spans) // 50, records.fallbacks()
