@@ -47,6 +47,43 @@ def measured_traffic(records, schedule):
     return int(t["traffic_bytes_per_launch"])
 
 
+def serve_pass(fa, ctx, compressed, voffs, n_requests, log):
+    """Range queries over the same BAM resident in HBM: log-uniform 64 KiB - 256 MiB
+    compressed extents snapped to record starts, Poisson arrivals. Capacity is
+    measured first with every request arriving at once; then arrival rates at 30%,
+    60% and 90% of that capacity are served with continuous batching (admission
+    by free HBM) and, at 60%, by the plain per-call path for comparison."""
+    from fritillaria_amd import serve as sv
+    server = fa.QueryServer(ctx, compressed)
+    co, _ = server.member_table()
+    free, total = fa.device_memory(ctx)
+    budget = int(min(free * 0.4, 48 << 30))
+    burst = sv.make_requests(voffs, co, n_requests, rate_hz=1e9, seed=0x5EED0005)
+    st = sv.serve(server, burst, budget)
+    cap = st.summary()
+    bytes_per_req = st.inflated_bytes / max(1, len(burst))
+    req_per_s = len(burst) / st.makespan_s
+    log(f"serving capacity: {cap['throughput_gbps']} GB/s, {req_per_s:.1f} requests/s "
+        f"({bytes_per_req / 1e6:.1f} MB inflated per request, mean batch {cap['mean_batch']})")
+    out = {"workload": "range queries over the batched large BAM: log-uniform 64 KiB-256 MiB compressed extents, "
+                       "record-start virtual offsets, Poisson arrivals",
+           "hbm_budget_bytes": budget, "capacity": cap, "loads": {}}
+    for load in (0.3, 0.6, 0.9):
+        reqs = sv.make_requests(voffs, co, n_requests, rate_hz=load * req_per_s, seed=0x5EED0005 + int(load * 10))
+        r = sv.serve(server, reqs, budget).summary()
+        r["rate_hz"] = round(load * req_per_s, 2)
+        out["loads"][str(load)] = r
+        log(f"serving at {load:.0%} load: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, {r['throughput_gbps']} GB/s, "
+            f"mean batch {r['mean_batch']}")
+    server.close()
+    reqs = sv.make_requests(voffs, co, max(20, n_requests // 4), rate_hz=0.6 * req_per_s, seed=0x5EED0005 + 6)
+    plain = sv.serve_plain(ctx, compressed, reqs, co).summary()
+    plain["rate_hz"] = round(0.6 * req_per_s, 2)
+    out["plain_per_call_at_0.6"] = plain
+    log(f"plain per-call at 60% of batched capacity: p50 {plain['p50_ms']} ms, p99 {plain['p99_ms']} ms")
+    return out
+
+
 def per_call_pass(fa, ctx, compressed, blocks_per_batch=256, max_batches=None):
     """The drop-in loop of crates/fritillaria-cuda/tests/bam_reader.rs:76-115."""
     codec = fa.HipCodec(ctx)
@@ -95,6 +132,8 @@ def main():
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--schedule", type=int, default=-1, help="inflate variant; -1 = autotune")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (range query) measurement")
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -153,6 +192,7 @@ def main():
     pos = pipe.column_to_host("position")
     lseq = pipe.column_to_host("sequence_len")
     batched_sig = (n_rec, int(pos.astype(np.int64).sum()), int(lseq.astype(np.int64).sum()))
+    voffs = pipe.column_to_host("virtual_offsets") if not args.no_serve else None
     del pos, lseq, spans, crc_written
 
     # inflate-kernel roofline: one run without a graph, HIP events around the launch
@@ -186,6 +226,11 @@ def main():
                     "blocks_per_batch": 256, "host_fed": True, "matches_batched_records": pc_rec == n_rec}
         log(f"per-call: {pdt:.2f} s, {per_call['gbps']:.2f} GB/s, {pc_rec} records in {pc_batches} batches")
 
+    serving = None
+    if not args.no_serve:
+        serving = serve_pass(fa, ctx, s.compressed, voffs, args.serve_requests, log)
+        del voffs
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -206,14 +251,11 @@ def main():
         log(f"inflate HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic} bytes per launch")
 
     if args.copy_calibration:
-        import torch
-        # an elementwise kernel that reads and writes every byte once (a plain
-        # device-to-device copy may go to a DMA engine, which the counters miss)
-        src = torch.zeros(args.copy_calibration, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.bitwise_not(src, out=dst)
-        torch.cuda.synchronize()
-        log(f"calibration kernel over {args.copy_calibration} bytes done")
+        # the library's plain copy kernel: reads and writes every byte once
+        src = ctx.alloc(args.copy_calibration)
+        dst = ctx.alloc(args.copy_calibration)
+        ctx.copy_device(dst, src)
+        log(f"calibration copy kernel over {args.copy_calibration} bytes done")
         del src, dst
 
     out = {
@@ -251,7 +293,9 @@ def main():
                               "achieved_gbps": round(pipe_bytes / (ms_per_step / 1e3) / 1e9, 2),
                               "frac": round(pipe_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)},
         "cpu_baseline": cpu,
-        "p99_latency_ms": None,
+        "p99_latency_ms": serving["loads"]["0.6"]["p99_ms"] if serving else None,
+        "p50_latency_ms": serving["loads"]["0.6"]["p50_ms"] if serving else None,
+        "serving": serving,
         "batched_signature": list(batched_sig),
         "inflated_crc_recheck_mismatches": bad_members,
     }

@@ -10,5 +10,6 @@ from .hip import (  # noqa: F401
     DeviceInflateBatch, DeviceRecordBatch, FrError, HipCodec, HipContext, HipError, HipUnavailable,
     InflateBatch, InflateError, InvalidBlock, InvalidDeviceBatch, Malformed, SizeMismatch, SPAN_DTYPE,
     BatchedPipeline, FASTQ_COLUMNS, FastqRecordBatch, FastqScanner, SpeculativeScan, bam_parse_header,
-    crc32_combine, device_is_available, discover, select_codec,
+    crc32_combine, device_is_available, discover, select_codec, QueryServer, QUERY_RESULT_DTYPE, TO_END,
+    device_memory,
 )

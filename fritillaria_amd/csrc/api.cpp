@@ -18,6 +18,7 @@
 #include "crc_math.h"
 #include "frhip_internal.h"
 #include "host_internal.h"
+#include "pipeline_internal.h"
 
 using namespace frhip;
 
@@ -289,6 +290,25 @@ extern "C" int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t ds
 }
 
 extern "C" void frhip_buf_free(frhip_buf *b) { buf_free(b); }
+
+extern "C" int frhip_buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_buf **out, frhip_error *err)
+{
+    (void)hipSetDevice(ctx->ordinal);
+    *out = buf_alloc(ctx, len, err);
+    return *out ? FRHIP_OK : (err ? err->kind : FRHIP_E_DEVICE);
+}
+
+extern "C" int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip_buf *src, frhip_error *err)
+{
+    if (!dst || !src || dst->len != src->len)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "copy between buffers of different lengths");
+    if (dst->ordinal != ctx->ordinal || src->ordinal != ctx->ordinal)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "buffers are not on the context's device");
+    (void)hipSetDevice(ctx->ordinal);
+    hipError_t r = launch_copy((uint8_t *)dst->ptr, (const uint8_t *)src->ptr, src->len, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "device copy");
+}
 
 // ------------------------------------------------------------------------
 // discovery (host)

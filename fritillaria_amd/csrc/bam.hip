@@ -29,8 +29,11 @@ enum : uint64_t { BAM_OK = 0, BAM_MALFORMED = 1 };
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uni64(uint64_t v)
 {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)v);
+    // the builtin returns int: each half is narrowed to uint32_t before it is
+    // widened, or the low half would be sign-extended over the high one
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t ld32(const uint8_t *p)
@@ -119,7 +122,9 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t b = blockIdx.x;
-    const uint64_t sb = a.block_starts[b], sb1 = a.block_starts[b + 1];
+    const uint64_t sb = a.block_starts[b];
+    const uint64_t sb1 = a.mem_until ? a.mem_until[b] : a.block_starts[b + 1];
+    const uint64_t len = a.mem_len ? a.mem_len[b] : a.len;
     if (sb >= sb1) {
         if (lane == 0) {
             a.spec_ok[b] = 0;
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
         if (p < T) {
             alive = 1;
             for (;;) {
-                Step s = cand_step(a.buf, a.len, cur, T);
+                Step s = cand_step(a.buf, len, cur, T);
                 if (s.kind == 1) break;
                 if (s.kind == 3) { alive = 0; break; }
                 ++cnt;
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
     uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
     if (complete) {
         for (;;) {
-            Step s = step(a.buf, a.len, cur, sb1);
+            Step s = step(a.buf, len, cur, sb1);
             if (s.kind == 1) break;
             if (s.kind == 2) { complete = 0; break; }
             if (s.kind == 3) { valid = 0; break; }
@@ -227,14 +232,27 @@ __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t j)
 __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
 {
     const uint32_t lane = threadIdx.x;
-    if (a.len_dev) a.len = *a.len_dev;
-    uint64_t cur = a.start_dev ? *a.start_dev : a.start, emitted = 0, tail = a.len, fallbacks = 0;
+    uint32_t mb0 = 0, mb1 = a.nb;
+    uint64_t *totals = a.totals;
+    uint64_t cur;
+    if (a.seg_mb) {
+        const uint32_t q = blockIdx.x;
+        mb0 = a.seg_mb[q];
+        mb1 = a.seg_mb[q + 1];
+        a.len = a.seg_len[q];
+        cur = a.seg_start[q];
+        totals = a.seg_totals + 4ull * q;
+    } else {
+        if (a.len_dev) a.len = *a.len_dev;
+        cur = a.start_dev ? *a.start_dev : a.start;
+    }
+    uint64_t emitted = 0, tail = a.len, fallbacks = 0;
     int stopped = 0;
     uint64_t status = BAM_OK;
-    for (uint32_t base = 0; base < a.nb; base += 64) {
+    for (uint32_t base = mb0; base < mb1; base += 64) {
         const uint32_t b = base + lane;
-        const bool in = b < a.nb;
-        uint64_t s1 = in ? a.block_starts[b + 1] : 0;
+        const bool in = b < mb1;
+        uint64_t s1 = in ? (a.mem_until ? a.mem_until[b] : a.block_starts[b + 1]) : 0;
         uint32_t ok = in ? a.spec_ok[b] : 0;
         uint64_t X = in && ok ? a.exitp[b] : 0;
         uint32_t comp = in && ok ? a.complete[b] : 0;
@@ -255,7 +273,7 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
         }
         uint64_t o_entry = BAM_NONE, o_first = 0;
         uint32_t o_cnt = 0, o_mode = 0;
-        uint32_t lim = a.nb - base < 64 ? a.nb - base : 64;
+        uint32_t lim = mb1 - base < 64 ? mb1 - base : 64;
         for (uint32_t j = 0; j < lim; ++j) {
             uint64_t js1 = rl64(s1, j);
             if (stopped || cur >= js1) {
@@ -320,11 +338,22 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
     }
     if (!stopped) tail = cur;
     if (lane == 0) {
-        a.totals[0] = emitted;
-        a.totals[1] = tail;
-        a.totals[2] = status;
-        a.totals[3] = fallbacks;
+        totals[0] = emitted;
+        totals[1] = tail;
+        totals[2] = status;
+        totals[3] = fallbacks;
     }
+}
+
+__global__ void bam_segment_bases_kernel(const uint64_t *seg_totals, uint32_t n_seg, uint64_t *seg_base)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t run = 0;
+    for (uint32_t q = 0; q < n_seg; ++q) {
+        seg_base[q] = run;
+        run += seg_totals[4ull * q];
+    }
+    seg_base[n_seg] = run;
 }
 
 __global__ __launch_bounds__(64) void bam_emit_kernel(BamEmitArgs a)
@@ -334,7 +363,7 @@ __global__ __launch_bounds__(64) void bam_emit_kernel(BamEmitArgs a)
     const uint64_t e = a.entry[b];
     if (e == BAM_NONE) return;
     const uint32_t cnt = a.count[b];
-    const uint64_t first = a.first_index[b];
+    const uint64_t first = a.first_index[b] + (a.seg_base ? a.seg_base[a.mem_seg[b]] : 0);
     const uint32_t mode = a.mode[b];
     const uint64_t sb = a.block_starts[b];
     const uint64_t vbase = a.coffsets ? (a.coffsets[b] << 16) : 0;
@@ -400,6 +429,19 @@ hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(bam_reconcile_kernel, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_reconcile_segments(const BamReconcileArgs &a, uint32_t n_seg, hipStream_t s)
+{
+    if (n_seg == 0) return hipSuccess;
+    hipLaunchKernelGGL(bam_reconcile_kernel, dim3(n_seg), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bam_segment_bases(const uint64_t *seg_totals, uint32_t n_seg, uint64_t *seg_base, hipStream_t s)
+{
+    hipLaunchKernelGGL(bam_segment_bases_kernel, dim3(1), dim3(64), 0, s, seg_totals, n_seg, seg_base);
     return hipGetLastError();
 }
 

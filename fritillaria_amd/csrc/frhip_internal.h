@@ -58,6 +58,11 @@ struct BamSpecArgs {
     uint32_t *cnt_tail;
     uint8_t *complete;
     uint16_t *rel_offs;  // nb * BAM_CAPB
+    // Several independent buffers in one launch (queries of a serving batch):
+    // per member, where its walks stop and where its buffer ends. Null: the
+    // member end and `len`.
+    const uint64_t *mem_until;
+    const uint64_t *mem_len;
 };
 
 struct BamReconcileArgs {
@@ -80,6 +85,15 @@ struct BamReconcileArgs {
     uint64_t *first_index;
     uint8_t *mode;
     uint64_t *totals;    // {records, tail, status, fallbacks}
+    // Segments (one workgroup each): members [seg_mb[q], seg_mb[q+1]) form one
+    // chain from seg_start[q] over a buffer ending at seg_len[q]; totals go to
+    // seg_totals[4q..4q+3] and first_index counts from the segment's first
+    // record. Null seg_mb: one segment of all nb members, start/len/totals above.
+    const uint32_t *seg_mb;
+    const uint64_t *seg_start;
+    const uint64_t *seg_len;
+    uint64_t *seg_totals;
+    const uint64_t *mem_until; // as in BamSpecArgs
 };
 
 struct BamEmitArgs {
@@ -95,6 +109,8 @@ struct BamEmitArgs {
     const uint16_t *rel_offs;
     uint64_t *record_offsets;
     uint64_t *virtual_offsets; // or null
+    const uint32_t *mem_seg;   // segment of each member, or null (one segment)
+    const uint64_t *seg_base;  // first record index of each segment, or null
 };
 
 struct BamColumns {
@@ -125,6 +141,9 @@ hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s);
 hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s);
 hipError_t launch_bam_emit(const BamEmitArgs &a, hipStream_t s);
 hipError_t launch_bam_decode(const BamDecodeArgs &a, uint64_t max_records, hipStream_t s);
+hipError_t launch_bam_reconcile_segments(const BamReconcileArgs &a, uint32_t n_seg, hipStream_t s);
+// seg_base[q] = sum of seg_totals[4p] for p < q; seg_base[n_seg] = the total
+hipError_t launch_bam_segment_bases(const uint64_t *seg_totals, uint32_t n_seg, uint64_t *seg_base, hipStream_t s);
 
 } // namespace frhip
 #endif

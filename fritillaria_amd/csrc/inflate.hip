@@ -99,6 +99,14 @@ struct alignas(16) WaveMem {
 #define WFENCE() __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront")
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+    // the builtin returns int: each half is narrowed to uint32_t before it is
+    // widened, or the low half would be sign-extended over the high one
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | lo;
+}
 
 // Wave-uniform LSB-first bit reader over one member's payload. Offsets are
 // relative to `sbase`, the payload start rounded down to CHUNK.
@@ -118,10 +126,10 @@ __device__ __forceinline__ void load_chunk(uint32_t *stage, const uint8_t *sbase
     reinterpret_cast<uint2 *>(stage)[((rel & (STAGE_BYTES - 1)) >> 3) + lane] = v;
 }
 
-// Makes bytes [r & ~3, (r & ~3) + 12) resident in the stage (r relative to sbase).
+// Makes bytes [r & ~7, (r & ~7) + 16) resident in the stage (r relative to sbase).
 __device__ __forceinline__ void ensure_stage(uint32_t *stage, Bits &b, uint32_t r, uint32_t lane)
 {
-    uint32_t lo = r & ~3u, hi = lo + 12;
+    uint32_t lo = r & ~7u, hi = lo + 16;
     if (hi <= b.stage_end && lo + STAGE_BYTES >= b.stage_end) return;
     if (b.stage_end != 0 && lo + CHUNK >= b.stage_end && hi <= b.stage_end + CHUNK) {
         load_chunk(stage, b.sbase, b.stage_end, lane);
@@ -144,13 +152,14 @@ __device__ __forceinline__ void refill(uint32_t *stage, Bits &b, uint32_t lane)
     if (nbytes > avail) nbytes = avail;
     uint32_t r = b.lead + b.pos;
     ensure_stage(stage, b, r, lane);
-    uint32_t w = (r >> 2) & (STAGE_WORDS - 1);
-    uint32_t w0 = uni(stage[w]);
-    uint32_t w1 = uni(stage[(w + 1) & (STAGE_WORDS - 1)]);
-    uint32_t w2 = uni(stage[(w + 2) & (STAGE_WORDS - 1)]);
-    uint64_t lo = ((uint64_t)w1 << 32) | w0;
-    uint32_t sh = (r & 3) * 8;
-    uint64_t v = sh ? ((lo >> sh) | ((uint64_t)w2 << (64 - sh))) : lo;
+    // two aligned 8-byte reads cover [r, r + 8)
+    const uint32_t q = (r >> 3) & (STAGE_WORDS / 2 - 1);
+    const uint2 x = reinterpret_cast<const uint2 *>(stage)[q];
+    const uint2 y = reinterpret_cast<const uint2 *>(stage)[(q + 1) & (STAGE_WORDS / 2 - 1)];
+    const uint64_t lo = ((uint64_t)uni(x.y) << 32) | uni(x.x);
+    const uint64_t hi = ((uint64_t)uni(y.y) << 32) | uni(y.x);
+    uint32_t sh = (r & 7) * 8;
+    uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
     if (nbytes < 8) v &= (1ull << (8 * nbytes)) - 1;
     b.buf |= v << b.cnt;
     b.cnt += 8 * nbytes;
@@ -531,6 +540,14 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
         }
         // literal / length / distance codes until end-of-block
         for (;;) {
+            // The state is the same in every lane; say so, so that it stays in
+            // SGPRs and the loop runs on the scalar unit.
+            b.buf = uni64(b.buf);
+            b.cnt = uni(b.cnt);
+            b.pos = uni(b.pos);
+            b.stage_end = uni(b.stage_end);
+            o.pos = uni(o.pos);
+            o.flushed = uni(o.flushed);
             if (o.pos - o.flushed >= G::FLUSH_AT) flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
             if (b.cnt < 32) refill(sm.stage, b, lane);
             int sym = decode_sym<LBITS>(b, sm.llut, sm.lcount, sm.lsym, ST_BAD_CODE, err);
@@ -647,8 +664,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         if (m >= a.n) break;
         if (a.order) m = a.order[m];
         uint32_t st, produced, crc;
-        inflate_member<RING>(sm, crctab, a.in, a.in_off[m], a.in_len[m], a.out + a.out_off[m], a.out_cap[m], lane,
-                             lane_factor, a.x2n, st, produced, crc);
+        // member parameters are wave-uniform: keep them (and all the decode
+        // derives from them) scalar; pointers stay typed, only offsets are made uniform
+        inflate_member<RING>(sm, crctab, a.in, uni64(a.in_off[m]), uni(a.in_len[m]), a.out + uni64(a.out_off[m]),
+                             uni(a.out_cap[m]), lane, lane_factor, a.x2n, st, produced, crc);
         if (lane == 0) {
             a.status[m] = st;
             a.produced[m] = produced;

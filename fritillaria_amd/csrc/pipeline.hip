@@ -225,6 +225,16 @@ __global__ void bam_header_kernel(const uint8_t *u, const uint64_t *u_len, uint6
     *status = st;
 }
 
+// Plain device copy: 16-byte words, grid-stride, byte tail.
+__global__ __launch_bounds__(256) void copy_kernel(uint8_t *dst, const uint8_t *src, uint64_t n)
+{
+    const uint64_t words = n >> 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride)
+        reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+    for (uint64_t i = (words << 4) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void fill_u64_kernel(uint64_t *p, uint64_t v, uint64_t n)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -287,6 +297,16 @@ hipError_t launch_bam_header(const uint8_t *u, const uint64_t *u_len, uint64_t *
                              hipStream_t s)
 {
     hipLaunchKernelGGL(bam_header_kernel, dim3(1), dim3(64), 0, s, u, u_len, records_start, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(uint8_t *dst, const uint8_t *src, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    uint64_t grid = (n / 16 + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)grid), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }
 

@@ -143,6 +143,8 @@ def lib():
         "frhip_buf_device_ptr": (_vp, [_vp]),
         "frhip_buf_copy_to_host": (C.c_int, [_vp, _vp, _u64, _ep]),
         "frhip_buf_free": (None, [_vp]),
+        "frhip_buf_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_buf_copy_device": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_discover": (C.c_int, [_vp, _u64, _u64, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64), _ep]),
         "frhip_inflate_batch_device": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _ep]),
         "frhip_inflate_batch": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _ep]),
@@ -184,6 +186,15 @@ def lib():
         "frhip_pipeline_column": (_vp, [_vp, C.c_int]),
         "frhip_pipeline_member_offsets": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_pipeline_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int]),
+        "frhip_server_create": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_server_free": (None, [_vp]),
+        "frhip_server_members": (_u64, [_vp]),
+        "frhip_server_member_table": (C.c_int, [_vp, _vp, _vp, _ep]),
+        "frhip_server_batch_footprint": (_u64, [_vp, _vp, _u64]),
+        "frhip_server_run": (C.c_int, [_vp, _vp, _u64, _vp, _ep]),
+        "frhip_server_column": (_vp, [_vp, C.c_int]),
+        "frhip_server_inflated": (_vp, [_vp]),
+        "frhip_device_memory": (C.c_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), _ep]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -198,7 +209,7 @@ def exported_symbols() -> list:
     import re
     here = os.path.dirname(os.path.abspath(__file__))
     names = []
-    for h in ("frhip.h", "frhip_pipeline.h"):
+    for h in ("frhip.h", "frhip_pipeline.h", "frhip_serve.h"):
         p = os.path.join(os.path.dirname(here), "include", h)
         if os.path.exists(p):
             txt = open(p).read()
@@ -263,6 +274,18 @@ class HipContext:
     def set_inflate_schedule(self, schedule: int) -> None:
         if lib().frhip_ctx_set_inflate_schedule(self._h, schedule) != OK:
             raise ValueError(f"no inflate schedule {schedule}")
+
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        """A device buffer of nbytes (contents undefined)."""
+        out = _vp()
+        e = _Err()
+        _raise(lib().frhip_buf_alloc(self.handle, nbytes, C.byref(out), C.byref(e)), e)
+        return DeviceBuffer(out)
+
+    def copy_device(self, dst: "DeviceBuffer", src: "DeviceBuffer") -> None:
+        """Plain device-to-device copy kernel over buffers of equal length."""
+        e = _Err()
+        _raise(lib().frhip_buf_copy_device(self.handle, dst._h, src._h, C.byref(e)), e)
 
     def set_bam_window(self, window: int) -> None:
         lib().frhip_ctx_set_bam_window(self._h, window)
@@ -822,6 +845,96 @@ class BatchedPipeline:
     def close(self):
         if getattr(self, "_h", None):
             lib().frhip_pipeline_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Batched virtual-offset range queries (include/frhip_serve.h)
+# ---------------------------------------------------------------------------
+
+QUERY_DTYPE = np.dtype([("vstart", "<u8"), ("vend", "<u8")])
+QUERY_RESULT_DTYPE = np.dtype([("first", "<u8"), ("records", "<u8"), ("members", "<u8"),
+                               ("inflated_bytes", "<u8"), ("status", "<i4"), ("reserved", "<u4"),
+                               ("error_position", "<u8")])
+TO_END = (1 << 64) - 1
+
+
+def device_memory(ctx: HipContext):
+    """(free, total) bytes of the context's device."""
+    f, t = _u64(), _u64()
+    e = _Err()
+    _raise(lib().frhip_device_memory(ctx.handle, C.byref(f), C.byref(t), C.byref(e)), e)
+    return f.value, t.value
+
+
+class QueryServer:
+    """Serves [vstart, vend) virtual-offset ranges of a BAM resident in HBM, many
+    queries per launch (noodles csi Query semantics,
+    crates/fritillaria-csi/src/io/query.rs:66-82)."""
+
+    def __init__(self, context: HipContext, file_bytes):
+        self.ctx = context
+        self._file = _u8(file_bytes)
+        h = _vp()
+        e = _Err()
+        _raise(lib().frhip_server_create(context.handle, _ptr(self._file), self._file.size, C.byref(h),
+                                         C.byref(e)), e)
+        self._h = h
+        self._file = None
+
+    def members(self) -> int:
+        return int(lib().frhip_server_members(self._h))
+
+    def member_table(self):
+        n = self.members()
+        co = np.zeros(max(n, 1), np.uint64)
+        up = np.zeros(n + 1, np.uint64)
+        e = _Err()
+        _raise(lib().frhip_server_member_table(self._h, _ptr(co), _ptr(up), C.byref(e)), e)
+        return co[:n], up
+
+    @staticmethod
+    def _queries(queries) -> np.ndarray:
+        q = np.zeros(len(queries), QUERY_DTYPE)
+        for i, (a, b) in enumerate(queries):
+            q[i] = (a, b)
+        return q
+
+    def footprint(self, queries) -> int:
+        q = self._queries(queries)
+        return int(lib().frhip_server_batch_footprint(self._h, _ptr(q), len(q)))
+
+    def run(self, queries) -> np.ndarray:
+        """Runs the (vstart, vend) queries as one batch; returns a QUERY_RESULT_DTYPE
+        array. Columns of the batch are read with column()/column_to_host()."""
+        q = self._queries(queries)
+        res = np.zeros(len(q), QUERY_RESULT_DTYPE)
+        e = _Err()
+        _raise(lib().frhip_server_run(self._h, _ptr(q), len(q), _ptr(res), C.byref(e)), e)
+        return res
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_server_column(self._h, _COL_INDEX[name]), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(BAM_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def inflated(self) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_server_inflated(self._h), owned=False, keepalive=self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_server_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -101,6 +101,12 @@ void *frhip_buf_device_ptr(const frhip_buf *b);
 /* copy_to_host: rejects dst_len != byte_len with FRHIP_E_INVALID_DEVICE_BATCH and synchronises. */
 int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t dst_len, frhip_error *err);
 void frhip_buf_free(frhip_buf *b);
+/* No reference counterpart. A device buffer of len bytes (contents undefined),
+   and a plain device-to-device copy kernel over two buffers of equal length
+   (synchronous). The copy reads and writes every byte once, with 16-byte
+   vector accesses, which makes it the calibration kernel for HBM byte counters. */
+int frhip_buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_buf **out, frhip_error *err);
+int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip_buf *src, frhip_error *err);
 
 /* ---- BGZF member discovery (crates/fritillaria-bgzf/src/discover.rs:51-113) ---- */
 

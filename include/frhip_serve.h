@@ -1,0 +1,68 @@
+/*
+ * frhip_serve.h -- virtual-offset range queries over a resident BGZF BAM,
+ * served in batches.
+ *
+ * No counterpart in the reference, which serves one range at a time on the host
+ * through noodles' csi::io::Query (crates/fritillaria-csi/src/io/query.rs:66-82)
+ * over a BgzfReader. A query here is [vstart, vend): seek to vstart (a record
+ * start's virtual offset, (coffset << 16) | uoffset) and serve records while
+ * their virtual offset is below vend. A batch of queries is one launch per
+ * stage: every member any query needs is inflated by one persistent kernel
+ * (verified like the per-call path), each query is a separate record chain in
+ * the same boundary launch, and all records land in one set of columns. A
+ * query's records and columns are the same whether it runs alone or in a batch.
+ */
+#ifndef FRHIP_SERVE_H
+#define FRHIP_SERVE_H
+#include <stdint.h>
+
+#include "frhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct frhip_server frhip_server;
+
+typedef struct {
+    uint64_t vstart; /* virtual offset of the first record served */
+    uint64_t vend;   /* serve records whose virtual offset is below this; UINT64_MAX = to the end */
+} frhip_query;
+
+typedef struct {
+    uint64_t first;          /* index of the query's first record in the batch columns */
+    uint64_t records;        /* records served */
+    uint64_t members;        /* BGZF members inflated for the query */
+    uint64_t inflated_bytes; /* bytes those members inflate to */
+    int32_t status;          /* FRHIP_OK, or the error kind of this query */
+    uint32_t reserved;
+    uint64_t error_position; /* for FRHIP_E_MALFORMED: position in the query's inflated bytes */
+} frhip_query_result;
+
+/* Uploads file[0..len) to HBM once and walks its member headers on the host
+   (the same strict walk as frhip_discover). */
+int frhip_server_create(frhip_ctx *ctx, const uint8_t *file, uint64_t len, frhip_server **out, frhip_error *err);
+void frhip_server_free(frhip_server *s);
+uint64_t frhip_server_members(const frhip_server *s);
+/* Compressed offset and inflated-prefix of every member (members and members+1 entries). */
+int frhip_server_member_table(const frhip_server *s, uint64_t *coffsets, uint64_t *uprefix, frhip_error *err);
+/* Device bytes a batch of these queries needs beyond the resident file (inflated
+   bytes, columns at their worst-case record count, scan scratch): what an
+   admission policy charges against free HBM. */
+uint64_t frhip_server_batch_footprint(const frhip_server *s, const frhip_query *q, uint64_t n);
+/* Runs the queries as one batch, synchronously. Per-query failures (a vstart
+   or vend that names no member, a malformed record, a member that fails
+   verification) are reported in res[i].status; the call itself fails only
+   when the device does. */
+int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t n, frhip_query_result *res, frhip_error *err);
+/* Columns of the last batch, in frhip_bam_column order (virtual offsets included),
+   and the inflated bytes they index. Owned by the server; valid until the next run. */
+const frhip_buf *frhip_server_column(const frhip_server *s, int column);
+const frhip_buf *frhip_server_inflated(const frhip_server *s);
+/* hipMemGetInfo of the context's device. */
+int frhip_device_memory(frhip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, frhip_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

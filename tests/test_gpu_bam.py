@@ -250,3 +250,22 @@ def test_speculation_holds_past_65536_members(codec, decoder, oracle_mod):
     records = decoder.decode(batch, rs)
     check_batch(oracle_mod, records, batch.data().to_vec(), batch.offsets(), batch.compressed_offsets(), rs)
     assert records.fallbacks() <= 2 + len(spans) // 50, records.fallbacks()
+
+
+def test_speculation_holds_past_2_gib(codec, decoder, oracle_mod):
+    """One batch of ~2.4 GB inflated: record positions with bit 31 set must
+    speculate like the rest (a sign-extended 64-bit broadcast once sent every
+    member in [2 GiB, 4 GiB) to a fallback walk)."""
+    print("[gpu] bam batch past 2 GiB", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(2, n_records=8_000_000)
+    assert s.inflated_len > (1 << 31) + (1 << 28)
+    spans, _ = fa.discover(s.compressed)
+    batch = codec.inflate_batch_device(s.compressed, spans)
+    records = decoder.decode(batch, s.records_start)
+    u = batch.data().to_vec()
+    rc, want, tail = oracle_mod.bam_scan_records(u, s.records_start)
+    assert rc == 0 and len(records) == len(want) == 8_000_000 and records.tail() == tail
+    assert np.array_equal(records.column_to_host("record_offsets"), want)
+    assert records.fallbacks() <= 2 + len(spans) // 50, records.fallbacks()

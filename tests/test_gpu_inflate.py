@@ -172,3 +172,18 @@ def test_crc32_blocks(codec):
     lens = [65000, 0, 7, 1000, 64]
     got = codec.crc32_blocks(b.data(), offs, lens)
     assert got.tolist() == [zlib.crc32(data[o:o + n]) for o, n in zip(offs, lens)]
+
+
+def test_device_copy_kernel(hip_ctx):
+    import fritillaria_amd as fa
+    import numpy as np
+    rng = np.random.default_rng(7)
+    data = rng.integers(0, 256, 1_000_003, dtype=np.uint8).tobytes()
+    codec = fa.HipCodec(hip_ctx)
+    from helpers import bgzf
+    raw = bgzf(data, level=1)
+    spans, _ = fa.discover(raw)
+    src = codec.inflate_batch_device(raw, spans).data()
+    dst = hip_ctx.alloc(src.byte_len())
+    hip_ctx.copy_device(dst, src)
+    assert dst.to_vec().tobytes() == data
