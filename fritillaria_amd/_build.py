@@ -20,7 +20,8 @@ OBJ = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["inflate.hip", "bam.hip", "pipeline.hip", "fastq.hip", "api.cpp", "bam_host.cpp", "pipeline_host.cpp",
-               "fastq_host.cpp", "serve_host.cpp"]
+               "fastq_host.cpp", "serve_host.cpp",
+               "bcf_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_FLAGS = [

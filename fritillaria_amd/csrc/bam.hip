@@ -96,6 +96,62 @@ __device__ __forceinline__ Step cand_step(const uint8_t *buf, uint64_t len, uint
     return {end, 0};
 }
 
+// BCF: one step of scan_records (crates/fritillaria-bcf/src/columnar/record.rs:60-95):
+// fewer than 8 bytes or a record past the buffer is the tail, l_shared < 24 is
+// malformed.
+__device__ __forceinline__ Step bcf_step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until)
+{
+    if (cur >= until) return {cur, 1};
+    if (cur + 8 > len) return {cur, 2};
+    const uint32_t ls = ld32(buf + cur), li = ld32(buf + cur + 4);
+    if (ls < 24) return {cur, 3};
+    const uint64_t end = cur + 8 + (uint64_t)ls + li;
+    if (end > len) return {cur, 2};
+    return {end, 0};
+}
+
+// BCF candidate filter: looks_like_a_record (record.rs:96-160) with the
+// header's sample and contig counts; records not whole in the buffer drop.
+__device__ __forceinline__ Step bcf_cand_step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until,
+                                              uint32_t samples, uint32_t contigs)
+{
+    if (cur >= until) return {cur, 1};
+    if (cur + 32 > len) return {cur, 3};
+    const uint32_t ls = ld32(buf + cur), li = ld32(buf + cur + 4);
+    if (ls < 24) return {cur, 3};
+    const uint64_t end = cur + 8 + (uint64_t)ls + li;
+    if (end > len) return {cur, 3};
+    const int32_t chrom = (int32_t)ld32(buf + cur + 8), pos = (int32_t)ld32(buf + cur + 12);
+    const int32_t rlen = (int32_t)ld32(buf + cur + 16);
+    if (chrom < 0 || (uint32_t)chrom >= contigs || pos < -1 || rlen < 0) return {cur, 3};
+    const uint32_t n_allele = ld16(buf + cur + 26);
+    const uint32_t n_sample = ld16(buf + cur + 28) | ((uint32_t)buf[cur + 30] << 16);
+    const uint32_t n_fmt = buf[cur + 31];
+    if (n_sample != samples || n_allele == 0) return {cur, 3};
+    if (n_fmt == 0 ? li != 0 : (uint64_t)li < 2ull * n_fmt) return {cur, 3};
+    return {end, 0};
+}
+
+// The chain's exact step and candidate filter, and the record length the emit
+// walk adds, per format.
+template <uint32_t F>
+__device__ __forceinline__ Step chain_step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until)
+{
+    return F == CHAIN_BCF ? bcf_step(buf, len, cur, until) : step(buf, len, cur, until);
+}
+template <uint32_t F>
+__device__ __forceinline__ Step chain_cand(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until,
+                                           uint32_t samples, uint32_t contigs)
+{
+    return F == CHAIN_BCF ? bcf_cand_step(buf, len, cur, until, samples, contigs) : cand_step(buf, len, cur, until);
+}
+template <uint32_t F>
+__device__ __forceinline__ uint64_t chain_next(const uint8_t *buf, uint64_t cur)
+{
+    return F == CHAIN_BCF ? cur + 8 + (uint64_t)ld32(buf + cur) + ld32(buf + cur + 4)
+                          : cur + 4 + (uint64_t)ld32(buf + cur);
+}
+
 struct Walk {
     uint64_t land;
     uint32_t count;
@@ -104,12 +160,13 @@ struct Walk {
 };
 
 // walk_segment (blocked.rs:141-187), uniform.
+template <uint32_t F>
 __device__ Walk walk_uniform(const uint8_t *buf, uint64_t len, uint64_t from, uint64_t until)
 {
     Walk w{from, 0, 1, 1};
     uint64_t cur = from;
     for (;;) {
-        Step s = step(buf, len, cur, until);
+        Step s = chain_step<F>(buf, len, cur, until);
         if (s.kind == 1) { w.land = cur; return w; }
         if (s.kind == 2) { w.land = cur; w.complete = 0; return w; }
         if (s.kind == 3) { w.land = cur; w.valid = 0; return w; }
@@ -118,6 +175,7 @@ __device__ Walk walk_uniform(const uint8_t *buf, uint64_t len, uint64_t from, ui
     }
 }
 
+template <uint32_t F>
 __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
 {
     const uint32_t lane = threadIdx.x;
@@ -148,7 +206,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
         if (p < T) {
             alive = 1;
             for (;;) {
-                Step s = cand_step(a.buf, len, cur, T);
+                Step s = chain_cand<F>(a.buf, len, cur, T, a.samples, a.contigs);
                 if (s.kind == 1) break;
                 if (s.kind == 3) { alive = 0; break; }
                 ++cnt;
@@ -186,7 +244,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
     uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
     if (complete) {
         for (;;) {
-            Step s = step(a.buf, len, cur, sb1);
+            Step s = chain_step<F>(a.buf, len, cur, sb1);
             if (s.kind == 1) break;
             if (s.kind == 2) { complete = 0; break; }
             if (s.kind == 3) { valid = 0; break; }
@@ -229,6 +287,7 @@ __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t j)
 // checks, in parallel, whether its member can be adopted assuming the previous
 // member was; the uniform pass then follows the chain with register reads and
 // walks only the members whose speculation does not apply.
+template <uint32_t F>
 __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
 {
     const uint32_t lane = threadIdx.x;
@@ -304,7 +363,7 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
                 comp_j = rl32(comp, j);
                 mode = found;
             } else {
-                Walk w = walk_uniform(a.buf, a.len, cur, js1);
+                Walk w = walk_uniform<F>(a.buf, a.len, cur, js1);
                 cnt = w.count;
                 land = w.land;
                 comp_j = (uint32_t)w.complete;
@@ -315,7 +374,9 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
             o_first = wl64(emitted, j, o_first);
             if (!valid) {
                 status = BAM_MALFORMED;
-                tail = cur;
+                // BAM reports the chain's entry into the member (blocked.rs);
+                // BCF the record itself (record.rs scan_records)
+                tail = F == CHAIN_BCF ? land : cur;
                 stopped = 1;
                 continue;
             }
@@ -356,6 +417,7 @@ __global__ void bam_segment_bases_kernel(const uint64_t *seg_totals, uint32_t n_
     seg_base[n_seg] = run;
 }
 
+template <uint32_t F>
 __global__ __launch_bounds__(64) void bam_emit_kernel(BamEmitArgs a)
 {
     const uint32_t lane = threadIdx.x;
@@ -375,7 +437,7 @@ __global__ __launch_bounds__(64) void bam_emit_kernel(BamEmitArgs a)
             a.record_offsets[first + i] = cur;
             if (a.virtual_offsets) a.virtual_offsets[first + i] = vbase | (cur - sb);
         }
-        cur += 4 + (uint64_t)ld32(a.buf + cur);
+        cur = chain_next<F>(a.buf, cur);
     }
     if (mode != BAM_MODE_FALLBACK) {
         const uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
@@ -417,25 +479,123 @@ __global__ __launch_bounds__(256) void bam_decode_kernel(BamDecodeArgs a)
     }
 }
 
+// BCF typed value (crates/fritillaria-bcf/src/columnar/typed.rs:255-300): bytes
+// per element of a type code, or 0xFFFFFFFF for a reserved code.
+__device__ __forceinline__ uint32_t bcf_type_size(uint32_t code)
+{
+    switch (code) {
+    case 0: return 0;
+    case 1: case 7: return 1;
+    case 2: return 2;
+    case 3: case 5: return 4;
+    default: return 0xFFFFFFFFu;
+    }
+}
+
+// End of the typed value at pos inside [.., end), or 0 when it does not parse:
+// a reserved type, an escaped count that is not an atomic integer >= 15, or a
+// value running past end.
+__device__ __forceinline__ uint64_t bcf_skip(const uint8_t *buf, uint64_t end, uint64_t pos)
+{
+    if (pos >= end) return 0;
+    const uint32_t d = buf[pos], code = d & 15;
+    const uint32_t size = bcf_type_size(code);
+    if (size == 0xFFFFFFFFu) return 0;
+    uint64_t count = d >> 4, header = 1;
+    if (count == 15) {
+        if (pos + 1 >= end) return 0;
+        const uint32_t in = buf[pos + 1], ic = in & 15;
+        if ((in >> 4) != 1 || (ic != 1 && ic != 2 && ic != 3)) return 0;
+        const uint32_t w = bcf_type_size(ic);
+        if (pos + 2 + w > end) return 0;
+        const uint8_t *p = buf + pos + 2;
+        const int32_t v = ic == 1 ? (int32_t)(int8_t)p[0] : ic == 2 ? (int32_t)(int16_t)ld16(p) : (int32_t)ld32(p);
+        if (v < 15) return 0;
+        count = (uint64_t)v;
+        header = 2 + w;
+    }
+    const uint64_t e = pos + header + count * size;
+    return e > end ? 0 : e;
+}
+
+// RecordBatch::decode for BCF (crates/fritillaria-bcf/src/columnar/batch.rs:158-172,
+// record.rs Record::bounds): the site core, then ID, the alleles and FILTER
+// skipped as typed values inside the record. A record whose values do not
+// parse is reported by index (the lowest one wins).
+__global__ __launch_bounds__(256) void bcf_decode_kernel(BcfDecodeArgs a)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t o = a.c.record_offsets[i];
+        const uint64_t e = i + 1 < a.n ? a.c.record_offsets[i + 1] : a.tail;
+        const uint8_t *r = a.buf + o;
+        const uint32_t ls = ld32(r), li = ld32(r + 4);
+        const uint32_t n_allele = ld16(r + 26);
+        a.c.chrom[i] = (int32_t)ld32(r + 8);
+        a.c.pos[i] = (int32_t)ld32(r + 12);
+        a.c.rlen[i] = (int32_t)ld32(r + 16);
+        a.c.qual[i] = ld32(r + 20);
+        a.c.n_info[i] = (uint16_t)ld16(r + 24);
+        a.c.n_allele[i] = (uint16_t)n_allele;
+        a.c.n_sample[i] = ld16(r + 28) | ((uint32_t)r[30] << 16);
+        a.c.n_fmt[i] = r[31];
+        a.c.genotypes_start[i] = 8 + ls;
+        a.c.record_end[i] = (uint32_t)(e - o);
+        bool ok = e - o >= 32 && 8ull + ls + li == e - o;
+        uint64_t at = ok ? bcf_skip(a.buf, e, o + 32) : 0;
+        ok = ok && at != 0;
+        a.c.alleles_start[i] = ok ? (uint32_t)(at - o) : 0;
+        for (uint32_t k = 0; ok && k < n_allele; ++k) {
+            at = bcf_skip(a.buf, e, at);
+            ok = at != 0;
+        }
+        a.c.filters_start[i] = ok ? (uint32_t)(at - o) : 0;
+        if (ok) {
+            at = bcf_skip(a.buf, e, at);
+            ok = at != 0;
+        }
+        a.c.info_start[i] = ok ? (uint32_t)(at - o) : 0;
+        if (!ok) atomicMin(a.first_bad, (unsigned long long)i);
+    }
+}
+
 } // namespace
+
+hipError_t launch_bcf_decode(const BcfDecodeArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    uint64_t blocks = (a.n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(bcf_decode_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 {
     if (a.nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(bam_spec_kernel, dim3(a.nb), dim3(64), 0, s, a);
+    if (a.format == CHAIN_BCF)
+        hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BCF>, dim3(a.nb), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BAM>, dim3(a.nb), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(bam_reconcile_kernel, dim3(1), dim3(64), 0, s, a);
+    if (a.format == CHAIN_BCF)
+        hipLaunchKernelGGL(bam_reconcile_kernel<CHAIN_BCF>, dim3(1), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(bam_reconcile_kernel<CHAIN_BAM>, dim3(1), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_bam_reconcile_segments(const BamReconcileArgs &a, uint32_t n_seg, hipStream_t s)
 {
     if (n_seg == 0) return hipSuccess;
-    hipLaunchKernelGGL(bam_reconcile_kernel, dim3(n_seg), dim3(64), 0, s, a);
+    if (a.format == CHAIN_BCF)
+        hipLaunchKernelGGL(bam_reconcile_kernel<CHAIN_BCF>, dim3(n_seg), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(bam_reconcile_kernel<CHAIN_BAM>, dim3(n_seg), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -448,7 +608,10 @@ hipError_t launch_bam_segment_bases(const uint64_t *seg_totals, uint32_t n_seg, 
 hipError_t launch_bam_emit(const BamEmitArgs &a, hipStream_t s)
 {
     if (a.nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(bam_emit_kernel, dim3(a.nb), dim3(64), 0, s, a);
+    if (a.format == CHAIN_BCF)
+        hipLaunchKernelGGL(bam_emit_kernel<CHAIN_BCF>, dim3(a.nb), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(bam_emit_kernel<CHAIN_BAM>, dim3(a.nb), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

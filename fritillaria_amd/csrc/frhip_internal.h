@@ -40,10 +40,15 @@ uint32_t inflate_waves_per_cu(int schedule);
 // Record-boundary discovery for BAM over an inflated batch.
 constexpr uint32_t BAM_K = 8;                 // survivors kept per block
 constexpr uint64_t BAM_NONE = ~0ull;          // block has no entry
-constexpr uint32_t BAM_CAPB = 65536 / 36 + 1; // most records that can start in one member
+constexpr uint32_t BAM_CAPB = 65536 / 32 + 1; // most records that can start in one member (BCF's are >= 32 B)
 constexpr uint32_t BAM_MODE_FALLBACK = 255;
 
+// Length-prefixed record chains the boundary scan handles.
+enum : uint32_t { CHAIN_BAM = 0, CHAIN_BCF = 1 };
+
 struct BamSpecArgs {
+    uint32_t format;           // CHAIN_*
+    uint32_t samples, contigs; // BCF: the header's sample and contig counts (candidate filter)
     const uint8_t *buf;
     uint64_t len;
     const uint64_t *block_starts; // nb + 1
@@ -66,6 +71,7 @@ struct BamSpecArgs {
 };
 
 struct BamReconcileArgs {
+    uint32_t format;
     const uint8_t *buf;
     uint64_t len;
     const uint64_t *block_starts;
@@ -97,6 +103,7 @@ struct BamReconcileArgs {
 };
 
 struct BamEmitArgs {
+    uint32_t format;
     const uint8_t *buf;
     const uint64_t *block_starts;
     const uint64_t *coffsets; // compressed offset per block (for virtual offsets), or null
@@ -137,6 +144,27 @@ struct BamDecodeArgs {
     BamColumns c;
 };
 
+// BCF site-core columns and typed-value bounds
+// (crates/fritillaria-bcf/src/columnar/device.rs:59-98 order).
+struct BcfColumns {
+    int32_t *chrom, *pos, *rlen;
+    uint32_t *qual;
+    uint16_t *n_info, *n_allele;
+    uint32_t *n_sample;
+    uint8_t *n_fmt;
+    uint64_t *record_offsets;
+    uint32_t *alleles_start, *filters_start, *info_start, *genotypes_start, *record_end;
+};
+
+struct BcfDecodeArgs {
+    const uint8_t *buf;
+    uint64_t n;
+    uint64_t tail;           // end of the last record
+    BcfColumns c;
+    unsigned long long *first_bad; // lowest record index whose typed values do not parse (init ~0)
+};
+
+hipError_t launch_bcf_decode(const BcfDecodeArgs &a, hipStream_t s);
 hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s);
 hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s);
 hipError_t launch_bam_emit(const BamEmitArgs &a, hipStream_t s);

@@ -34,7 +34,13 @@ namespace frhip {
 namespace {
 
 constexpr uint32_t STAGE_BYTES = 1024, STAGE_WORDS = STAGE_BYTES / 4, CHUNK = 512;
-constexpr int LBITS = 10, DBITS = 8, CBITS = 7;
+constexpr int PBITS = 9, DBITS = 8, CBITS = 7;
+// Literal/length table entry (PBITS-bit index): bits 0-3 length of the first
+// code (0: longer than PBITS, take the canonical walk), 4-12 its symbol; when
+// the first symbol is a literal and a second literal's code also fits in the
+// index, bit 13 is set, bits 14-17 hold both codes' length and 18-25 the
+// second literal.
+constexpr uint32_t PAIR = 1u << 13;
 
 enum : uint32_t {
     ST_OK = 0,
@@ -84,7 +90,7 @@ template <uint32_t RING>
 struct alignas(16) WaveMem {
     uint8_t ring[RING];
     uint32_t stage[STAGE_WORDS];
-    uint16_t llut[1 << LBITS];
+    uint32_t plut[1 << PBITS];
     uint16_t dlut[1 << DBITS];
     uint16_t clut[1 << CBITS];
     uint16_t lsym[288];
@@ -311,6 +317,84 @@ __device__ void build_lut(uint16_t *lut, const uint16_t *count, const uint16_t *
     WFENCE();
 }
 
+// First code of `e`'s entry (see PAIR), or the canonical walk for codes longer
+// than PBITS. Same results and error order as decode_sym.
+__device__ __forceinline__ int decode_litlen(Bits &b, uint32_t e, const uint16_t *count, const uint16_t *sym,
+                                             uint32_t &err)
+{
+    if (e & 15) {
+        const uint32_t L = e & 15;
+        if (L > b.cnt) {
+            err = ST_TRUNCATED;
+            return -1;
+        }
+        b.buf >>= L;
+        b.cnt -= L;
+        return (int)((e >> 4) & 511);
+    }
+    int code = 0, first = 0, index = 0;
+    for (int len = 1; len <= 15; ++len) {
+        if ((uint32_t)len > b.cnt) {
+            err = ST_TRUNCATED;
+            return -1;
+        }
+        code |= (int)((b.buf >> (len - 1)) & 1);
+        int c = (int)uni(count[len]);
+        if (code - first < c) {
+            b.buf >>= len;
+            b.cnt -= len;
+            return (int)uni(sym[index + code - first]);
+        }
+        index += c;
+        first = (first + c) << 1;
+        code <<= 1;
+    }
+    err = ST_BAD_CODE;
+    return -1;
+}
+
+// Canonical walk of the low bits of `bits`, at most `maxlen` of them; lane l of
+// cv holds the count of codes of length l. Returns length (0: none) and symbol.
+__device__ __forceinline__ uint32_t walk_code(uint32_t bits, uint32_t maxlen, uint32_t cv, const uint16_t *sym,
+                                              uint32_t &out_sym)
+{
+    int code = 0, first = 0, index = 0;
+    uint32_t found = 0;
+#pragma unroll
+    for (int len = 1; len <= PBITS; ++len) {
+        int c = (int)__builtin_amdgcn_readlane(cv, len);
+        if (!found && (uint32_t)len <= maxlen) {
+            code |= (int)((bits >> (len - 1)) & 1);
+            if (code - first < c) {
+                out_sym = sym[index + code - first];
+                found = (uint32_t)len;
+            } else {
+                index += c;
+                first = (first + c) << 1;
+                code <<= 1;
+            }
+        }
+    }
+    return found;
+}
+
+// The literal/length table with literal pairs (see PAIR).
+__device__ void build_pair_lut(uint32_t *lut, const uint16_t *count, const uint16_t *sym, uint32_t lane)
+{
+    uint32_t cv = lane < 16 ? count[lane] : 0;
+    for (uint32_t e = lane; e < (1u << PBITS); e += 64) {
+        uint32_t s1 = 0, s2 = 0;
+        const uint32_t l1 = walk_code(e, PBITS, cv, sym, s1);
+        uint32_t out = l1 ? (l1 | (s1 << 4)) : 0;
+        if (l1 && s1 < 256 && l1 < (uint32_t)PBITS) {
+            const uint32_t l2 = walk_code(e >> l1, PBITS - l1, cv, sym, s2);
+            if (l2 && s2 < 256) out |= PAIR | ((l1 + l2) << 14) | (s2 << 18);
+        }
+        lut[e] = out;
+    }
+    WFENCE();
+}
+
 struct Out {
     uint8_t *g;         // member's output in HBM
     uint32_t pos;       // bytes produced
@@ -456,7 +540,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
                 if (lane < 32) sm.lens[lane] = 5;
                 WFENCE();
                 construct(sm.lens, 32, sm.dcount, sm.dsym, lane, &c0);
-                build_lut<LBITS>(sm.llut, sm.lcount, sm.lsym, lane);
+                build_pair_lut(sm.plut, sm.lcount, sm.lsym, lane);
                 build_lut<DBITS>(sm.dlut, sm.dcount, sm.dsym, lane);
                 tables = 1;
             }
@@ -534,7 +618,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
                 err = ST_BAD_TABLE;
                 break;
             }
-            build_lut<LBITS>(sm.llut, sm.lcount, sm.lsym, lane);
+            build_pair_lut(sm.plut, sm.lcount, sm.lsym, lane);
             build_lut<DBITS>(sm.dlut, sm.dcount, sm.dsym, lane);
             tables = 2;
         }
@@ -550,14 +634,26 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
             o.flushed = uni(o.flushed);
             if (o.pos - o.flushed >= G::FLUSH_AT) flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
             if (b.cnt < 32) refill(sm.stage, b, lane);
-            int sym = decode_sym<LBITS>(b, sm.llut, sm.lcount, sm.lsym, ST_BAD_CODE, err);
+            const uint32_t e = uni(sm.plut[b.buf & ((1u << PBITS) - 1)]);
+            if ((e & PAIR) && ((e >> 14) & 15) <= b.cnt && o.pos + 2 <= o.cap) {
+                // two literals: even lanes store the first, odd lanes the second
+                // (every lane stores, so no lane mask is set up)
+                const uint32_t odd = lane & 1;
+                sm.ring[(o.skew + o.pos + odd) & G::RMASK] = (uint8_t)(odd ? (e >> 18) : (e >> 4));
+                const uint32_t t = (e >> 14) & 15;
+                b.buf >>= t;
+                b.cnt -= t;
+                o.pos += 2;
+                continue;
+            }
+            int sym = decode_litlen(b, e, sm.lcount, sm.lsym, err);
             if (err) break;
             if (sym < 256) {
                 if (o.pos >= o.cap) {
                     err = ST_OUTPUT_OVERFLOW;
                     break;
                 }
-                if (lane == 0) sm.ring[(o.skew + o.pos) & G::RMASK] = (uint8_t)sym;
+                sm.ring[(o.skew + o.pos) & G::RMASK] = (uint8_t)sym;
                 ++o.pos;
                 continue;
             }

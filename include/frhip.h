@@ -58,6 +58,7 @@ typedef struct frhip_ctx frhip_ctx;
 typedef struct frhip_buf frhip_buf;
 typedef struct frhip_bam_batch frhip_bam_batch;
 typedef struct frhip_fastq_batch frhip_fastq_batch;
+typedef struct frhip_bcf_batch frhip_bcf_batch;
 typedef struct frhip_reader frhip_reader;
 
 /* ---- device and context ------------------------------------------------ */
@@ -220,6 +221,51 @@ uint64_t frhip_fastq_batch_tail(const frhip_fastq_batch *b);
 uint64_t frhip_fastq_batch_survivors(const frhip_fastq_batch *b);
 const frhip_buf *frhip_fastq_batch_column(const frhip_fastq_batch *b, int column);
 void frhip_fastq_batch_free(frhip_fastq_batch *b);
+
+/* ---- BCF: BcfScanner::{scan, decode} (crates/fritillaria-cuda/src/bcf.rs:94-143, 451-470) ---- */
+
+/* Column indices of the BCF DeviceRecordBatch, in the order of
+   crates/fritillaria-bcf/src/columnar/device.rs:59-98. Bounds are relative to
+   the record start. */
+typedef enum {
+    FRHIP_BCF_CHROMOSOME_ID = 0,   /* i32 */
+    FRHIP_BCF_POSITION = 1,        /* i32, 0-based */
+    FRHIP_BCF_REFERENCE_SPAN = 2,  /* i32 (rlen) */
+    FRHIP_BCF_QUALITY = 3,         /* u32, the float's bits */
+    FRHIP_BCF_INFO_COUNT = 4,      /* u16 */
+    FRHIP_BCF_ALLELE_COUNT = 5,    /* u16 */
+    FRHIP_BCF_SAMPLE_COUNT = 6,    /* u32 */
+    FRHIP_BCF_FORMAT_COUNT = 7,    /* u8 */
+    FRHIP_BCF_RECORD_OFFSETS = 8,  /* u64, into the inflated batch */
+    FRHIP_BCF_ALLELES_START = 9,   /* u32 */
+    FRHIP_BCF_FILTERS_START = 10,  /* u32 */
+    FRHIP_BCF_INFO_START = 11,     /* u32 */
+    FRHIP_BCF_GENOTYPES_START = 12,/* u32 */
+    FRHIP_BCF_RECORD_END = 13,     /* u32 */
+    FRHIP_BCF_N_COLUMNS = 14
+} frhip_bcf_column;
+
+/* Record boundaries from `start` and the BCF columns of an inflated batch
+   (NULL or empty: no records, tail = start). block_offsets are its n_blocks+1
+   member boundaries; samples and contigs are the header's counts
+   (frhip_bcf_parse_header). The records are those of scan_records
+   (crates/fritillaria-bcf/src/columnar/record.rs:60-95): fewer than 8 bytes
+   left or a record running past the batch is the tail; l_shared below 24 is
+   FRHIP_E_MALFORMED at that record; a record whose ID, alleles or FILTER is not
+   a valid typed value is FRHIP_E_MALFORMED at the record's offset (format "bcf"). */
+int frhip_bcf_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets, uint64_t n_blocks,
+                     uint64_t start, uint32_t samples, uint32_t contigs, frhip_bcf_batch **out, frhip_error *err);
+uint64_t frhip_bcf_batch_len(const frhip_bcf_batch *b);
+uint64_t frhip_bcf_batch_tail(const frhip_bcf_batch *b);
+/* Members whose entry was not a speculated candidate (diagnostic). */
+uint64_t frhip_bcf_batch_fallbacks(const frhip_bcf_batch *b);
+const frhip_buf *frhip_bcf_batch_column(const frhip_bcf_batch *b, int column);
+void frhip_bcf_batch_free(frhip_bcf_batch *b);
+/* BCF header (crates/fritillaria-bcf/src/columnar/header.rs:43-140) on host
+   bytes: where records start, samples (#CHROM columns after FORMAT) and contigs
+   (##contig lines with an ID). */
+int frhip_bcf_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start, uint32_t *samples,
+                           uint32_t *contigs, frhip_error *err);
 
 /* ---- DeviceBgzfReader (crates/fritillaria-bgzf/src/device_read.rs:47-280) ---- */
 

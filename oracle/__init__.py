@@ -89,6 +89,12 @@ def lib():
                                                C.POINTER(C.c_uint64)]
         _lib.or_fastq_survivors.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         _lib.or_fastq_survivors.restype = C.c_uint64
+        _lib.or_bcf_looks_like.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32]
+        _lib.or_bcf_scan_records.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64,
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        _lib.or_bcf_decode_columns.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64] + [C.c_void_p] * 14 + \
+            [C.POINTER(C.c_uint64)]
+        _lib.or_bcf_validate.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         _lib.or_mt_bam_columns.restype = C.c_int
         _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
         del u8p
@@ -267,3 +273,70 @@ def mt_bam_columns(file_bytes, nthreads: int):
     return rc, {"records": int(res[0]), "inflated": int(res[1]), "sum_pos": int(res[2]),
                 "sum_flag": int(res[3]), "sum_lseq": int(res[4]), "sum_off": int(res[5]),
                 "members": int(res[6])}
+
+
+# ---------------------------------------------------------------------------
+# BCF
+# ---------------------------------------------------------------------------
+
+# (name, dtype) in the order of crates/fritillaria-bcf/src/columnar/device.rs:59-98
+BCF_COLUMNS = [
+    ("chromosome_id", np.int32), ("position", np.int32), ("reference_span", np.int32), ("quality", np.uint32),
+    ("info_count", np.uint16), ("allele_count", np.uint16), ("sample_count", np.uint32),
+    ("format_count", np.uint8), ("record_offsets", np.uint64), ("alleles_start", np.uint32),
+    ("filters_start", np.uint32), ("info_start", np.uint32), ("genotypes_start", np.uint32),
+    ("record_end", np.uint32),
+]
+
+
+def bcf_parse_header(buf):
+    """records_start, sample names and contig count of a BCF header
+    (crates/fritillaria-bcf/src/columnar/header.rs:43-140): magic BCF, major
+    version 2, l_text, NUL-stripped text; contigs are ##contig lines with an
+    ID, samples the #CHROM columns after FORMAT."""
+    b = bytes(_as_u8(buf)[: 1 << 26])
+    if len(b) < 9 or b[:3] != b"BCF" or b[3] != 2:
+        raise ValueError("not a BCF v2 header")
+    l_text = int.from_bytes(b[5:9], "little")
+    text = b[9:9 + l_text].rstrip(b"\0")
+    contigs, samples = 0, []
+    for line in text.split(b"\n"):
+        line = line.rstrip(b"\r")
+        if line.startswith(b"#CHROM"):
+            cols = line.split(b"\t")
+            samples = cols[9:] if len(cols) > 9 else []
+        elif line.startswith(b"##contig=<") and (b"<ID=" in line or b",ID=" in line):
+            contigs += 1
+    return 9 + l_text, samples, contigs
+
+
+def bcf_scan_records(buf, start: int):
+    a = _as_u8(buf)
+    cap = a.size // 32 + 1
+    offs = np.zeros(cap, dtype=np.uint64)
+    n, tail, err = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+    rc = lib().or_bcf_scan_records(_ptr(a), a.size, start, _ptr(offs), cap, C.byref(n), C.byref(tail),
+                                   C.byref(err))
+    return rc, offs[:n.value].copy(), tail.value, err.value
+
+
+def bcf_decode_columns(buf, offsets, tail: int):
+    """Returns (rc, columns dict, index of the first failing record)."""
+    a = _as_u8(buf)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = offs.size
+    cols = {name: np.zeros(max(n, 1), dtype=dt) for name, dt in BCF_COLUMNS}
+    bad = C.c_uint64(0)
+    rc = lib().or_bcf_decode_columns(_ptr(a), _ptr(offs), n, tail, *[_ptr(cols[k]) for k, _ in BCF_COLUMNS],
+                                     C.byref(bad))
+    return rc, {k: v[:n] for k, v in cols.items()}, bad.value
+
+
+def bcf_looks_like(buf, pos: int, samples: int, contigs: int) -> bool:
+    a = _as_u8(buf)
+    return bool(lib().or_bcf_looks_like(_ptr(a), a.size, pos, samples, contigs))
+
+
+def bcf_validate(buf, pos: int) -> bool:
+    a = _as_u8(buf)
+    return bool(lib().or_bcf_validate(_ptr(a), a.size, pos))

@@ -695,3 +695,177 @@ uint64_t or_fastq_survivors(const uint8_t *buf, uint64_t len, uint64_t start)
         if ((i == start || buf[i - 1] == '\n') && or_fastq_bounds(buf, len, i, b)) ++s;
     return s;
 }
+
+/* ------------------------------------------------------------------------ */
+/* BCF record boundaries, typed values and site-core columns.               */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t bcf_type_size(uint32_t code)
+{
+    switch (code) {
+    case 0: return 0;
+    case 1: case 7: return 1;
+    case 2: return 2;
+    case 3: case 5: return 4;
+    default: return 0xFFFFFFFFu;
+    }
+}
+
+/* Header bytes and element count of the typed value at pos of buf[0..len), or 0
+   when it is not one (crates/fritillaria-bcf/src/columnar/typed.rs:255-300):
+   reserved type codes fail; an escaped count (nibble 15) is an atomic integer
+   value of at least 15 that follows the descriptor. */
+static uint32_t bcf_descriptor(const uint8_t *buf, uint64_t len, uint64_t pos, uint64_t *count, uint32_t *kind)
+{
+    if (pos >= len) return 0;
+    uint32_t d = buf[pos], code = d & 15;
+    if (bcf_type_size(code) == 0xFFFFFFFFu) return 0;
+    *kind = code;
+    if ((d >> 4) < 15) {
+        *count = d >> 4;
+        return 1;
+    }
+    if (pos + 1 >= len) return 0;
+    uint32_t in = buf[pos + 1], ic = in & 15;
+    if ((in >> 4) != 1 || (ic != 1 && ic != 2 && ic != 3)) return 0;
+    uint32_t w = bcf_type_size(ic);
+    if (pos + 2 + w > len) return 0;
+    const uint8_t *p = buf + pos + 2;
+    int32_t v = ic == 1 ? (int32_t)(int8_t)p[0] : ic == 2 ? (int32_t)(int16_t)rd16(p) : (int32_t)rd32(p);
+    if (v < 15) return 0;
+    *count = (uint64_t)v;
+    return 2 + w;
+}
+
+/* End of the typed value at pos within buf[0..len), or 0 (typed.rs skip). */
+static uint64_t bcf_skip(const uint8_t *buf, uint64_t len, uint64_t pos)
+{
+    uint64_t count = 0;
+    uint32_t kind = 0;
+    uint32_t h = bcf_descriptor(buf, len, pos, &count, &kind);
+    if (!h) return 0;
+    uint64_t end = pos + h + count * bcf_type_size(kind);
+    return end > len ? 0 : end;
+}
+
+/* looks_like_a_record (crates/fritillaria-bcf/src/columnar/record.rs:96-160). */
+int or_bcf_looks_like(const uint8_t *buf, uint64_t len, uint64_t pos, uint32_t samples, uint32_t contigs)
+{
+    if (pos + 8 > len) return 0;
+    uint32_t ls = rd32(buf + pos), li = rd32(buf + pos + 4);
+    if (ls < 24) return 0;
+    uint64_t end = pos + 8 + (uint64_t)ls + li;
+    if (end > len || pos + 32 > len) return 0;
+    int32_t chrom = (int32_t)rd32(buf + pos + 8), p = (int32_t)rd32(buf + pos + 12), rlen = (int32_t)rd32(buf + pos + 16);
+    uint32_t n_allele = rd16(buf + pos + 26);
+    uint32_t n_sample = buf[pos + 28] | (buf[pos + 29] << 8) | (buf[pos + 30] << 16);
+    uint32_t n_fmt = buf[pos + 31];
+    if (chrom < 0 || (uint32_t)chrom >= contigs) return 0;
+    if (p < -1 || rlen < 0) return 0;
+    if (n_sample != samples || n_allele == 0) return 0;
+    if (n_fmt == 0) return li == 0;
+    return (uint64_t)li >= 2ull * n_fmt;
+}
+
+/* Serial record walk (record.rs scan_records): fewer than 8 bytes left or a
+   record past the end is the tail; l_shared < 24 is Malformed at that record. */
+int or_bcf_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets, uint64_t cap,
+                        uint64_t *n, uint64_t *tail, uint64_t *err_pos)
+{
+    uint64_t pos = start, k = 0;
+    for (;;) {
+        if (pos + 8 > len) break;
+        uint32_t ls = rd32(buf + pos), li = rd32(buf + pos + 4);
+        if (ls < 24) {
+            *n = k;
+            *tail = pos;
+            *err_pos = pos;
+            return OR_E_MALFORMED;
+        }
+        uint64_t end = pos + 8 + (uint64_t)ls + li;
+        if (end > len) break;
+        if (k < cap) offsets[k] = pos;
+        ++k;
+        pos = end;
+    }
+    *n = k;
+    *tail = pos;
+    return 0;
+}
+
+/* RecordBatch::decode columns (crates/fritillaria-bcf/src/columnar/batch.rs:158-172,
+   record.rs Record::new / bounds): fixed fields of the site core, then ID,
+   n_allele alleles and FILTER skipped as typed values inside the record. Returns
+   0, or OR_E_MALFORMED with *bad = index of the first record that fails. */
+int or_bcf_decode_columns(const uint8_t *buf, const uint64_t *offsets, uint64_t n, uint64_t tail,
+                          int32_t *chrom, int32_t *pos, int32_t *rlen, uint32_t *qual, uint16_t *n_info,
+                          uint16_t *n_allele, uint32_t *n_sample, uint8_t *n_fmt, uint64_t *rec_off,
+                          uint32_t *alleles_start, uint32_t *filters_start, uint32_t *info_start,
+                          uint32_t *genotypes_start, uint32_t *rec_end, uint64_t *bad)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t o = offsets[i], e = i + 1 < n ? offsets[i + 1] : tail;
+        const uint8_t *r = buf + o;
+        const uint64_t L = e - o;
+        uint32_t ls = rd32(r), li = rd32(r + 4);
+        if (L < 32 || ls < 24 || 8ull + ls + li != L) {
+            *bad = i;
+            return OR_E_MALFORMED;
+        }
+        chrom[i] = (int32_t)rd32(r + 8);
+        pos[i] = (int32_t)rd32(r + 12);
+        rlen[i] = (int32_t)rd32(r + 16);
+        qual[i] = rd32(r + 20);
+        n_info[i] = rd16(r + 24);
+        n_allele[i] = rd16(r + 26);
+        n_sample[i] = r[28] | (r[29] << 8) | (r[30] << 16);
+        n_fmt[i] = r[31];
+        rec_off[i] = o;
+        uint64_t a = bcf_skip(r, L, 32);
+        if (!a) { *bad = i; return OR_E_MALFORMED; }
+        uint64_t f = a;
+        for (uint32_t k = 0; k < n_allele[i]; ++k) {
+            f = bcf_skip(r, L, f);
+            if (!f) { *bad = i; return OR_E_MALFORMED; }
+        }
+        uint64_t inf = bcf_skip(r, L, f);
+        if (!inf) { *bad = i; return OR_E_MALFORMED; }
+        alleles_start[i] = (uint32_t)a;
+        filters_start[i] = (uint32_t)f;
+        info_start[i] = (uint32_t)inf;
+        genotypes_start[i] = 8 + ls;
+        rec_end[i] = (uint32_t)L;
+    }
+    return 0;
+}
+
+/* Record::validate (record.rs:292-330): every site field and every FORMAT field
+   walks exactly to its block's end. 1 = valid. */
+int or_bcf_validate(const uint8_t *buf, uint64_t len, uint64_t pos)
+{
+    if (pos + 32 > len) return 0;
+    uint64_t ls = rd32(buf + pos), li = rd32(buf + pos + 4);
+    uint64_t indiv = pos + 8 + ls, end = indiv + li;
+    if (ls < 24 || end > len) return 0;
+    uint32_t n_info = rd16(buf + pos + 24), n_allele = rd16(buf + pos + 26);
+    uint32_t n_sample = buf[pos + 28] | (buf[pos + 29] << 8) | (buf[pos + 30] << 16), n_fmt = buf[pos + 31];
+    uint64_t at = bcf_skip(buf, indiv, pos + 32);
+    if (!at) return 0;
+    for (uint32_t i = 0; i < n_allele; ++i) if (!(at = bcf_skip(buf, indiv, at))) return 0;
+    if (!(at = bcf_skip(buf, indiv, at))) return 0;
+    for (uint32_t i = 0; i < n_info; ++i) {
+        if (!(at = bcf_skip(buf, indiv, at))) return 0;
+        if (!(at = bcf_skip(buf, indiv, at))) return 0;
+    }
+    if (at != indiv) return 0;
+    for (uint32_t i = 0; i < n_fmt; ++i) {
+        if (!(at = bcf_skip(buf, end, at))) return 0;
+        uint64_t count = 0;
+        uint32_t kind = 0;
+        uint32_t h = bcf_descriptor(buf, end, at, &count, &kind);
+        if (!h) return 0;
+        at += h + count * bcf_type_size(kind) * n_sample;
+        if (at > end) return 0;
+    }
+    return at == end;
+}

@@ -90,6 +90,15 @@ int or_fastq_bounds(const uint8_t *buf, uint64_t len, uint64_t pos, uint32_t b[5
 int or_fastq_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets,
                           uint32_t *bounds, uint64_t cap, uint64_t *n, uint64_t *tail, uint64_t *err_pos);
 uint64_t or_fastq_survivors(const uint8_t *buf, uint64_t len, uint64_t start);
+int or_bcf_looks_like(const uint8_t *buf, uint64_t len, uint64_t pos, uint32_t samples, uint32_t contigs);
+int or_bcf_scan_records(const uint8_t *buf, uint64_t len, uint64_t start, uint64_t *offsets, uint64_t cap,
+                        uint64_t *n, uint64_t *tail, uint64_t *err_pos);
+int or_bcf_decode_columns(const uint8_t *buf, const uint64_t *offsets, uint64_t n, uint64_t tail,
+                          int32_t *chrom, int32_t *pos, int32_t *rlen, uint32_t *qual, uint16_t *n_info,
+                          uint16_t *n_allele, uint32_t *n_sample, uint8_t *n_fmt, uint64_t *rec_off,
+                          uint32_t *alleles_start, uint32_t *filters_start, uint32_t *info_start,
+                          uint32_t *genotypes_start, uint32_t *rec_end, uint64_t *bad);
+int or_bcf_validate(const uint8_t *buf, uint64_t len, uint64_t pos);
 
 #ifdef __cplusplus
 }
