@@ -157,6 +157,14 @@ def lib():
         "frhip_bam_batch_column": (_vp, [_vp, C.c_int]),
         "frhip_bam_batch_free": (None, [_vp]),
         "frhip_bam_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32), _ep]),
+        "frhip_bcf_decode": (C.c_int, [_vp, _vp, _vp, _u64, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp), _ep]),
+        "frhip_bcf_batch_len": (_u64, [_vp]),
+        "frhip_bcf_batch_tail": (_u64, [_vp]),
+        "frhip_bcf_batch_fallbacks": (_u64, [_vp]),
+        "frhip_bcf_batch_column": (_vp, [_vp, C.c_int]),
+        "frhip_bcf_batch_free": (None, [_vp]),
+        "frhip_bcf_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32), _ep]),
         "frhip_fastq_decode": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
         "frhip_fastq_batch_len": (_u64, [_vp]),
         "frhip_fastq_batch_tail": (_u64, [_vp]),
@@ -653,6 +661,98 @@ class FastqScanner:
         offs = d.column_to_host("record_offsets")
         n, surv = len(d), d.survivors()
         return SpeculativeScan(offs, d.tail(), surv, "tiled" if surv == n else "walked")
+
+
+# ---------------------------------------------------------------------------
+# BCF
+# ---------------------------------------------------------------------------
+
+# (name, dtype) in the order of crates/fritillaria-bcf/src/columnar/device.rs:59-98
+BCF_COLUMNS = [
+    ("chromosome_id", np.int32), ("position", np.int32), ("reference_span", np.int32), ("quality", np.uint32),
+    ("info_count", np.uint16), ("allele_count", np.uint16), ("sample_count", np.uint32),
+    ("format_count", np.uint8), ("record_offsets", np.uint64), ("alleles_start", np.uint32),
+    ("filters_start", np.uint32), ("info_start", np.uint32), ("genotypes_start", np.uint32),
+    ("record_end", np.uint32),
+]
+_BCF_INDEX = {n: i for i, (n, _) in enumerate(BCF_COLUMNS)}
+
+
+class BcfRecordBatch:
+    """BCF columns in device memory (crates/fritillaria-bcf/src/columnar/device.rs:100-200).
+    Offsets index the inflated batch, which this object keeps alive."""
+
+    def __init__(self, handle, source: DeviceInflateBatch):
+        self._h = handle
+        self._source = source
+
+    def __len__(self) -> int:
+        return int(lib().frhip_bcf_batch_len(self._h))
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def tail(self) -> int:
+        return int(lib().frhip_bcf_batch_tail(self._h))
+
+    def fallbacks(self) -> int:
+        return int(lib().frhip_bcf_batch_fallbacks(self._h))
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_bcf_batch_column(self._h, _BCF_INDEX[name]), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(BCF_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def to_host(self) -> dict:
+        return {name: self.column_to_host(name) for name, _ in BCF_COLUMNS}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_bcf_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BcfScanner:
+    """BcfScanner (crates/fritillaria-cuda/src/bcf.rs:42-143): `samples` and
+    `contigs` come from the header (bcf_parse_header)."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def decode(self, batch: DeviceInflateBatch, start: int, samples: int, contigs: int) -> BcfRecordBatch:
+        data = batch.data()
+        offs = np.ascontiguousarray(batch.offsets(), dtype=np.uint64)
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_bcf_decode(self.ctx.handle, None if data is None else data._h, _ptr(offs), len(offs) - 1,
+                                    start, samples, contigs, C.byref(out), C.byref(e))
+        _raise(rc, e)
+        return BcfRecordBatch(out, batch)
+
+    def scan(self, batch: DeviceInflateBatch, start: int, samples: int, contigs: int):
+        """(record offsets, tail)."""
+        d = self.decode(batch, start, samples, contigs)
+        return d.column_to_host("record_offsets"), d.tail()
+
+
+def bcf_parse_header(buf):
+    """(records_start, samples, contigs) of a BCF header in host bytes."""
+    a = _u8(buf)
+    rs, ns, nc = _u64(), C.c_uint32(), C.c_uint32()
+    e = _Err()
+    _raise(lib().frhip_bcf_parse_header(_ptr(a), a.size, C.byref(rs), C.byref(ns), C.byref(nc), C.byref(e)), e)
+    return rs.value, ns.value, nc.value
 
 
 def bam_parse_header(buf):

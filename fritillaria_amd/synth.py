@@ -187,3 +187,81 @@ def config_fastq(n_records=None, threads=None) -> SynthFastq:
     u = fastq_stream(n, CONFIG_SEED + 4)
     c = bgzf(u, level=6, payload_cap=65280, seed=CONFIG_SEED + 4, threads=threads)
     return SynthFastq(c, u, n)
+
+
+@dataclass
+class SynthBcf:
+    compressed: np.ndarray
+    inflated: np.ndarray
+    records_start: int
+    n_records: int
+    samples: int
+    contigs: int
+
+
+def bcf_stream(n_sites: int, n_samples: int, seed: int, contig_len: int = 50_000_000):
+    """Uncompressed BCF: one contig, biallelic SNVs at sorted uniform positions,
+    INFO AC and AN (int32) and AF (float), FILTER PASS, GT for every sample with
+    alleles drawn from AF ~ Beta(0.1, 0.9). Returns (bytes, records_start)."""
+    rng = np.random.default_rng(seed)
+    names = "\t".join(f"S{i:05d}" for i in range(n_samples))
+    text = ("##fileformat=VCFv4.2\n"
+            '##FILTER=<ID=PASS,Description="All filters passed">\n'
+            f"##contig=<ID=1,length={contig_len}>\n"
+            '##INFO=<ID=AC,Number=A,Type=Integer,Description="Allele count">\n'
+            '##INFO=<ID=AN,Number=1,Type=Integer,Description="Allele number">\n'
+            '##INFO=<ID=AF,Number=A,Type=Float,Description="Allele frequency">\n'
+            '##FORMAT=<ID=GT,Number=1,Type=String,Description="Genotype">\n'
+            f"#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t{names}\n").encode() + b"\0"
+    head = b"BCF\x02\x02" + len(text).to_bytes(4, "little") + text
+    l_shared = 24 + 1 + 2 + 2 + 2 + 3 * 7
+    l_indiv = 2 + 1 + 2 * n_samples
+    width = 8 + l_shared + l_indiv
+    rec = np.zeros((n_sites, width), dtype=np.uint8)
+
+    def put(col, values, dtype):
+        v = np.ascontiguousarray(values, dtype=dtype).view(np.uint8).reshape(n_sites, -1)
+        rec[:, col:col + v.shape[1]] = v
+
+    pos = np.sort(rng.integers(0, contig_len, n_sites)).astype(np.int32)
+    af = rng.beta(0.1, 0.9, n_sites)
+    hap = (rng.random((n_sites, 2 * n_samples)) < af[:, None]).astype(np.uint8)
+    ac = hap.sum(axis=1).astype(np.int32)
+    an = np.full(n_sites, 2 * n_samples, np.int32)
+    bases = np.frombuffer(b"ACGT", np.uint8)
+    ref = rng.integers(0, 4, n_sites)
+    alt = (ref + rng.integers(1, 4, n_sites)) % 4
+    put(0, np.full(n_sites, l_shared), np.uint32)
+    put(4, np.full(n_sites, l_indiv), np.uint32)
+    put(8, np.zeros(n_sites), np.int32)                       # CHROM 0
+    put(12, pos, np.int32)
+    put(16, np.ones(n_sites), np.int32)                       # rlen 1
+    put(20, np.full(n_sites, 0x7F800001), np.uint32)          # QUAL missing
+    put(24, np.full(n_sites, 3), np.uint16)                   # n_info
+    put(26, np.full(n_sites, 2), np.uint16)                   # n_allele
+    rec[:, 28:31] = np.frombuffer(n_samples.to_bytes(3, "little"), np.uint8)
+    rec[:, 31] = 1                                            # n_fmt
+    c = 32
+    rec[:, c] = 0x07; c += 1                                  # ID missing
+    rec[:, c] = 0x17; rec[:, c + 1] = bases[ref]; c += 2      # REF
+    rec[:, c] = 0x17; rec[:, c + 1] = bases[alt]; c += 2      # ALT
+    rec[:, c] = 0x11; rec[:, c + 1] = 0; c += 2               # FILTER PASS
+    for key, typ, val in ((1, 0x13, ac), (2, 0x13, an), (3, 0x15, (ac / an).astype(np.float32))):
+        rec[:, c] = 0x11; rec[:, c + 1] = key; rec[:, c + 2] = typ
+        put(c + 3, val, np.float32 if typ == 0x15 else np.int32)
+        c += 7
+    assert c == 8 + l_shared
+    rec[:, c] = 0x11; rec[:, c + 1] = 4; rec[:, c + 2] = 0x21; c += 3   # FORMAT GT, 2 x int8
+    gt = ((hap.astype(np.uint8) + 1) << 1)                    # unphased allele index + 1
+    rec[:, c:c + 2 * n_samples] = gt
+    return np.concatenate([np.frombuffer(head, np.uint8), rec.reshape(-1)]), len(head)
+
+
+def config_bcf(n_sites=None, n_samples=2504, threads=None) -> SynthBcf:
+    """BCF half of the mixed BCF + FASTQ configuration: 200,000 biallelic sites
+    x 2,504 diploid samples (GT only), bcftools framing (65,280-byte payloads,
+    level 6); `n_sites` scales it down for tests."""
+    n = n_sites or 200_000
+    u, rs = bcf_stream(n, n_samples, CONFIG_SEED + 4)
+    c = bgzf(u, level=6, payload_cap=65280, seed=CONFIG_SEED + 4, threads=threads)
+    return SynthBcf(c, u, rs, n, n_samples, 1)
