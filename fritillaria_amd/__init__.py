@@ -11,5 +11,5 @@ from .hip import (  # noqa: F401
     InflateBatch, InflateError, InvalidBlock, InvalidDeviceBatch, Malformed, SizeMismatch, SPAN_DTYPE,
     BatchedPipeline, FASTQ_COLUMNS, FastqRecordBatch, FastqScanner, SpeculativeScan, bam_parse_header,
     crc32_combine, device_is_available, discover, select_codec, QueryServer, QUERY_RESULT_DTYPE, TO_END,
-    device_memory, BCF_COLUMNS, BcfRecordBatch, BcfScanner, bcf_parse_header,
+    device_memory, BamSoa, SOA_COLUMNS, AUX_ENTRY_DTYPE, BCF_COLUMNS, BcfRecordBatch, BcfScanner, bcf_parse_header,
 )

@@ -223,6 +223,7 @@ void ctx_release(frhip_ctx *c)
     c->status_scratch.release();
     c->bam_scratch.release();
     c->bcf_scratch.release();
+    c->soa_scratch.release();
     c->fq_small.release();
     c->fq_big.release();
     c->pinned.release();

@@ -18,6 +18,8 @@
 #include "frhip_internal.h"
 #include "host_internal.h"
 #include "bam_layout.h"
+#include "pipeline_internal.h"
+#include "soa_internal.h"
 
 using namespace frhip;
 
@@ -359,4 +361,113 @@ extern "C" int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_bu
         r->carry_start = 0;
         return FRHIP_OK;
     }
+}
+
+// ---- structure-of-arrays extras (resolved CIGAR, packed sequence, aux index) ----
+
+static const int SOA_WIDTHS[FRHIP_SOA_N_COLUMNS] = {8, 4, 8, 1, 8, 16};
+
+struct frhip_bam_soa {
+    uint64_t n = 0;
+    frhip_buf *cols[FRHIP_SOA_N_COLUMNS] = {};
+};
+
+extern "C" void frhip_bam_soa_free(frhip_bam_soa *s)
+{
+    if (!s) return;
+    for (auto *c : s->cols) buf_free(c);
+    delete s;
+}
+extern "C" uint64_t frhip_bam_soa_len(const frhip_bam_soa *s) { return s->n; }
+extern "C" const frhip_buf *frhip_bam_soa_column(const frhip_bam_soa *s, int column)
+{
+    if (!s || column < 0 || column >= FRHIP_SOA_N_COLUMNS) return nullptr;
+    return s->cols[column];
+}
+
+extern "C" int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const frhip_bam_batch *records,
+                             frhip_bam_soa **out, frhip_error *err)
+{
+    *out = nullptr;
+    frhip_bam_soa *o = new (std::nothrow) frhip_bam_soa;
+    if (!o) return set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+    const uint64_t n = records ? records->n : 0;
+    o->n = n;
+    (void)hipSetDevice(ctx->ordinal);
+    hipStream_t st = ctx->stream;
+    auto fail = [&](int rc) {
+        frhip_bam_soa_free(o);
+        return rc;
+    };
+    if (n == 0 || !data) {
+        const uint64_t lens[FRHIP_SOA_N_COLUMNS] = {8, 0, 8, 0, 8, 0};
+        for (int c = 0; c < FRHIP_SOA_N_COLUMNS; ++c) {
+            o->cols[c] = buf_alloc(ctx, lens[c], err);
+            if (!o->cols[c]) return fail(err ? err->kind : FRHIP_E_DEVICE);
+            if (lens[c]) {
+                hipError_t r = hipMemsetAsync(o->cols[c]->ptr, 0, 8, st);
+                if (r == hipSuccess) r = hipStreamSynchronize(st);
+                if (r != hipSuccess) return fail(hip_err(err, r, "empty offsets"));
+            }
+        }
+        *out = o;
+        return FRHIP_OK;
+    }
+    // the three offset columns are outputs; the per-record counts live in scratch
+    for (int c : {FRHIP_SOA_CIGAR_OFFSETS, FRHIP_SOA_SEQUENCE_OFFSETS, FRHIP_SOA_AUX_OFFSETS}) {
+        o->cols[c] = buf_alloc(ctx, (n + 1) * 8, err);
+        if (!o->cols[c]) return fail(err ? err->kind : FRHIP_E_DEVICE);
+    }
+    int rc;
+    const uint64_t cnt_bytes = ((n * 4 + 255) / 256) * 256;
+    uint8_t *s = (uint8_t *)ctx->soa_scratch.get(4 * cnt_bytes + 256, err, &rc);
+    if (!s) return fail(rc);
+    SoaArgs a{};
+    a.buf = (const uint8_t *)data->ptr;
+    a.record_offsets = (const uint64_t *)records->cols[FRHIP_BAM_RECORD_OFFSETS]->ptr;
+    a.n = n;
+    a.n_ops = (uint32_t *)s;
+    a.seq_bytes = (uint32_t *)(s + cnt_bytes);
+    a.n_aux = (uint32_t *)(s + 2 * cnt_bytes);
+    a.cg_at = (uint32_t *)(s + 3 * cnt_bytes);
+    a.first_bad = (unsigned long long *)(s + 4 * cnt_bytes);
+    a.op_off = (const uint64_t *)o->cols[FRHIP_SOA_CIGAR_OFFSETS]->ptr;
+    a.seq_off = (const uint64_t *)o->cols[FRHIP_SOA_SEQUENCE_OFFSETS]->ptr;
+    a.aux_off = (const uint64_t *)o->cols[FRHIP_SOA_AUX_OFFSETS]->ptr;
+    hipError_t r = launch_fill_u64((uint64_t *)a.first_bad, ~0ull, 1, st);
+    if (r == hipSuccess) r = launch_soa_count(a, st);
+    if (r == hipSuccess) r = launch_scan_u32_u64(a.n_ops, (uint64_t *)a.op_off, nullptr, n, st);
+    if (r == hipSuccess) r = launch_scan_u32_u64(a.seq_bytes, (uint64_t *)a.seq_off, nullptr, n, st);
+    if (r == hipSuccess) r = launch_scan_u32_u64(a.n_aux, (uint64_t *)a.aux_off, nullptr, n, st);
+    uint64_t tot[3];
+    unsigned long long bad = ~0ull;
+    if (r == hipSuccess) r = hipMemcpyAsync(&tot[0], a.op_off + n, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(&tot[1], a.seq_off + n, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(&tot[2], a.aux_off + n, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(&bad, a.first_bad, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return fail(hip_err(err, r, "SoA counts"));
+    if (bad != ~0ull) {
+        uint64_t at = 0;
+        (void)hipMemcpy(&at, a.record_offsets + bad, 8, hipMemcpyDeviceToHost);
+        int k = set_err(err, FRHIP_E_MALFORMED, at,
+                        "record %llu: malformed aux field, or a long-CIGAR placeholder without CG:B:I",
+                        (unsigned long long)bad);
+        if (err) std::strcpy(err->format, "bam");
+        return fail(k);
+    }
+    const uint64_t sizes[3] = {tot[0] * 4, tot[1], tot[2] * 16};
+    const int data_cols[3] = {FRHIP_SOA_CIGAR_OPS, FRHIP_SOA_SEQUENCE_PACKED, FRHIP_SOA_AUX_ENTRIES};
+    for (int k = 0; k < 3; ++k) {
+        o->cols[data_cols[k]] = buf_alloc(ctx, sizes[k], err);
+        if (!o->cols[data_cols[k]]) return fail(err ? err->kind : FRHIP_E_DEVICE);
+    }
+    a.ops = (uint32_t *)o->cols[FRHIP_SOA_CIGAR_OPS]->ptr;
+    a.seq = (uint8_t *)o->cols[FRHIP_SOA_SEQUENCE_PACKED]->ptr;
+    a.aux = (AuxEntry *)o->cols[FRHIP_SOA_AUX_ENTRIES]->ptr;
+    r = launch_soa_fill(a, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return fail(hip_err(err, r, "SoA fill"));
+    *out = o;
+    return FRHIP_OK;
 }

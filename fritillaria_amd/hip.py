@@ -157,6 +157,10 @@ def lib():
         "frhip_bam_batch_column": (_vp, [_vp, C.c_int]),
         "frhip_bam_batch_free": (None, [_vp]),
         "frhip_bam_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32), _ep]),
+        "frhip_bam_soa_build": (C.c_int, [_vp, _vp, _vp, C.POINTER(_vp), _ep]),
+        "frhip_bam_soa_len": (_u64, [_vp]),
+        "frhip_bam_soa_column": (_vp, [_vp, C.c_int]),
+        "frhip_bam_soa_free": (None, [_vp]),
         "frhip_bcf_decode": (C.c_int, [_vp, _vp, _vp, _u64, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp), _ep]),
         "frhip_bcf_batch_len": (_u64, [_vp]),
         "frhip_bcf_batch_tail": (_u64, [_vp]),
@@ -551,11 +555,64 @@ class DeviceRecordBatch:
             pass
 
 
+AUX_ENTRY_DTYPE = np.dtype([("tag", "S2"), ("type", "u1"), ("subtype", "u1"), ("value_offset", "<u4"),
+                            ("count", "<u4"), ("size", "<u4")])
+SOA_COLUMNS = [("cigar_offsets", np.uint64), ("cigar_ops", np.uint32), ("sequence_offsets", np.uint64),
+               ("sequence_packed", np.uint8), ("aux_offsets", np.uint64), ("aux_entries", AUX_ENTRY_DTYPE)]
+_SOA_INDEX = {n: i for i, (n, _) in enumerate(SOA_COLUMNS)}
+
+
+class BamSoa:
+    """Resolved CIGAR ops, packed sequences and the aux tag index of a decoded
+    BAM batch, as device columns (include/frhip.h, frhip_bam_soa_build)."""
+
+    def __init__(self, handle, records: "DeviceRecordBatch"):
+        self._h = handle
+        self._records = records
+
+    def __len__(self) -> int:
+        return int(lib().frhip_bam_soa_len(self._h))
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_bam_soa_column(self._h, _SOA_INDEX[name]), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = np.dtype(dict(SOA_COLUMNS)[name])
+        out = np.empty(buf.byte_len() // dt.itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def to_host(self) -> dict:
+        return {name: self.column_to_host(name) for name, _ in SOA_COLUMNS}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_bam_soa_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class BamDecoder:
     """BamDecoder (crates/fritillaria-cuda/src/bam.rs:103-187)."""
 
     def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
         self.ctx = context or HipContext(ordinal)
+
+    def soa(self, batch: DeviceInflateBatch, records: DeviceRecordBatch) -> BamSoa:
+        """Resolved CIGARs, packed sequences and the aux index of decoded records."""
+        data = batch.data()
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_bam_soa_build(self.ctx.handle, None if data is None else data._h, records._h, C.byref(out),
+                                       C.byref(e))
+        _raise(rc, e)
+        return BamSoa(out, records)
 
     def decode(self, batch: DeviceInflateBatch, start: int) -> DeviceRecordBatch:
         data = batch.data()

@@ -59,6 +59,7 @@ typedef struct frhip_buf frhip_buf;
 typedef struct frhip_bam_batch frhip_bam_batch;
 typedef struct frhip_fastq_batch frhip_fastq_batch;
 typedef struct frhip_bcf_batch frhip_bcf_batch;
+typedef struct frhip_bam_soa frhip_bam_soa;
 typedef struct frhip_reader frhip_reader;
 
 /* ---- device and context ------------------------------------------------ */
@@ -183,6 +184,34 @@ uint64_t frhip_bam_batch_fallbacks(const frhip_bam_batch *b);
 /* Column i, or NULL if absent. Owned by the batch. */
 const frhip_buf *frhip_bam_batch_column(const frhip_bam_batch *b, int column);
 void frhip_bam_batch_free(frhip_bam_batch *b);
+
+/* Structure-of-arrays extras over decoded records. No device counterpart in the
+   reference, which reads these per record on the host
+   (crates/fritillaria-bam/src/columnar/record.rs:220-361, aux.rs):
+   - CIGAR: the ops of Record::cigar_resolved, so a long-CIGAR placeholder
+     ("<l_seq>S <n>N", two ops) is replaced by its CG:B:I array; record i owns
+     ops [CIGAR_OFFSETS[i], CIGAR_OFFSETS[i+1]), u32 len << 4 | op each;
+   - SEQUENCE: Record::sequence_packed (4-bit, high nibble first) back to back;
+   - AUX: one 16-byte entry per aux field, in order: tag[2], type, B subtype
+     (else 0), u32 value offset from the record start, u32 element count
+     (scalars 1; Z/H string length without the NUL; B array count), u32 value
+     bytes (Z/H with the NUL).
+   Offset columns have n+1 u64 entries. A record with a malformed aux field, or
+   with the placeholder and no CG:B:I, is FRHIP_E_MALFORMED at its offset. */
+typedef enum {
+    FRHIP_SOA_CIGAR_OFFSETS = 0,    /* u64, n + 1 */
+    FRHIP_SOA_CIGAR_OPS = 1,        /* u32 */
+    FRHIP_SOA_SEQUENCE_OFFSETS = 2, /* u64, n + 1 (bytes) */
+    FRHIP_SOA_SEQUENCE_PACKED = 3,  /* u8 */
+    FRHIP_SOA_AUX_OFFSETS = 4,      /* u64, n + 1 (entries) */
+    FRHIP_SOA_AUX_ENTRIES = 5,      /* 16 bytes each */
+    FRHIP_SOA_N_COLUMNS = 6
+} frhip_soa_column;
+int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const frhip_bam_batch *records, frhip_bam_soa **out,
+                  frhip_error *err);
+uint64_t frhip_bam_soa_len(const frhip_bam_soa *s);
+const frhip_buf *frhip_bam_soa_column(const frhip_bam_soa *s, int column);
+void frhip_bam_soa_free(frhip_bam_soa *s);
 
 /* BAM header (crates/fritillaria-bam/src/columnar/header.rs:43-103) on host bytes. */
 int frhip_bam_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start,

@@ -95,6 +95,7 @@ def lib():
         _lib.or_bcf_decode_columns.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64] + [C.c_void_p] * 14 + \
             [C.POINTER(C.c_uint64)]
         _lib.or_bcf_validate.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
+        _lib.or_bam_soa.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_void_p] * 9 + [C.POINTER(C.c_uint64)]
         _lib.or_mt_bam_columns.restype = C.c_int
         _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
         del u8p
@@ -340,3 +341,35 @@ def bcf_looks_like(buf, pos: int, samples: int, contigs: int) -> bool:
 def bcf_validate(buf, pos: int) -> bool:
     a = _as_u8(buf)
     return bool(lib().or_bcf_validate(_ptr(a), a.size, pos))
+
+
+AUX_ENTRY_DTYPE = np.dtype([("tag", "S2"), ("type", "u1"), ("subtype", "u1"), ("value_offset", "<u4"),
+                            ("count", "<u4"), ("size", "<u4")])
+
+
+def bam_soa(buf, offsets):
+    """Resolved CIGAR ops, packed sequences and the aux tag index of BAM records
+    (or_bam_soa). Returns (rc, dict of arrays, first failing record)."""
+    a = _as_u8(buf)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    n = offs.size
+    n_ops = np.zeros(max(n, 1), np.uint32)
+    seq_bytes = np.zeros(max(n, 1), np.uint32)
+    n_aux = np.zeros(max(n, 1), np.uint32)
+    bad = C.c_uint64(0)
+    f = lib().or_bam_soa
+    rc = f(_ptr(a), _ptr(offs), n, _ptr(n_ops), _ptr(seq_bytes), _ptr(n_aux), None, None, None, None, None, None,
+           C.byref(bad))
+    if rc != 0:
+        return rc, None, bad.value
+    op_off = np.concatenate([[0], np.cumsum(n_ops[:n], dtype=np.uint64)]).astype(np.uint64)
+    seq_off = np.concatenate([[0], np.cumsum(seq_bytes[:n], dtype=np.uint64)]).astype(np.uint64)
+    aux_off = np.concatenate([[0], np.cumsum(n_aux[:n], dtype=np.uint64)]).astype(np.uint64)
+    ops = np.zeros(max(int(op_off[-1]), 1), np.uint32)
+    seq = np.zeros(max(int(seq_off[-1]), 1), np.uint8)
+    aux = np.zeros(max(int(aux_off[-1]), 1), AUX_ENTRY_DTYPE)
+    rc = f(_ptr(a), _ptr(offs), n, _ptr(n_ops), _ptr(seq_bytes), _ptr(n_aux), _ptr(op_off), _ptr(ops),
+           _ptr(seq_off), _ptr(seq), _ptr(aux_off), _ptr(aux), C.byref(bad))
+    return rc, {"cigar_offsets": op_off, "cigar_ops": ops[: int(op_off[-1])],
+                "sequence_offsets": seq_off, "sequence_packed": seq[: int(seq_off[-1])],
+                "aux_offsets": aux_off, "aux_entries": aux[: int(aux_off[-1])]}, bad.value

@@ -869,3 +869,120 @@ int or_bcf_validate(const uint8_t *buf, uint64_t len, uint64_t pos)
     }
     return at == end;
 }
+
+/* ------------------------------------------------------------------------ */
+/* BAM structure-of-arrays extras: resolved CIGAR ops, packed sequence, aux  */
+/* tag index.                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* One aux field at p of the record's aux block [.., end) (crates/fritillaria-bam/src/columnar/aux.rs
+   read_value / read_array / Fields): writes the entry and returns the next
+   field's offset, or 0 when the field is malformed. Entry: tag[2], type,
+   subtype ('B' only), value offset (relative to the record), element count
+   (1 for scalars, the string length without its NUL for Z/H, the array count
+   for B), value bytes (NUL included for Z/H). */
+static uint64_t aux_field(const uint8_t *r, uint64_t p, uint64_t end, or_aux_entry_t *e)
+{
+    if (p + 3 > end) return 0;
+    const uint8_t ty = r[p + 2];
+    uint64_t v = p + 3, size, count = 1;
+    uint8_t sub = 0;
+    switch (ty) {
+    case 'A': case 'c': case 'C': size = 1; break;
+    case 's': case 'S': size = 2; break;
+    case 'i': case 'I': case 'f': size = 4; break;
+    case 'Z': case 'H': {
+        uint64_t q = v;
+        while (q < end && r[q] != 0) ++q;
+        if (q >= end) return 0;
+        count = q - v;
+        size = count + 1;
+        if (ty == 'H') {
+            if (count % 2) return 0;
+            for (uint64_t k = v; k < q; ++k) {
+                uint8_t c = r[k];
+                if (!((c >= '0' && c <= '9') || (c >= 'a' && c <= 'f') || (c >= 'A' && c <= 'F'))) return 0;
+            }
+        }
+        break;
+    }
+    case 'B': {
+        if (v + 5 > end) return 0;
+        sub = r[v];
+        uint32_t w = sub == 'c' || sub == 'C' ? 1 : sub == 's' || sub == 'S' ? 2
+                   : sub == 'i' || sub == 'I' || sub == 'f' ? 4 : 0;
+        if (!w) return 0;
+        count = rd32(r + v + 1);
+        v += 5;
+        size = count * w;
+        break;
+    }
+    default: return 0;
+    }
+    if (v + size > end) return 0;
+    if (e) {
+        e->tag[0] = r[p];
+        e->tag[1] = r[p + 1];
+        e->type = ty;
+        e->subtype = sub;
+        e->value_offset = (uint32_t)v;
+        e->count = (uint32_t)count;
+        e->size = (uint32_t)size;
+    }
+    return v + size;
+}
+
+/*
+ * Per record (offsets from a scan): the resolved CIGAR (Record::cigar_resolved,
+ * crates/fritillaria-bam/src/columnar/record.rs:316-361: the placeholder
+ * "<l_seq>S <n>N" with exactly two ops is replaced by the CG:B:I array, which
+ * must exist), the packed 4-bit sequence and the aux fields. Counts go to
+ * n_ops / seq_bytes / n_aux (per record); with non-NULL outputs the data are
+ * written at the given prefix offsets. Returns 0, or OR_E_MALFORMED with
+ * *bad = first failing record.
+ */
+int or_bam_soa(const uint8_t *buf, const uint64_t *offsets, uint64_t n, uint32_t *n_ops, uint32_t *seq_bytes,
+               uint32_t *n_aux, const uint64_t *op_off, uint32_t *ops, const uint64_t *seq_off, uint8_t *seq,
+               const uint64_t *aux_off, or_aux_entry_t *aux, uint64_t *bad)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t *r = buf + offsets[i];
+        const uint64_t end = 4 + (uint64_t)rd32(r);
+        const uint32_t name_len = r[12], nc = rd16(r + 16), ls = rd32(r + 20);
+        const uint64_t c0 = 36 + name_len, s0 = c0 + 4ull * nc, q0 = s0 + (ls + 1) / 2, a0 = q0 + ls;
+        uint32_t k = 0;
+        uint64_t cg = 0, cg_count = 0;
+        int cg_ok = 0;
+        for (uint64_t p = a0; p < end;) {
+            or_aux_entry_t e;
+            uint64_t nx = aux_field(r, p, end, &e);
+            if (!nx) { *bad = i; return OR_E_MALFORMED; }
+            if (aux) aux[aux_off[i] + k] = e;
+            if (e.tag[0] == 'C' && e.tag[1] == 'G' && !cg) {
+                cg = e.value_offset;
+                cg_count = e.count;
+                cg_ok = e.type == 'B' && e.subtype == 'I';
+                cg = cg ? cg : 1;
+            }
+            ++k;
+            p = nx;
+        }
+        n_aux[i] = k;
+        int placeholder = 0;
+        if (nc == 2) {
+            uint32_t o0 = rd32(r + c0), o1 = rd32(r + c0 + 4);
+            placeholder = (o0 & 15) == 4 && (o0 >> 4) == ls && (o1 & 15) == 3;
+        }
+        if (placeholder) {
+            if (!cg || !cg_ok) { *bad = i; return OR_E_MALFORMED; }
+            n_ops[i] = (uint32_t)cg_count;
+            if (ops) for (uint64_t j = 0; j < cg_count; ++j) ops[op_off[i] + j] = rd32(r + cg + 4 * j);
+        } else {
+            n_ops[i] = nc;
+            if (ops) for (uint32_t j = 0; j < nc; ++j) ops[op_off[i] + j] = rd32(r + c0 + 4ull * j);
+        }
+        seq_bytes[i] = (ls + 1) / 2;
+        if (seq) memcpy(seq + seq_off[i], r + s0, (ls + 1) / 2);
+    }
+    return 0;
+}

@@ -100,6 +100,19 @@ int or_bcf_decode_columns(const uint8_t *buf, const uint64_t *offsets, uint64_t 
                           uint32_t *genotypes_start, uint32_t *rec_end, uint64_t *bad);
 int or_bcf_validate(const uint8_t *buf, uint64_t len, uint64_t pos);
 
+typedef struct {
+    uint8_t tag[2];
+    uint8_t type;
+    uint8_t subtype;
+    uint32_t value_offset;
+    uint32_t count;
+    uint32_t size;
+} or_aux_entry_t;
+
+int or_bam_soa(const uint8_t *buf, const uint64_t *offsets, uint64_t n, uint32_t *n_ops, uint32_t *seq_bytes,
+               uint32_t *n_aux, const uint64_t *op_off, uint32_t *ops, const uint64_t *seq_off, uint8_t *seq,
+               const uint64_t *aux_off, or_aux_entry_t *aux, uint64_t *bad);
+
 #ifdef __cplusplus
 }
 #endif
