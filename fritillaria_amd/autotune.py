@@ -45,7 +45,7 @@ def autotune(ctx: hip.HipContext, compressed: np.ndarray, *, sample_bytes=512 <<
                     best = ms if best is None or ms["total"] < best["total"] else best
                 p.close()
                 rec = {"inflate_schedule": sched, "ring": table[sched][0], "waves_per_group": table[sched][1],
-                       "waves_per_cu": table[sched][2], "order": order, "bam_window": window,
+                       "waves_per_cu": table[sched][2], "tokens": table[sched][3], "order": order, "bam_window": window,
                        "inflate_ms": best["inflate"], "total_ms": best["total"]}
                 results.append(rec)
                 if log:

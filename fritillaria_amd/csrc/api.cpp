@@ -250,13 +250,15 @@ extern "C" int frhip_ctx_set_inflate_schedule(frhip_ctx *c, int schedule)
     c->inflate_schedule = schedule;
     return FRHIP_OK;
 }
-extern "C" int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, int cap)
+extern "C" int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu,
+                                       uint32_t *tokens, int cap)
 {
     int k = cap < N_INFLATE_SCHEDULES ? cap : N_INFLATE_SCHEDULES;
     for (int i = 0; i < k; ++i) {
         if (ring) ring[i] = INFLATE_SCHEDULES[i].ring;
         if (waves_per_group) waves_per_group[i] = INFLATE_SCHEDULES[i].waves_per_group;
         if (waves_per_cu) waves_per_cu[i] = inflate_waves_per_cu(i);
+        if (tokens) tokens[i] = INFLATE_SCHEDULES[i].tokens;
     }
     return N_INFLATE_SCHEDULES;
 }

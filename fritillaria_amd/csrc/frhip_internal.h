@@ -27,6 +27,7 @@ struct InflateArgs {
 struct InflateSchedule {
     uint32_t ring;
     uint32_t waves_per_group;
+    uint32_t tokens; // 1: decode into tokens, resolve 64 at a time
 };
 extern const InflateSchedule INFLATE_SCHEDULES[];
 extern const int N_INFLATE_SCHEDULES;

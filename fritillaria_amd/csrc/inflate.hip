@@ -99,6 +99,7 @@ struct alignas(16) WaveMem {
     uint16_t lcount[16];
     uint16_t dcount[16];
     uint16_t ccount[16];
+    uint32_t tok[64];   // decoded, not yet resolved symbols (token form)
     uint8_t lens[320];
 };
 
@@ -411,7 +412,90 @@ struct Geometry {
     static constexpr uint32_t FLUSH_AT = RING / 2;       // unflushed bytes that trigger a write-back
     static constexpr uint32_t CHUNK_MAX = FLUSH_AT + 258; // most bytes one write-back can carry
     static constexpr uint32_t SEG = (CHUNK_MAX + 63) / 64; // CRC segment per lane
+    // token form: write-back at RING/4 unflushed bytes at a chunk start, and a
+    // chunk of tokens stops once it has produced RING/4 - 258 bytes, so a chunk
+    // adds under RING/4 and one write-back never carries more than RING/2
+    static constexpr uint32_t TOK_FLUSH_AT = RING / 4;
+    static constexpr uint32_t TOK_LIMIT = RING / 4 - 258;
 };
+
+// Token form of a symbol: a literal is its byte; a match is
+// TOK_MATCH | length << 16 | distance.
+constexpr uint32_t TOK_MATCH = 1u << 31;
+// Matches longer than this are copied by all 64 lanes in the ordered pass.
+constexpr uint32_t TOK_SHORT = 16;
+
+// Writes the output of tokens [0, ntok) that start at chunk_pos. First every
+// lane takes one token: literals, and matches whose source lies wholly before
+// the chunk, are written at once (their sources are final); sources at or
+// above `lim` are read from the ring, older ones from HBM, where the write-back
+// put them (the chunk's own writes only overwrite ring slots of positions below
+// lim). Matches that read this chunk's output, or are long, follow in token
+// order with all lanes copying.
+template <uint32_t RING>
+__device__ void resolve_tokens(uint8_t *ring, const uint32_t *tok, Out &o, uint32_t chunk_pos, uint32_t ntok,
+                               uint32_t lane)
+{
+    using G = Geometry<RING>;
+    WFENCE();
+    const bool live = lane < ntok;
+    const uint32_t t = live ? tok[lane] : 0;
+    const bool match = live && (t & TOK_MATCH);
+    const uint32_t ln = !live ? 0 : match ? (t >> 16) & 511 : 1;
+    uint32_t inc = ln;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, 64);
+        if ((int)lane >= d) inc += y;
+    }
+    const uint32_t end = chunk_pos + uni((uint32_t)__shfl(inc, 63, 64));
+    const uint32_t lim = end > RING ? end - RING : 0;
+    const uint32_t dst = chunk_pos + inc - ln;
+    const uint32_t dist = t & 0xFFFF;
+    const uint32_t src = dst - dist;
+    const bool deferred = match && (src + ln > chunk_pos || ln > TOK_SHORT);
+    if (__ballot(match && src < lim) != 0 && o.visible < o.flushed) {
+        // reads from HBM below: the write-back's stores must have completed
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        o.visible = o.flushed;
+    }
+    if (live && !match) {
+        ring[(o.skew + dst) & G::RMASK] = (uint8_t)t;
+    } else if (match && !deferred) {
+        for (uint32_t i = 0; i < ln; ++i) {
+            const uint32_t p = src + i;
+            const uint8_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
+            ring[(o.skew + dst + i) & G::RMASK] = v;
+        }
+    }
+    WFENCE();
+    uint64_t pending = __ballot(deferred);
+    while (pending) {
+        const uint32_t j = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+        pending &= pending - 1;
+        const uint32_t d0 = uni((uint32_t)__shfl(dst, j, 64));
+        const uint32_t L = uni((uint32_t)__shfl(ln, j, 64));
+        const uint32_t D = uni((uint32_t)__shfl(dist, j, 64));
+        if (D >= L) {
+            for (uint32_t off = lane; off < L; off += 64) {
+                const uint32_t p = d0 - D + off;
+                const uint8_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
+                ring[(o.skew + d0 + off) & G::RMASK] = v;
+            }
+        } else {
+            // period-D pattern: every source byte precedes d0 and lies in the ring
+            uint32_t r = lane % D;
+            const uint32_t step = 64u % D;
+            for (uint32_t off = lane; off < L; off += 64) {
+                ring[(o.skew + d0 + off) & G::RMASK] = ring[(o.skew + d0 - D + r) & G::RMASK];
+                r += step;
+                if (r >= D) r -= D;
+            }
+        }
+        WFENCE();
+    }
+}
 
 // Writes ring bytes [flushed, pos) to HBM and folds them into the CRC: lane l
 // takes the l-th of 64 SEG-byte segments counted back from the end (leading
@@ -460,7 +544,7 @@ __device__ void flush(const uint8_t *ring, const uint32_t *crctab, Out &o, uint3
     o.flushed = o.pos;
 }
 
-template <uint32_t RING>
+template <uint32_t RING, bool TOK>
 __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const uint8_t *gin, uint64_t in_off,
                                uint32_t in_len, uint8_t *gout, uint32_t cap, uint32_t lane, uint32_t lane_factor,
                                const uint32_t *x2n, uint32_t &status_out, uint32_t &produced_out, uint32_t &crc_out)
@@ -622,6 +706,82 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
             build_lut<DBITS>(sm.dlut, sm.dcount, sm.dsym, lane);
             tables = 2;
         }
+        if constexpr (TOK) {
+            // decode into tokens; resolve them 64 (or TOK_LIMIT bytes) at a time
+            uint32_t ntok = 0, chunk_pos = o.pos;
+            for (;;) {
+                b.buf = uni64(b.buf);
+                b.cnt = uni(b.cnt);
+                b.pos = uni(b.pos);
+                b.stage_end = uni(b.stage_end);
+                o.pos = uni(o.pos);
+                o.flushed = uni(o.flushed);
+                ntok = uni(ntok);
+                chunk_pos = uni(chunk_pos);
+                if (ntok >= 62 || o.pos - chunk_pos >= G::TOK_LIMIT) {
+                    resolve_tokens<RING>(sm.ring, sm.tok, o, chunk_pos, ntok, lane);
+                    ntok = 0;
+                    chunk_pos = o.pos;
+                }
+                if (ntok == 0 && o.pos - o.flushed >= G::TOK_FLUSH_AT)
+                    flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
+                if (b.cnt < 32) refill(sm.stage, b, lane);
+                const uint32_t e = uni(sm.plut[b.buf & ((1u << PBITS) - 1)]);
+                if ((e & PAIR) && ((e >> 14) & 15) <= b.cnt && o.pos + 2 <= o.cap) {
+                    const uint32_t odd = lane & 1;
+                    sm.tok[ntok + odd] = odd ? ((e >> 18) & 255) : ((e >> 4) & 255);
+                    const uint32_t tb = (e >> 14) & 15;
+                    b.buf >>= tb;
+                    b.cnt -= tb;
+                    o.pos += 2;
+                    ntok += 2;
+                    continue;
+                }
+                int sym = decode_litlen(b, e, sm.lcount, sm.lsym, err);
+                if (err) break;
+                if (sym < 256) {
+                    if (o.pos >= o.cap) {
+                        err = ST_OUTPUT_OVERFLOW;
+                        break;
+                    }
+                    sm.tok[ntok] = (uint32_t)sym;
+                    ++ntok;
+                    ++o.pos;
+                    continue;
+                }
+                if (sym == 256) break;
+                const uint32_t lc = (uint32_t)sym - 257;
+                if (lc >= 29) {
+                    err = ST_BAD_CODE;
+                    break;
+                }
+                const uint32_t len = len_base(lc) + getbits(b, len_extra(lc), err);
+                if (err) break;
+                if (b.cnt < 28) refill(sm.stage, b, lane);
+                int ds = decode_sym<DBITS>(b, sm.dlut, sm.dcount, sm.dsym, ST_BAD_DISTANCE, err);
+                if (err) break;
+                if (ds >= 30) {
+                    err = ST_BAD_DISTANCE;
+                    break;
+                }
+                const uint32_t dist = dist_base((uint32_t)ds) + getbits(b, dist_extra((uint32_t)ds), err);
+                if (err) break;
+                if (dist > o.pos) {
+                    err = ST_BAD_DISTANCE;
+                    break;
+                }
+                if (o.pos + len > o.cap) {
+                    err = ST_OUTPUT_OVERFLOW;
+                    break;
+                }
+                sm.tok[ntok] = TOK_MATCH | (len << 16) | dist;
+                ++ntok;
+                o.pos += len;
+            }
+            if (err) break;
+            if (ntok) resolve_tokens<RING>(sm.ring, sm.tok, o, chunk_pos, ntok, lane);
+            continue;
+        }
         // literal / length / distance codes until end-of-block
         for (;;) {
             // The state is the same in every lane; say so, so that it stays in
@@ -731,7 +891,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
 // NW waves per workgroup, each inflating its own members; the CRC table is
 // shared by the workgroup. Members come from the work-queue counter when one is
 // given, else member (block, wave) of a grid-stride walk.
-template <uint32_t RING, uint32_t NW>
+template <uint32_t RING, uint32_t NW, bool TOK>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void fr_inflate_kernel(InflateArgs a)
 {
     static_assert(RING >= 4096 && RING <= 32768, "ring size");
@@ -762,7 +922,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         uint32_t st, produced, crc;
         // member parameters are wave-uniform: keep them (and all the decode
         // derives from them) scalar; pointers stay typed, only offsets are made uniform
-        inflate_member<RING>(sm, crctab, a.in, uni64(a.in_off[m]), uni(a.in_len[m]), a.out + uni64(a.out_off[m]),
+        inflate_member<RING, TOK>(sm, crctab, a.in, uni64(a.in_off[m]), uni(a.in_len[m]), a.out + uni64(a.out_off[m]),
                              uni(a.out_cap[m]), lane, lane_factor, a.x2n, st, produced, crc);
         if (lane == 0) {
             a.status[m] = st;
@@ -813,17 +973,18 @@ hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const u
     return hipGetLastError();
 }
 
-template <uint32_t RING, uint32_t NW>
+template <uint32_t RING, uint32_t NW, bool TOK = false>
 static hipError_t launch_variant(const InflateArgs &a, uint32_t waves, hipStream_t stream)
 {
     uint32_t grid = (waves + NW - 1) / NW;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL((fr_inflate_kernel<RING, NW>), dim3(grid), dim3(64 * NW), 0, stream, a);
+    hipLaunchKernelGGL((fr_inflate_kernel<RING, NW, TOK>), dim3(grid), dim3(64 * NW), 0, stream, a);
     return hipGetLastError();
 }
 
 const InflateSchedule INFLATE_SCHEDULES[] = {
-    {32768, 1}, {16384, 1}, {16384, 2}, {8192, 1}, {8192, 2}, {8192, 4}, {4096, 2}, {4096, 4},
+    {32768, 1, 0}, {16384, 1, 0}, {16384, 2, 0}, {8192, 1, 0}, {8192, 2, 0}, {8192, 4, 0}, {4096, 2, 0}, {4096, 4, 0},
+    {8192, 2, 1}, {8192, 4, 1}, {4096, 2, 1}, {4096, 4, 1},
 };
 const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHEDULES[0]);
 
@@ -861,6 +1022,10 @@ hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stre
     case 5: return launch_variant<8192, 4>(a, waves, stream);
     case 6: return launch_variant<4096, 2>(a, waves, stream);
     case 7: return launch_variant<4096, 4>(a, waves, stream);
+    case 8: return launch_variant<8192, 2, true>(a, waves, stream);
+    case 9: return launch_variant<8192, 4, true>(a, waves, stream);
+    case 10: return launch_variant<4096, 2, true>(a, waves, stream);
+    case 11: return launch_variant<4096, 4, true>(a, waves, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -135,7 +135,7 @@ def lib():
         "frhip_ctx_set_bam_window": (C.c_int, [_vp, C.c_uint32]),
         "frhip_ctx_synchronize": (C.c_int, [_vp, _ep]),
         "frhip_ctx_set_inflate_schedule": (C.c_int, [_vp, C.c_int]),
-        "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+        "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
         "frhip_codec_name": (C.c_char_p, []),
         "frhip_device_codec_name": (C.c_char_p, []),
         "frhip_buf_len": (_u64, [_vp]),
@@ -230,12 +230,13 @@ def exported_symbols() -> list:
 
 
 def inflate_schedules() -> list:
-    """(ring bytes, waves per workgroup, resident waves per CU) per inflate variant."""
+    """(ring bytes, waves per workgroup, resident waves per CU, token form) per inflate variant."""
     ring = np.zeros(32, np.uint32)
     nw = np.zeros(32, np.uint32)
     wpc = np.zeros(32, np.uint32)
-    k = lib().frhip_inflate_schedules(_ptr(ring), _ptr(nw), _ptr(wpc), 32)
-    return [(int(ring[i]), int(nw[i]), int(wpc[i])) for i in range(k)]
+    tok = np.zeros(32, np.uint32)
+    k = lib().frhip_inflate_schedules(_ptr(ring), _ptr(nw), _ptr(wpc), _ptr(tok), 32)
+    return [(int(ring[i]), int(nw[i]), int(wpc[i]), int(tok[i])) for i in range(k)]
 
 
 def device_is_available() -> bool:

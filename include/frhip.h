@@ -89,8 +89,10 @@ int frhip_ctx_set_bam_window(frhip_ctx *ctx, uint32_t window);
    depend on it. No reference counterpart. */
 int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
 /* Writes up to cap entries of (ring bytes, waves per workgroup, resident waves
-   per CU); returns the number of schedules. */
-int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, int cap);
+   per CU, token form: 1 when the symbols are decoded into a token list and
+   resolved 64 at a time); returns the number of schedules. Any array may be NULL. */
+int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, uint32_t *tokens,
+                            int cap);
 /* Codec names of CudaCodec (lib.rs:152-294): host path and device path. */
 const char *frhip_codec_name(void);        /* "hip" */
 const char *frhip_device_codec_name(void); /* "hip-device" */
