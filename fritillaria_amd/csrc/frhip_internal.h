@@ -27,7 +27,7 @@ struct InflateArgs {
 struct InflateSchedule {
     uint32_t ring;
     uint32_t waves_per_group;
-    uint32_t tokens; // 1: decode into tokens, resolve 64 at a time
+    uint32_t tokens; // 0 scalar decode into the ring; 1 tokens, resolved 64 at a time; 2 lane-speculative tokens
 };
 extern const InflateSchedule INFLATE_SCHEDULES[];
 extern const int N_INFLATE_SCHEDULES;

@@ -433,13 +433,12 @@ constexpr uint32_t TOK_SHORT = 16;
 // lim). Matches that read this chunk's output, or are long, follow in token
 // order with all lanes copying.
 template <uint32_t RING>
-__device__ void resolve_tokens(uint8_t *ring, const uint32_t *tok, Out &o, uint32_t chunk_pos, uint32_t ntok,
-                               uint32_t lane)
+__device__ void resolve_lane_tokens(uint8_t *ring, uint32_t t_lane, Out &o, uint32_t chunk_pos, uint32_t ntok,
+                                    uint32_t lane)
 {
     using G = Geometry<RING>;
-    WFENCE();
     const bool live = lane < ntok;
-    const uint32_t t = live ? tok[lane] : 0;
+    const uint32_t t = live ? t_lane : 0;
     const bool match = live && (t & TOK_MATCH);
     const uint32_t ln = !live ? 0 : match ? (t >> 16) & 511 : 1;
     uint32_t inc = ln;
@@ -497,6 +496,15 @@ __device__ void resolve_tokens(uint8_t *ring, const uint32_t *tok, Out &o, uint3
     }
 }
 
+// Token list held in LDS (one per lane).
+template <uint32_t RING>
+__device__ void resolve_tokens(uint8_t *ring, const uint32_t *tok, Out &o, uint32_t chunk_pos, uint32_t ntok,
+                               uint32_t lane)
+{
+    WFENCE();
+    resolve_lane_tokens<RING>(ring, lane < ntok ? tok[lane] : 0, o, chunk_pos, ntok, lane);
+}
+
 // Writes ring bytes [flushed, pos) to HBM and folds them into the CRC: lane l
 // takes the l-th of 64 SEG-byte segments counted back from the end (leading
 // zeros do not change a zero-initialised CRC register), then the segments are
@@ -544,7 +552,55 @@ __device__ void flush(const uint8_t *ring, const uint32_t *crctab, Out &o, uint3
     o.flushed = o.pos;
 }
 
-template <uint32_t RING, bool TOK>
+// Canonical walk over the low bits of `bits` (codes up to 15 bits, `avail`
+// of them present): returns the code length, or 0 with err set when the input
+// ends inside the code or no code matches, as decode_litlen's slow path.
+__device__ __forceinline__ uint32_t walk_bits(uint32_t bits, uint32_t avail, const uint16_t *count, const uint16_t *sym,
+                                              uint32_t bad, uint32_t &out_sym, uint32_t &err)
+{
+    int code = 0, first = 0, index = 0;
+    for (int len = 1; len <= 15; ++len) {
+        if ((uint32_t)len > avail) {
+            err = ST_TRUNCATED;
+            return 0;
+        }
+        code |= (int)((bits >> (len - 1)) & 1);
+        int c = (int)uni(count[len]);
+        if (code - first < c) {
+            out_sym = uni(sym[index + code - first]);
+            return (uint32_t)len;
+        }
+        index += c;
+        first = (first + c) << 1;
+        code <<= 1;
+    }
+    err = bad;
+    return 0;
+}
+
+// Makes stage bytes [lo, lo + 24) resident (lo a multiple of 8, relative to sbase).
+__device__ __forceinline__ void ensure_span(uint32_t *stage, Bits &b, uint32_t lo, uint32_t lane)
+{
+    const uint32_t hi = lo + 24;
+    if (hi <= b.stage_end && lo + STAGE_BYTES >= b.stage_end) return;
+    if (b.stage_end != 0 && lo + CHUNK >= b.stage_end && hi <= b.stage_end + CHUNK) {
+        load_chunk(stage, b.sbase, b.stage_end, lane);
+        b.stage_end += CHUNK;
+    } else {
+        const uint32_t c0 = lo & ~(CHUNK - 1);
+        load_chunk(stage, b.sbase, c0, lane);
+        load_chunk(stage, b.sbase, c0 + CHUNK, lane);
+        b.stage_end = c0 + 2 * CHUNK;
+    }
+    WFENCE();
+}
+
+__device__ __forceinline__ uint32_t put_lane(uint32_t v, uint32_t at, uint32_t old)
+{
+    return __lane_id() == at ? v : old;
+}
+
+template <uint32_t RING, int TOK>
 __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const uint8_t *gin, uint64_t in_off,
                                uint32_t in_len, uint8_t *gout, uint32_t cap, uint32_t lane, uint32_t lane_factor,
                                const uint32_t *x2n, uint32_t &status_out, uint32_t &produced_out, uint32_t &crc_out)
@@ -706,7 +762,150 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
             build_lut<DBITS>(sm.dlut, sm.dcount, sm.dsym, lane);
             tables = 2;
         }
-        if constexpr (TOK) {
+        if constexpr (TOK == 2) {
+            // Lane-speculative decode. Each round, lane l looks up the code
+            // that would start l bits past the round's start B (one table read
+            // per lane, all at once); the scalar chain then hops from symbol to
+            // symbol by reading the lanes it lands on, and a symbol whose
+            // distance code would start past lane 63 waits for the next round.
+            // Symbols become tokens held one per lane in a register, resolved
+            // as in the token form.
+            const uint32_t total = b.len * 8;
+            uint32_t B = b.pos * 8 - b.cnt; // bits consumed, from the payload start
+            uint32_t tokv = 0, ntok = 0, chunk_pos = o.pos;
+            bool eob = false;
+            while (!eob) {
+                B = uni(B);
+                ntok = uni(ntok);
+                chunk_pos = uni(chunk_pos);
+                o.pos = uni(o.pos);
+                o.flushed = uni(o.flushed);
+                b.stage_end = uni(b.stage_end);
+                if (ntok > 48 || o.pos - chunk_pos >= G::TOK_LIMIT) {
+                    resolve_lane_tokens<RING>(sm.ring, tokv, o, chunk_pos, ntok, lane);
+                    ntok = 0;
+                    chunk_pos = o.pos;
+                }
+                if (ntok == 0 && o.pos - o.flushed >= G::TOK_FLUSH_AT)
+                    flush<RING>(sm.ring, crctab, o, lane, lane_factor, x2n);
+                // lane l: 32 bits from bit B + l
+                const uint32_t r0 = b.lead + (B >> 3);
+                ensure_span(sm.stage, b, r0 & ~7u, lane);
+                const uint32_t bit = (B & 7) + lane;
+                const uint32_t byte = r0 + (bit >> 3);
+                const uint32_t w = (byte >> 3) & (STAGE_WORDS / 2 - 1);
+                const uint2 x = reinterpret_cast<const uint2 *>(sm.stage)[w];
+                const uint2 y = reinterpret_cast<const uint2 *>(sm.stage)[(w + 1) & (STAGE_WORDS / 2 - 1)];
+                const uint64_t lo = ((uint64_t)x.y << 32) | x.x, hi = ((uint64_t)y.y << 32) | y.x;
+                const uint32_t o8 = (byte & 7) * 8;
+                const uint64_t v = o8 ? ((lo >> o8) | (hi << (64 - o8))) : lo;
+                const uint32_t fl = (uint32_t)(v >> (bit & 7));
+                const uint32_t ev = sm.plut[fl & ((1u << PBITS) - 1)];
+                const uint32_t dv = sm.dlut[fl & ((1u << DBITS) - 1)];
+                uint32_t p = 0;
+                while (p < 64 && ntok < 62 && o.pos - chunk_pos < G::TOK_LIMIT) {
+                    const uint32_t e = uni(__builtin_amdgcn_readlane(ev, p));
+                    const uint32_t f = uni(__builtin_amdgcn_readlane(fl, p));
+                    uint32_t L = e & 15, sym = (e >> 4) & 511;
+                    if (L) {
+                        if (B + p + L > total) {
+                            err = ST_TRUNCATED;
+                            break;
+                        }
+                    } else {
+                        L = walk_bits(f, total - (B + p), sm.lcount, sm.lsym, ST_BAD_CODE, sym, err);
+                        if (err) break;
+                    }
+                    if (sym < 256) {
+                        const uint32_t tb = (e >> 14) & 15;
+                        if ((e & PAIR) && B + p + tb <= total && o.pos + 2 <= o.cap) {
+                            tokv = put_lane((e >> 4) & 255, ntok, tokv);
+                            tokv = put_lane((e >> 18) & 255, ntok + 1, tokv);
+                            ntok += 2;
+                            o.pos += 2;
+                            p += tb;
+                            continue;
+                        }
+                        if (o.pos >= o.cap) {
+                            err = ST_OUTPUT_OVERFLOW;
+                            break;
+                        }
+                        tokv = put_lane(sym, ntok, tokv);
+                        ++ntok;
+                        ++o.pos;
+                        p += L;
+                        continue;
+                    }
+                    if (sym == 256) {
+                        p += L;
+                        eob = true;
+                        break;
+                    }
+                    const uint32_t lc = sym - 257;
+                    if (lc >= 29) {
+                        err = ST_BAD_CODE;
+                        break;
+                    }
+                    // length extra bits follow the code in this lane's 32 bits
+                    const uint32_t lx = len_extra(lc);
+                    if (B + p + L + lx > total) {
+                        err = ST_TRUNCATED;
+                        break;
+                    }
+                    const uint32_t len = len_base(lc) + ((f >> L) & ((1u << lx) - 1));
+                    const uint32_t q = p + L + lx;
+                    if (q >= 64) break; // distance code past the lanes: next round
+                    const uint32_t ed = uni(__builtin_amdgcn_readlane(dv, q));
+                    const uint32_t g = uni(__builtin_amdgcn_readlane(fl, q));
+                    uint32_t Ld = ed & 15, ds = ed >> 4;
+                    if (Ld) {
+                        if (B + q + Ld > total) {
+                            err = ST_TRUNCATED;
+                            break;
+                        }
+                    } else {
+                        Ld = walk_bits(g, total - (B + q), sm.dcount, sm.dsym, ST_BAD_DISTANCE, ds, err);
+                        if (err) break;
+                    }
+                    if (ds >= 30) {
+                        err = ST_BAD_DISTANCE;
+                        break;
+                    }
+                    const uint32_t dx = dist_extra(ds);
+                    if (B + q + Ld + dx > total) {
+                        err = ST_TRUNCATED;
+                        break;
+                    }
+                    const uint32_t dist = dist_base(ds) + ((g >> Ld) & ((1u << dx) - 1));
+                    if (dist > o.pos) {
+                        err = ST_BAD_DISTANCE;
+                        break;
+                    }
+                    if (o.pos + len > o.cap) {
+                        err = ST_OUTPUT_OVERFLOW;
+                        break;
+                    }
+                    tokv = put_lane(TOK_MATCH | (len << 16) | dist, ntok, tokv);
+                    ++ntok;
+                    o.pos += len;
+                    p = q + Ld + dx;
+                }
+                if (err) break;
+                B += p;
+            }
+            if (err) break;
+            if (ntok) resolve_lane_tokens<RING>(sm.ring, tokv, o, chunk_pos, ntok, lane);
+            // give the position back to the bit reader for the next block header
+            b.buf = 0;
+            b.cnt = 0;
+            b.pos = B >> 3;
+            if (B & 7) {
+                refill(sm.stage, b, lane);
+                (void)getbits(b, B & 7, err);
+            }
+            continue;
+        }
+        if constexpr (TOK == 1) {
             // decode into tokens; resolve them 64 (or TOK_LIMIT bytes) at a time
             uint32_t ntok = 0, chunk_pos = o.pos;
             for (;;) {
@@ -891,7 +1090,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
 // NW waves per workgroup, each inflating its own members; the CRC table is
 // shared by the workgroup. Members come from the work-queue counter when one is
 // given, else member (block, wave) of a grid-stride walk.
-template <uint32_t RING, uint32_t NW, bool TOK>
+template <uint32_t RING, uint32_t NW, int TOK>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void fr_inflate_kernel(InflateArgs a)
 {
     static_assert(RING >= 4096 && RING <= 32768, "ring size");
@@ -973,7 +1172,7 @@ hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const u
     return hipGetLastError();
 }
 
-template <uint32_t RING, uint32_t NW, bool TOK = false>
+template <uint32_t RING, uint32_t NW, int TOK = 0>
 static hipError_t launch_variant(const InflateArgs &a, uint32_t waves, hipStream_t stream)
 {
     uint32_t grid = (waves + NW - 1) / NW;
@@ -985,6 +1184,7 @@ static hipError_t launch_variant(const InflateArgs &a, uint32_t waves, hipStream
 const InflateSchedule INFLATE_SCHEDULES[] = {
     {32768, 1, 0}, {16384, 1, 0}, {16384, 2, 0}, {8192, 1, 0}, {8192, 2, 0}, {8192, 4, 0}, {4096, 2, 0}, {4096, 4, 0},
     {8192, 2, 1}, {8192, 4, 1}, {4096, 2, 1}, {4096, 4, 1},
+    {8192, 2, 2}, {8192, 4, 2}, {4096, 2, 2}, {4096, 4, 2},
 };
 const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHEDULES[0]);
 
@@ -1022,10 +1222,14 @@ hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stre
     case 5: return launch_variant<8192, 4>(a, waves, stream);
     case 6: return launch_variant<4096, 2>(a, waves, stream);
     case 7: return launch_variant<4096, 4>(a, waves, stream);
-    case 8: return launch_variant<8192, 2, true>(a, waves, stream);
-    case 9: return launch_variant<8192, 4, true>(a, waves, stream);
-    case 10: return launch_variant<4096, 2, true>(a, waves, stream);
-    case 11: return launch_variant<4096, 4, true>(a, waves, stream);
+    case 8: return launch_variant<8192, 2, 1>(a, waves, stream);
+    case 9: return launch_variant<8192, 4, 1>(a, waves, stream);
+    case 10: return launch_variant<4096, 2, 1>(a, waves, stream);
+    case 11: return launch_variant<4096, 4, 1>(a, waves, stream);
+    case 12: return launch_variant<8192, 2, 2>(a, waves, stream);
+    case 13: return launch_variant<8192, 4, 2>(a, waves, stream);
+    case 14: return launch_variant<4096, 2, 2>(a, waves, stream);
+    case 15: return launch_variant<4096, 4, 2>(a, waves, stream);
     default: return hipErrorInvalidValue;
     }
 }
