@@ -226,6 +226,7 @@ void ctx_release(frhip_ctx *c)
     c->soa_scratch.release();
     c->fq_small.release();
     c->fq_big.release();
+    c->rec_scratch.release();
     c->pinned.release();
     if (c->x2n) (void)hipFree(c->x2n);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -493,6 +494,12 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     a.queue = nullptr;
     a.order = nullptr;
     a.x2n = ctx->x2n;
+    void *d_rec = nullptr;
+    if (const uint64_t sb = inflate_scratch_bytes(ctx->inflate_schedule, total, n)) {
+        d_rec = ctx->rec_scratch.get(sb, err, &rc);
+        if (!d_rec) { buf_free(o); return rc; }
+    }
+    inflate_scratch_bind(a, d_rec, n);
     r = launch_inflate(a, (uint32_t)n, ctx->stream, ctx->inflate_schedule);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
     if (t) (void)hipEventRecord(e2, ctx->stream);

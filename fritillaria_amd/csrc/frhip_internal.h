@@ -21,13 +21,16 @@ struct InflateArgs {
     uint32_t *queue;         // work-queue counter (persistent form) or null
     const uint32_t *order;   // member order for the queue, or null
     const uint32_t *x2n;     // x^(2^k) mod P, k = 0..31
+    uint32_t *nrec;          // two-phase schedules: match records per member
+    uint32_t *rec;           // two-phase schedules: match record lists
 };
 
 // Inflate kernel variants: output ring bytes per member and waves per workgroup.
 struct InflateSchedule {
     uint32_t ring;
     uint32_t waves_per_group;
-    uint32_t tokens; // 0 scalar decode into the ring; 1 tokens, resolved 64 at a time; 2 lane-speculative tokens
+    uint32_t tokens; // 0 scalar decode into the ring; 1 tokens, resolved 64 at a time; 2 lane-speculative
+                     // tokens; 3 two-phase (lane-per-member decode, then wave-per-member copies)
 };
 extern const InflateSchedule INFLATE_SCHEDULES[];
 extern const int N_INFLATE_SCHEDULES;
@@ -35,6 +38,10 @@ constexpr int DEFAULT_INFLATE_SCHEDULE = 5;
 
 // Launches `waves` waves (one member each at a time) of schedule `schedule`.
 hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stream, int schedule);
+// Device scratch a schedule needs for n members of out_bytes inflated bytes in
+// total (0 for all but the two-phase schedules), and how it is laid out.
+uint64_t inflate_scratch_bytes(int schedule, uint64_t out_bytes, uint64_t n);
+void inflate_scratch_bind(InflateArgs &a, void *scratch, uint64_t n);
 uint32_t inflate_lds_bytes(int schedule);
 uint32_t inflate_waves_per_cu(int schedule);
 

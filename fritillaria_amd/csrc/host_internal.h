@@ -57,7 +57,8 @@ struct frhip_ctx {
     uint32_t *x2n = nullptr;
     uint32_t bam_window = 2048;
     int inflate_schedule = 5; // index into INFLATE_SCHEDULES
-    frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big;
+    frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
+        rec_scratch;
     frhip::PinnedScratch pinned;
 };
 

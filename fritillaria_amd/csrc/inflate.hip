@@ -1131,6 +1131,645 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-phase form: the Huffman decode runs one lane per member, the copies one
+// wave per member.
+//
+// Phase 1 (fr_inflate_lanes_kernel): lane t of the grid owns member t. Its
+// decode is per lane -- canonical limits in VGPRs, symbol lists sorted by
+// (code length, value) in lane-minor LDS -- so 64 members advance one symbol
+// per step on the vector unit. Literals and stored bytes go straight to their
+// output offsets; each match becomes a 32-bit record in a scratch list:
+// bits 23-31 the count of literal bytes since the previous record's span
+// (REC_SKIP: a record that only covers REC_SKIP literal bytes), bits 15-22
+// length - 3, bits 0-14 distance - 1. Status, produced bytes and the record
+// count are final after this phase; the CRC is left to phase 2.
+// Phase 2 (fr_inflate_resolve_kernel): one wave per member walks its records
+// 64 at a time. The output span they cover (literals already in place) is
+// read into the LDS ring, the matches are copied as in the token form, and
+// the ring is written back with the CRC folded in.
+
+constexpr uint32_t REC_SKIP = 511;
+constexpr uint32_t LQ = 14; // packed limits for code lengths 1..14
+
+// First record slot of member m. Each record covers at least 3 output bytes,
+// so a member of ISIZE c needs at most c / 3 slots, below its share here.
+__device__ __forceinline__ uint64_t rec_base(uint64_t out_off, uint32_t m) { return (out_off * 11) / 32 + 2ull * m; }
+
+// Canonical code in limit form. With v the next 15 bits read MSB-first, the
+// next code has length L exactly when lim[L-1] <= v < lim[L] (lim[0] = 0,
+// lim[k] = (first code of length k + count[k]) << (15 - k)), and its symbol is
+// entry offs[L] + ((v - lim[L-1]) >> (15 - L)) of the sorted list. q[k-1]
+// packs lim[k] << 16 | (k + 1) << 12 | offs[k + 1] for k = 1..14.
+struct LaneCode {
+    uint32_t q[LQ];
+    uint32_t lim15;
+};
+
+// Builds `c` from counts cnt(1..15) of n symbols; returns puff's construct
+// value: 0 complete (or no codes at all), > 0 incomplete, < 0 over-subscribed.
+template <typename Cnt>
+__device__ __forceinline__ int lane_code(LaneCode &c, uint32_t n, Cnt cnt)
+{
+    uint32_t code = 0, offs = 0;
+    int left = 1;
+#pragma unroll
+    for (int k = 1; k <= 15; ++k) {
+        const uint32_t ck = cnt(k);
+        left = (left << 1) - (int)ck;
+        const uint32_t lim = (code + ck) << (15 - k);
+        if (k < 15)
+            c.q[k - 1] = (lim << 16) | ((uint32_t)(k + 1) << 12) | (offs + ck);
+        else
+            c.lim15 = lim;
+        code = (code + ck) << 1;
+        offs += ck;
+    }
+    if (offs == 0) return 0;
+    (void)n;
+    return left;
+}
+
+// Sorted-list index of the code at the front of v (15 bits, MSB first); L = 0
+// when v starts with no code of this table.
+__device__ __forceinline__ uint32_t lane_find(const LaneCode &c, uint32_t v, uint32_t &L)
+{
+    // q[] increases with k, so the entry that applies is the largest q[k] <=
+    // v16; two independent chains (even and odd k) keep the compares apart
+    const uint32_t v16 = (v << 16) | 0xFFFFu;
+    uint32_t se = 1u << 12, so = 1u << 12;
+#pragma unroll
+    for (int k = 0; k < (int)LQ; k += 2) {
+        se = v16 >= c.q[k] ? c.q[k] : se;
+        so = v16 >= c.q[k + 1] ? c.q[k + 1] : so;
+    }
+    const uint32_t sel = se > so ? se : so;
+    const uint32_t len = (sel >> 12) & 15;
+    L = v < c.lim15 ? len : 0;
+    return (sel & 0xFFFu) + ((v - (sel >> 16)) >> (15 - len));
+}
+
+// Per-lane LSB-first bit reader. Input is staged in two 32-byte buffers: the
+// decode takes words only from A; B holds the next 32 bytes. B is read (moved
+// into A) and refetched only at points where the whole wave does so -- every
+// few decode steps, or in the header path -- because a wait for a load is a
+// wave-wide counter: a buffer that some lane reads every step must never have
+// a load in flight.
+struct LaneBits {
+    uint64_t bb;      // next bits, LSB first (may run past the payload)
+    uint32_t bc;      // bits in bb
+    uint32_t avail;   // payload bits not yet consumed
+    uint32_t k;       // next word of A (8: A used up)
+    uint4 A0, A1, B0, B1;
+    const uint4 *ip;  // next 32 bytes to fetch
+};
+
+__device__ __forceinline__ uint32_t pick(bool c, uint32_t x, uint32_t y)
+{
+    const uint32_t m = 0u - (uint32_t)c;
+    return (x & m) | (y & ~m);
+}
+
+// Word k (0..7) of A, by masks: a dynamic index (or a select the compiler
+// turns into one) would put the reader in scratch memory.
+#define LB_WORD(b)                                                                                         \
+    pick((b).k & 4, pick((b).k & 2, pick((b).k & 1, (b).A1.w, (b).A1.z), pick((b).k & 1, (b).A1.y, (b).A1.x)), \
+         pick((b).k & 2, pick((b).k & 1, (b).A0.w, (b).A0.z), pick((b).k & 1, (b).A0.y, (b).A0.x)))
+
+// Moves B into A and refetches B. The empty statement (with a memory
+// clobber) keeps the loads after the moves, so they land in B's own
+// registers. Macros, not functions: compiled on their own, the conditional
+// block moves would go through memory.
+#define LB_SWAP(b)                                                                                        \
+    do {                                                                                                  \
+        (b).A0 = (b).B0;                                                                                  \
+        (b).A1 = (b).B1;                                                                                  \
+        asm volatile("" : "+v"((b).A0.x), "+v"((b).A0.y), "+v"((b).A0.z), "+v"((b).A0.w), "+v"((b).A1.x),   \
+                     "+v"((b).A1.y), "+v"((b).A1.z), "+v"((b).A1.w)::"memory");                          \
+        (b).B0 = (b).ip[0];                                                                               \
+        (b).B1 = (b).ip[1];                                                                               \
+        (b).ip += 2;                                                                                      \
+        (b).k = 0;                                                                                        \
+    } while (0)
+
+// Tops bb up from A when it holds under 32 bits and A has words left.
+#define LB_REFILL(b)                                                                                      \
+    do {                                                                                                  \
+        if ((b).bc < 32 && (b).k < 8) {                                                                   \
+            (b).bb |= (uint64_t)LB_WORD(b) << (b).bc;                                                     \
+            (b).bc += 32;                                                                                 \
+            ++(b).k;                                                                                      \
+        }                                                                                                 \
+    } while (0)
+
+// Same, moving B in first when A is used up (waits for B's load).
+#define LB_REFILL_SYNC(b)                                                                                 \
+    do {                                                                                                  \
+        if ((b).bc < 32) {                                                                                \
+            if ((b).k == 8) LB_SWAP(b);                                                                   \
+            LB_REFILL(b);                                                                                 \
+        }                                                                                                 \
+    } while (0)
+
+// n <= 16 bits; bb must hold at least n bits.
+__device__ __forceinline__ uint32_t lb_bits(LaneBits &b, uint32_t n, uint32_t &err)
+{
+    if (n > b.avail) {
+        err = ST_TRUNCATED;
+        return 0;
+    }
+    const uint32_t v = (uint32_t)b.bb & ((1u << n) - 1);
+    b.bb >>= n;
+    b.bc -= n;
+    b.avail -= n;
+    return v;
+}
+
+// Next code of table c: its sorted-list index, or err (puff's order: input
+// that ends inside a code is truncated, else no code after 15 bits is `bad`).
+__device__ __forceinline__ uint32_t lb_decode(LaneBits &b, const LaneCode &c, uint32_t bad, uint32_t &err)
+{
+    const uint32_t v = __builtin_bitreverse32((uint32_t)b.bb) >> 17;
+    uint32_t L;
+    const uint32_t idx = lane_find(c, v, L);
+    if (L == 0 || L > b.avail) {
+        err = L == 0 && b.avail >= 15 ? bad : ST_TRUNCATED;
+        return 0;
+    }
+    b.bb >>= L;
+    b.bc -= L;
+    b.avail -= L;
+    return idx;
+}
+
+// Counts per code length (0..15) in 16-bit (F16) or 8-bit (F8) fields of
+// named 64-bit words: the length is per lane, and an array indexed by it
+// would live in scratch memory.
+struct F16 {
+    uint64_t w0, w1, w2, w3;
+};
+struct F8 {
+    uint64_t w0, w1;
+};
+__device__ __forceinline__ uint64_t sel_mask(bool c) { return 0ull - (uint64_t)c; }
+__device__ __forceinline__ uint32_t f16_get(const F16 &f, uint32_t L)
+{
+    const uint32_t g = L >> 2;
+    const uint64_t w = (f.w0 & sel_mask(g == 0)) | (f.w1 & sel_mask(g == 1)) | (f.w2 & sel_mask(g == 2)) |
+                       (f.w3 & sel_mask(g == 3));
+    return (uint32_t)(w >> ((L & 3) * 16)) & 0xFFFFu;
+}
+__device__ __forceinline__ void f16_add(F16 &f, uint32_t L, uint32_t v)
+{
+    const uint64_t d = (uint64_t)v << ((L & 3) * 16);
+    const uint32_t g = L >> 2;
+    f.w0 += d & sel_mask(g == 0);
+    f.w1 += d & sel_mask(g == 1);
+    f.w2 += d & sel_mask(g == 2);
+    f.w3 += d & sel_mask(g == 3);
+}
+__device__ __forceinline__ uint32_t f8_get(const F8 &f, uint32_t L)
+{
+    const uint64_t m = sel_mask(L < 8);
+    return (uint32_t)(((f.w0 & m) | (f.w1 & ~m)) >> ((L & 7) * 8)) & 0xFFu;
+}
+__device__ __forceinline__ void f8_add(F8 &f, uint32_t L, uint32_t v)
+{
+    const uint64_t d = (uint64_t)v << ((L & 7) * 8);
+    const uint64_t m = sel_mask(L < 8);
+    f.w0 += d & m;
+    f.w1 += d & ~m;
+}
+__device__ __forceinline__ void lb_copy(LaneBits &d, const LaneBits &s)
+{
+    d.bb = s.bb;
+    d.bc = s.bc;
+    d.avail = s.avail;
+    d.k = s.k;
+    d.A0 = s.A0;
+    d.A1 = s.A1;
+    d.B0 = s.B0;
+    d.B1 = s.B1;
+    d.ip = s.ip;
+}
+
+enum : uint32_t { LS_HEADER = 0, LS_CODES = 1, LS_DONE = 2 };
+
+template <uint32_t NW>
+__global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a)
+{
+    constexpr uint32_t LANES = 64 * NW;
+    __shared__ uint16_t lsym_all[288 * LANES]; // entry e of lane t at [e * LANES + t]
+    __shared__ uint8_t dsym_all[32 * LANES];
+    const uint32_t t = threadIdx.x;
+    const uint32_t slot = blockIdx.x * LANES + t;
+    if (slot >= a.n) return;
+    const uint32_t m = a.order ? a.order[slot] : slot;
+    uint16_t *lsym = lsym_all + t;
+    uint8_t *dsym = dsym_all + t;
+    const uint64_t in_off = a.in_off[m];
+    const uint32_t in_len = a.in_len[m];
+    const uint64_t out_off = a.out_off[m];
+    const uint32_t cap = a.out_cap[m];
+    uint8_t *out = a.out + out_off;
+    uint32_t *rec = a.rec + rec_base(out_off, m);
+
+    LaneBits b;
+    {
+        // the input base is 256-byte aligned; pointer arithmetic (not integer
+        // casts) keeps the loads in the global address space
+        const uint4 *blk = reinterpret_cast<const uint4 *>(a.in + (in_off & ~15ull));
+        b.A0 = blk[0];
+        b.A1 = blk[1];
+        b.B0 = blk[2];
+        b.B1 = blk[3];
+        b.ip = blk + 4;
+        const uint32_t byte = (uint32_t)(in_off & 15);
+        b.k = byte >> 2;
+        b.bb = 0;
+        b.bc = 0;
+        b.avail = in_len * 8;
+        LB_REFILL(b);
+        b.bb >>= (byte & 3) * 8;
+        b.bc -= (byte & 3) * 8;
+    }
+    LaneCode lc, dc;
+    uint32_t state = LS_HEADER, last = 0, err = ST_OK, pos = 0, nrec = 0, mark = 0;
+    // the member's parameters have landed: otherwise their first use inside
+    // the loop would leave a wait on the memory counter in every step
+    __builtin_amdgcn_s_waitcnt(0);
+    for (uint32_t step = 1; state != LS_DONE; ++step) {
+        // every 4th step (the same step in every lane): lanes that used up A
+        // take B and refetch it
+        if ((step & 3) == 0 && b.k == 8) LB_SWAP(b);
+        if (state == LS_CODES) {
+            LB_REFILL(b);
+            // a symbol needs up to 20 bits, then (a match) a refill and 28;
+            // a lane without them waits for its next swap
+            if (b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
+            uint32_t idx = lb_decode(b, lc, ST_BAD_CODE, err);
+            if (err == ST_OK) {
+                const uint32_t sym = lsym[(idx < 288 ? idx : 287) * LANES];
+                if (sym < 256) {
+                    if (pos >= cap)
+                        err = ST_OUTPUT_OVERFLOW;
+                    else
+                        out[pos++] = (uint8_t)sym;
+                } else if (sym == 256) {
+                    state = last ? LS_DONE : LS_HEADER;
+                } else if (sym - 257 >= 29) {
+                    err = ST_BAD_CODE;
+                } else {
+                    const uint32_t lcode = sym - 257;
+                    const uint32_t len = len_base(lcode) + lb_bits(b, len_extra(lcode), err);
+                    if (err == ST_OK) {
+                        LB_REFILL(b);
+                        idx = lb_decode(b, dc, ST_BAD_DISTANCE, err);
+                    }
+                    uint32_t dist = 0;
+                    if (err == ST_OK) {
+                        const uint32_t ds = dsym[(idx < 32 ? idx : 31) * LANES];
+                        if (ds >= 30)
+                            err = ST_BAD_DISTANCE;
+                        else
+                            dist = dist_base(ds) + lb_bits(b, dist_extra(ds), err);
+                    }
+                    if (err == ST_OK) {
+                        if (dist > pos) {
+                            err = ST_BAD_DISTANCE;
+                        } else if (pos + len > cap) {
+                            err = ST_OUTPUT_OVERFLOW;
+                        } else {
+                            uint32_t gap = pos - mark;
+                            while (gap >= REC_SKIP) {
+                                rec[nrec++] = REC_SKIP << 23;
+                                gap -= REC_SKIP;
+                            }
+                            rec[nrec++] = (gap << 23) | ((len - 3) << 15) | (dist - 1);
+                            pos += len;
+                            mark = pos;
+                        }
+                    }
+                }
+            }
+            if (err != ST_OK) state = LS_DONE;
+            continue;
+        }
+        // block header
+        LB_REFILL_SYNC(b);
+        last = lb_bits(b, 1, err);
+        const uint32_t type = lb_bits(b, 2, err);
+        if (err != ST_OK) {
+            state = LS_DONE;
+            continue;
+        }
+        if (type == 0) {
+            // stored: to a byte boundary, then LEN, NLEN and the bytes
+            const uint32_t skip = (in_len * 8 - b.avail) & 7;
+            (void)lb_bits(b, skip ? 8 - skip : 0, err);
+            if (b.avail < 32) {
+                err = ST_TRUNCATED;
+            } else {
+                LB_REFILL_SYNC(b);
+                const uint32_t len = lb_bits(b, 16, err);
+                LB_REFILL_SYNC(b);
+                const uint32_t nlen = lb_bits(b, 16, err);
+                if (len != (~nlen & 0xFFFFu))
+                    err = ST_STORED_LEN;
+                else if (len * 8 > b.avail)
+                    err = ST_TRUNCATED;
+                else if (pos + len > cap)
+                    err = ST_OUTPUT_OVERFLOW;
+                else
+                    for (uint32_t i = 0; i < len; ++i) {
+                        LB_REFILL_SYNC(b);
+                        out[pos++] = (uint8_t)lb_bits(b, 8, err);
+                    }
+            }
+            state = err != ST_OK || last ? LS_DONE : LS_HEADER;
+            continue;
+        }
+        if (type == 3) {
+            err = ST_BLOCK_TYPE;
+            state = LS_DONE;
+            continue;
+        }
+        if (type == 1) {
+            // fixed code: sorted lists in closed form
+            for (uint32_t e = 0; e < 288; ++e)
+                lsym[e * LANES] = (uint16_t)(e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176);
+            for (uint32_t e = 0; e < 32; ++e) dsym[e * LANES] = (uint8_t)e;
+            (void)lane_code(lc, 288, [](int k) -> uint32_t { return k == 7 ? 24 : k == 8 ? 152 : k == 9 ? 112 : 0; });
+            (void)lane_code(dc, 32, [](int k) -> uint32_t { return k == 5 ? 32 : 0; });
+            state = LS_CODES;
+            continue;
+        }
+        // dynamic header
+        const uint32_t nlen = lb_bits(b, 5, err) + 257;
+        const uint32_t ndist = lb_bits(b, 5, err) + 1;
+        const uint32_t ncode = lb_bits(b, 4, err) + 4;
+        if (err == ST_OK && (nlen > 286 || ndist > 30)) err = ST_BAD_COUNTS;
+        uint64_t cl = 0; // code-length code lengths, 3 bits per symbol 0..18
+#pragma unroll
+        for (uint32_t i = 0; i < 19; ++i) {
+            if (err == ST_OK && i < ncode) {
+                LB_REFILL_SYNC(b);
+                cl |= (uint64_t)lb_bits(b, 3, err) << (3 * clen_order(i));
+            }
+        }
+        LaneCode cc;
+        uint64_t cs0 = 0, cs1 = 0; // code-length code symbols sorted, 5 bits each
+        if (err == ST_OK) {
+            F8 c8 = {0, 0};
+#pragma unroll
+            for (uint32_t s = 0; s < 19; ++s) f8_add(c8, (uint32_t)(cl >> (3 * s)) & 7, 1);
+            const int e = lane_code(cc, 19, [&](int k) -> uint32_t { return k < 8 ? f8_get(c8, (uint32_t)k) : 0; });
+            if (e != 0) err = ST_BAD_TABLE;
+            F8 o8 = {0, 0};
+            uint32_t run = 0;
+#pragma unroll
+            for (uint32_t k = 1; k < 8; ++k) {
+                f8_add(o8, k, run);
+                run += f8_get(c8, k);
+            }
+#pragma unroll
+            for (uint32_t s = 0; s < 19; ++s) {
+                const uint32_t L = (uint32_t)(cl >> (3 * s)) & 7;
+                if (L) {
+                    const uint32_t p = f8_get(o8, L);
+                    f8_add(o8, L, 1);
+                    cs0 |= p < 12 ? (uint64_t)s << (5 * p) : 0;
+                    cs1 |= p < 12 ? 0 : (uint64_t)s << (5 * (p - 12));
+                }
+            }
+        }
+        if (err != ST_OK) {
+            state = LS_DONE;
+            continue;
+        }
+        // Two passes over the code lengths: the first counts them per length
+        // and checks the sequence, the second (from the same input position)
+        // places the symbols in the sorted lists.
+        const uint32_t total = nlen + ndist;
+        LaneBits saved;
+        lb_copy(saved, b);
+        F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
+        F8 cd = {0, 0}, od = {0, 0};
+        for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
+            if (pass == 1) {
+                lb_copy(b, saved);
+                uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    f16_add(oll, k, r0);
+                    r0 += f16_get(cll, k);
+                    f8_add(od, k, r1);
+                    r1 += f8_get(cd, k);
+                }
+            }
+            uint32_t index = 0, prev = 0, len256 = 0;
+            while (index < total) {
+                LB_REFILL_SYNC(b);
+                const uint32_t ci = lb_decode(b, cc, ST_BAD_CODE, err);
+                if (err != ST_OK) break;
+                const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
+                uint32_t len = 0, rep = 1;
+                if (sym < 16) {
+                    len = sym;
+                } else if (sym == 16) {
+                    if (index == 0) {
+                        err = ST_BAD_LENGTH_REPEAT;
+                        break;
+                    }
+                    len = prev;
+                    rep = 3 + lb_bits(b, 2, err);
+                } else if (sym == 17) {
+                    rep = 3 + lb_bits(b, 3, err);
+                } else {
+                    rep = 11 + lb_bits(b, 7, err);
+                }
+                if (err != ST_OK) break;
+                if (index + rep > total) {
+                    err = ST_BAD_LENGTH_REPEAT;
+                    break;
+                }
+                const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
+                if (len) {
+                    if (pass == 0) {
+                        f16_add(cll, len, nll);
+                        f8_add(cd, len, rep - nll);
+                    } else {
+                        for (uint32_t j = 0; j < rep; ++j) {
+                            const uint32_t s = index + j;
+                            if (s < nlen) {
+                                const uint32_t p = f16_get(oll, len);
+                                f16_add(oll, len, 1);
+                                lsym[(p < 288 ? p : 287) * LANES] = (uint16_t)s;
+                            } else {
+                                const uint32_t p = f8_get(od, len);
+                                f8_add(od, len, 1);
+                                dsym[(p < 32 ? p : 31) * LANES] = (uint8_t)(s - nlen);
+                            }
+                        }
+                    }
+                }
+                if (index <= 256 && 256 < index + rep) len256 = len;
+                prev = len;
+                index += rep;
+            }
+            if (pass == 0 && err == ST_OK) {
+                if (len256 == 0) {
+                    err = ST_BAD_TABLE;
+                    break;
+                }
+                uint32_t sll = 0, sd = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    sll += f16_get(cll, k);
+                    sd += f8_get(cd, k);
+                }
+                int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
+                if (e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
+                e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
+                if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
+            }
+        }
+        state = err == ST_OK ? LS_CODES : LS_DONE;
+    }
+    a.status[m] = err;
+    a.produced[m] = pos;
+    a.crc[m] = 0;
+    a.nrec[m] = nrec;
+}
+
+// Phase 2: records of one member, 64 at a time, into the ring.
+template <uint32_t RING, uint32_t NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void fr_inflate_resolve_kernel(InflateArgs a)
+{
+    using G = Geometry<RING>;
+    // a round covers at most this many output bytes (one record spans <= 768)
+    constexpr uint32_t SPAN = RING / 4;
+    static_assert(SPAN >= REC_SKIP + 258, "ring too small for a record");
+    __shared__ uint32_t crctab[256];
+    __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
+        uint32_t c = i;
+        for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (FR_CRC_POLY & (0u - (c & 1u)));
+        crctab[i] = c;
+    }
+    __syncthreads();
+    uint8_t *ring = rings[wave];
+    const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)G::SEG * (63 - lane));
+    uint32_t iter = 0;
+    for (;;) {
+        uint32_t m;
+        if (a.queue) {
+            uint32_t q = 0;
+            if (lane == 0) q = atomicAdd(a.queue, 1u);
+            m = uni(q);
+        } else {
+            m = (blockIdx.x + iter * gridDim.x) * NW + wave;
+        }
+        ++iter;
+        if (m >= a.n) break;
+        if (a.order) m = uni(a.order[m]);
+        if (uni(a.status[m]) != ST_OK) continue;
+        const uint32_t P = uni(a.produced[m]);
+        const uint32_t nr = uni(a.nrec[m]);
+        const uint64_t out_off = uni64(a.out_off[m]);
+        const uint32_t *rec = a.rec + rec_base(out_off, m);
+        Out o;
+        o.g = a.out + out_off;
+        o.pos = 0;
+        o.flushed = 0;
+        o.visible = 0;
+        o.cap = P;
+        o.raw = 0;
+        o.skew = (uint32_t)((uintptr_t)o.g & 16383);
+        uint32_t ri = 0;
+        while (ri < nr || o.pos < P) {
+            ri = uni(ri);
+            o.pos = uni(o.pos);
+            o.flushed = uni(o.flushed);
+            if (o.pos - o.flushed >= G::TOK_FLUSH_AT) flush<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+            const uint32_t R0 = o.pos;
+            const bool live = ri + lane < nr;
+            const uint32_t r = live ? rec[ri + lane] : 0;
+            const uint32_t gap = r >> 23;
+            const uint32_t len = !live || gap == REC_SKIP ? 0 : ((r >> 15) & 255) + 3;
+            const uint32_t dist = (r & 0x7FFFu) + 1;
+            uint32_t inc = live ? gap + len : 0;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d, 64);
+                if ((int)lane >= d) inc += y;
+            }
+            const uint32_t k = (uint32_t)__popcll(__ballot(live && inc <= SPAN));
+            uint32_t R1;
+            if (k == 0) {
+                R1 = P - R0 < SPAN ? P : R0 + SPAN; // literals after the last record
+            } else {
+                R1 = R0 + uni((uint32_t)__shfl(inc, (int)k - 1, 64));
+            }
+            if ((k == 0 && ri < nr) || R1 > P) {
+                // records that do not fit the member (phase 1 never writes such)
+                if (lane == 0) a.status[m] = ST_OUTPUT_OVERFLOW;
+                break;
+            }
+            // the span's literals (match bytes are overwritten below)
+            for (uint32_t i = lane; i < R1 - R0; i += 64) ring[(o.skew + R0 + i) & G::RMASK] = o.g[R0 + i];
+            WFENCE();
+            const bool match = lane < k && len > 0;
+            const uint32_t dst = R0 + inc - len;
+            const uint32_t src = dst - dist;
+            const uint32_t lim = R1 > RING ? R1 - RING : 0;
+            const bool deferred = match && (src + len > R0 || len > TOK_SHORT);
+            if (__ballot(match && src < lim) != 0 && o.visible < o.flushed) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                o.visible = o.flushed;
+            }
+            if (match && !deferred) {
+                for (uint32_t i = 0; i < len; ++i) {
+                    const uint32_t p = src + i;
+                    const uint8_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
+                    ring[(o.skew + dst + i) & G::RMASK] = v;
+                }
+            }
+            WFENCE();
+            uint64_t pending = __ballot(deferred);
+            while (pending) {
+                const uint32_t j = (uint32_t)__ffsll((unsigned long long)pending) - 1;
+                pending &= pending - 1;
+                const uint32_t d0 = uni((uint32_t)__shfl(dst, (int)j, 64));
+                const uint32_t L = uni((uint32_t)__shfl(len, (int)j, 64));
+                const uint32_t D = uni((uint32_t)__shfl(dist, (int)j, 64));
+                if (D >= L) {
+                    for (uint32_t off = lane; off < L; off += 64) {
+                        const uint32_t p = d0 - D + off;
+                        const uint8_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
+                        ring[(o.skew + d0 + off) & G::RMASK] = v;
+                    }
+                } else {
+                    uint32_t rr = lane % D;
+                    const uint32_t step = 64u % D;
+                    for (uint32_t off = lane; off < L; off += 64) {
+                        ring[(o.skew + d0 + off) & G::RMASK] = ring[(o.skew + d0 - D + rr) & G::RMASK];
+                        rr += step;
+                        if (rr >= D) rr -= D;
+                    }
+                }
+                WFENCE();
+            }
+            o.pos = R1;
+            ri += k;
+        }
+        flush<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+        if (lane == 0) a.crc[m] = o.raw ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(a.x2n, o.pos)) ^ 0xFFFFFFFFu;
+    }
+}
+
 // CRC32 of independent byte ranges, one wave per range: lane l takes the l-th
 // of 64 equal segments; segments fold with x^(8 * bytes after the segment).
 __global__ __launch_bounds__(64) void fr_crc32_kernel(const uint8_t *data, const uint64_t *off,
@@ -1181,16 +1820,44 @@ static hipError_t launch_variant(const InflateArgs &a, uint32_t waves, hipStream
     return hipGetLastError();
 }
 
+template <uint32_t RING, uint32_t NW>
+static hipError_t launch_two_phase(const InflateArgs &a, uint32_t waves, hipStream_t stream)
+{
+    if (!a.rec || !a.nrec) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fr_inflate_lanes_kernel<1>, dim3((a.n + 63) / 64), dim3(64), 0, stream, a);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    uint32_t grid = (waves + NW - 1) / NW;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL((fr_inflate_resolve_kernel<RING, NW>), dim3(grid), dim3(64 * NW), 0, stream, a);
+    return hipGetLastError();
+}
+
+uint64_t inflate_scratch_bytes(int schedule, uint64_t out_bytes, uint64_t n)
+{
+    if (schedule < 0 || schedule >= N_INFLATE_SCHEDULES || INFLATE_SCHEDULES[schedule].tokens != 3) return 0;
+    const uint64_t head = (4 * n + 255) & ~255ull;
+    return head + 4 * ((out_bytes * 11) / 32 + 2 * n + 4);
+}
+
+void inflate_scratch_bind(InflateArgs &a, void *scratch, uint64_t n)
+{
+    a.nrec = (uint32_t *)scratch;
+    a.rec = scratch ? (uint32_t *)((uint8_t *)scratch + ((4 * n + 255) & ~255ull)) : nullptr;
+}
+
 const InflateSchedule INFLATE_SCHEDULES[] = {
     {32768, 1, 0}, {16384, 1, 0}, {16384, 2, 0}, {8192, 1, 0}, {8192, 2, 0}, {8192, 4, 0}, {4096, 2, 0}, {4096, 4, 0},
     {8192, 2, 1}, {8192, 4, 1}, {4096, 2, 1}, {4096, 4, 1},
     {8192, 2, 2}, {8192, 4, 2}, {4096, 2, 2}, {4096, 4, 2},
+    {4096, 4, 3}, {4096, 2, 3}, {8192, 4, 3}, {8192, 2, 3},
 };
 const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHEDULES[0]);
 
 uint32_t inflate_lds_bytes(int schedule)
 {
     const InflateSchedule &s = INFLATE_SCHEDULES[schedule];
+    if (s.tokens == 3) return 1024 + s.waves_per_group * s.ring; // the copy phase's rings
     uint32_t per_wave = 0;
     switch (s.ring) {
     case 32768: per_wave = sizeof(WaveMem<32768>); break;
@@ -1230,6 +1897,10 @@ hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stre
     case 13: return launch_variant<8192, 4, 2>(a, waves, stream);
     case 14: return launch_variant<4096, 2, 2>(a, waves, stream);
     case 15: return launch_variant<4096, 4, 2>(a, waves, stream);
+    case 16: return launch_two_phase<4096, 4>(a, waves, stream);
+    case 17: return launch_two_phase<4096, 2>(a, waves, stream);
+    case 18: return launch_two_phase<8192, 4>(a, waves, stream);
+    case 19: return launch_two_phase<8192, 2>(a, waves, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -59,6 +59,8 @@ struct frhip_pipeline {
     uint32_t *d_st = nullptr;    // status, produced, crc: 3n
     uint64_t *d_queue = nullptr; // low 32 bits are the work-queue counter
     uint32_t *d_order = nullptr;
+    int sched = 0;               // inflate schedule, fixed at creation
+    void *d_rec = nullptr;       // two-phase schedules: match records
     // bam
     uint8_t *d_bam = nullptr;
     BamScratchLayout L{};
@@ -135,9 +137,8 @@ hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
         a.queue = (uint32_t *)p->d_queue;
         a.order = p->cfg.order ? p->d_order : nullptr;
         a.x2n = p->ctx->x2n;
-        int sched = p->cfg.inflate_schedule >= 0 && p->cfg.inflate_schedule < N_INFLATE_SCHEDULES
-                        ? p->cfg.inflate_schedule
-                        : p->ctx->inflate_schedule;
+        const int sched = p->sched;
+        inflate_scratch_bind(a, p->d_rec, p->n);
         uint32_t wpc = p->cfg.waves_per_cu ? p->cfg.waves_per_cu : inflate_waves_per_cu(sched);
         uint64_t waves = (uint64_t)p->ctx->n_cu * wpc;
         if (waves > p->n) waves = p->n;
@@ -393,6 +394,10 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
     if ((rc = dalloc(p, (void **)&p->d_u, p->U, err, "inflated output"))) return fail(rc);
     if ((rc = dalloc(p, (void **)&p->d_st, nn * 12, err, "member status"))) return fail(rc);
     if ((rc = dalloc(p, (void **)&p->d_queue, 8, err, "work queue"))) return fail(rc);
+    p->sched = p->cfg.inflate_schedule >= 0 && p->cfg.inflate_schedule < N_INFLATE_SCHEDULES ? p->cfg.inflate_schedule
+                                                                                            : ctx->inflate_schedule;
+    if (const uint64_t sb = inflate_scratch_bytes(p->sched, p->U, nn))
+        if ((rc = dalloc(p, &p->d_rec, sb, err, "match records"))) return fail(rc);
     if (p->cfg.order) {
         // largest compressed payload first: the long members start early
         std::vector<uint32_t> in_len(p->n);

@@ -50,7 +50,7 @@ struct frhip_server {
     uint64_t n = 0;
     std::vector<uint64_t> coff, in_off, uprefix;
     std::vector<uint32_t> in_len, isize, crc;
-    Scratch meta, work, arena, cols[N_COLS];
+    Scratch meta, work, arena, recs, cols[N_COLS];
     PinnedScratch pin;
     frhip_buf col_view[N_COLS] = {};
     frhip_buf arena_view = {};
@@ -201,6 +201,7 @@ extern "C" void frhip_server_free(frhip_server *s)
     s->meta.release();
     s->work.release();
     s->arena.release();
+    s->recs.release();
     for (auto &c : s->cols) c.release();
     s->pin.release();
     if (s->ctx) ctx_release(s->ctx);
@@ -229,7 +230,8 @@ extern "C" uint64_t frhip_server_batch_footprint(const frhip_server *s, const fr
     }
     uint64_t per_rec = 0;
     for (int c = 0; c < N_COLS; ++c) per_rec += WIDTHS[c];
-    return bytes + records_cap(bytes) * per_rec + meta_layout(members, n).bytes + work_layout(members, n).bytes;
+    return bytes + records_cap(bytes) * per_rec + meta_layout(members, n).bytes + work_layout(members, n).bytes +
+           inflate_scratch_bytes(s->ctx->inflate_schedule, bytes, members);
 }
 
 extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t nq, frhip_query_result *res,
@@ -327,6 +329,12 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
     a.queue = (uint32_t *)(dw + W.o_queue);
     a.order = nullptr;
     a.x2n = ctx->x2n;
+    void *d_rec = nullptr;
+    if (const uint64_t sb = inflate_scratch_bytes(ctx->inflate_schedule, total, nb)) {
+        d_rec = s->recs.get(sb, err, &rc);
+        if (!d_rec) return rc;
+    }
+    inflate_scratch_bind(a, d_rec, nb);
     if (r == hipSuccess) {
         const int sched = ctx->inflate_schedule;
         uint64_t waves = (uint64_t)ctx->n_cu * inflate_waves_per_cu(sched);

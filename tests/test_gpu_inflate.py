@@ -194,7 +194,7 @@ def _schedule_ids():
     try:
         return list(range(len(fa.hip.inflate_schedules())))
     except Exception:  # library not built: collection still succeeds
-        return list(range(16))
+        return list(range(20))
 
 
 @pytest.mark.parametrize("schedule", _schedule_ids())
