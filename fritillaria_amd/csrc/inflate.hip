@@ -1192,6 +1192,9 @@ __device__ __forceinline__ int lane_code(LaneCode &c, uint32_t n, Cnt cnt)
 
 // Sorted-list index of the code at the front of v (15 bits, MSB first); L = 0
 // when v starts with no code of this table.
+// MAXL: the longest code the table can hold (15, or 7 for the code-length
+// code, whose limits for lengths 8..15 all equal lim[7]).
+template <int MAXL = 15>
 __device__ __forceinline__ uint32_t lane_find(const LaneCode &c, uint32_t v, uint32_t &L)
 {
     // q[] increases with k, so the entry that applies is the largest q[k] <=
@@ -1199,13 +1202,14 @@ __device__ __forceinline__ uint32_t lane_find(const LaneCode &c, uint32_t v, uin
     const uint32_t v16 = (v << 16) | 0xFFFFu;
     uint32_t se = 1u << 12, so = 1u << 12;
 #pragma unroll
-    for (int k = 0; k < (int)LQ; k += 2) {
+    for (int k = 0; k < MAXL - 1; k += 2) {
         se = v16 >= c.q[k] ? c.q[k] : se;
-        so = v16 >= c.q[k + 1] ? c.q[k + 1] : so;
+        if (k + 1 < MAXL - 1) so = v16 >= c.q[k + 1] ? c.q[k + 1] : so;
     }
     const uint32_t sel = se > so ? se : so;
     const uint32_t len = (sel >> 12) & 15;
-    L = v < c.lim15 ? len : 0;
+    const uint32_t lim = MAXL == 15 ? c.lim15 : c.q[MAXL - 1] >> 16;
+    L = v < lim ? len : 0;
     return (sel & 0xFFFu) + ((v - (sel >> 16)) >> (15 - len));
 }
 
@@ -1287,11 +1291,12 @@ __device__ __forceinline__ uint32_t lb_bits(LaneBits &b, uint32_t n, uint32_t &e
 
 // Next code of table c: its sorted-list index, or err (puff's order: input
 // that ends inside a code is truncated, else no code after 15 bits is `bad`).
+template <int MAXL = 15>
 __device__ __forceinline__ uint32_t lb_decode(LaneBits &b, const LaneCode &c, uint32_t bad, uint32_t &err)
 {
     const uint32_t v = __builtin_bitreverse32((uint32_t)b.bb) >> 17;
     uint32_t L;
-    const uint32_t idx = lane_find(c, v, L);
+    const uint32_t idx = lane_find<MAXL>(c, v, L);
     if (L == 0 || L > b.avail) {
         err = L == 0 && b.avail >= 15 ? bad : ST_TRUNCATED;
         return 0;
@@ -1359,14 +1364,15 @@ template <uint32_t NW>
 __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a)
 {
     constexpr uint32_t LANES = 64 * NW;
-    __shared__ uint16_t lsym_all[288 * LANES]; // entry e of lane t at [e * LANES + t]
-    __shared__ uint8_t dsym_all[32 * LANES];
+    // per lane: the literal/length symbol list (288 entries) then the distance
+    // list (32); entry e of lane t at [e * LANES + t]
+    __shared__ uint16_t tab_all[(288 + 32) * LANES];
     const uint32_t t = threadIdx.x;
     const uint32_t slot = blockIdx.x * LANES + t;
     if (slot >= a.n) return;
     const uint32_t m = a.order ? a.order[slot] : slot;
-    uint16_t *lsym = lsym_all + t;
-    uint8_t *dsym = dsym_all + t;
+    uint16_t *lsym = tab_all + t;
+    uint16_t *dsym = tab_all + 288 * LANES + t;
     const uint64_t in_off = a.in_off[m];
     const uint32_t in_len = a.in_len[m];
     const uint64_t out_off = a.out_off[m];
@@ -1409,7 +1415,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             if (b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
             uint32_t idx = lb_decode(b, lc, ST_BAD_CODE, err);
             if (err == ST_OK) {
-                const uint32_t sym = lsym[(idx < 288 ? idx : 287) * LANES];
+                const uint32_t sym = lsym[(idx < 288 ? idx : 287) * LANES] & 0xFFFu;
                 if (sym < 256) {
                     if (pos >= cap)
                         err = ST_OUTPUT_OVERFLOW;
@@ -1498,7 +1504,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             // fixed code: sorted lists in closed form
             for (uint32_t e = 0; e < 288; ++e)
                 lsym[e * LANES] = (uint16_t)(e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176);
-            for (uint32_t e = 0; e < 32; ++e) dsym[e * LANES] = (uint8_t)e;
+            for (uint32_t e = 0; e < 32; ++e) dsym[e * LANES] = (uint16_t)e;
             (void)lane_code(lc, 288, [](int k) -> uint32_t { return k == 7 ? 24 : k == 8 ? 152 : k == 9 ? 112 : 0; });
             (void)lane_code(dc, 32, [](int k) -> uint32_t { return k == 5 ? 32 : 0; });
             state = LS_CODES;
@@ -1547,91 +1553,102 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             state = LS_DONE;
             continue;
         }
-        // Two passes over the code lengths: the first counts them per length
-        // and checks the sequence, the second (from the same input position)
-        // places the symbols in the sorted lists.
+        // One pass over the code lengths counts them per length and keeps
+        // them: literal/length lengths in the high nibble of the lane's
+        // symbol-list slots (slot s = symbol s), distance lengths in register
+        // nibbles. The sorted lists are then placed in the slots' low bits.
         const uint32_t total = nlen + ndist;
-        LaneBits saved;
-        lb_copy(saved, b);
-        F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
-        F8 cd = {0, 0}, od = {0, 0};
-        for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
-            if (pass == 1) {
-                lb_copy(b, saved);
-                uint32_t r0 = 0, r1 = 0;
-#pragma unroll
-                for (uint32_t k = 1; k < 16; ++k) {
-                    f16_add(oll, k, r0);
-                    r0 += f16_get(cll, k);
-                    f8_add(od, k, r1);
-                    r1 += f8_get(cd, k);
-                }
-            }
-            uint32_t index = 0, prev = 0, len256 = 0;
-            while (index < total) {
-                LB_REFILL_SYNC(b);
-                const uint32_t ci = lb_decode(b, cc, ST_BAD_CODE, err);
-                if (err != ST_OK) break;
-                const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
-                uint32_t len = 0, rep = 1;
-                if (sym < 16) {
-                    len = sym;
-                } else if (sym == 16) {
-                    if (index == 0) {
-                        err = ST_BAD_LENGTH_REPEAT;
-                        break;
-                    }
-                    len = prev;
-                    rep = 3 + lb_bits(b, 2, err);
-                } else if (sym == 17) {
-                    rep = 3 + lb_bits(b, 3, err);
-                } else {
-                    rep = 11 + lb_bits(b, 7, err);
-                }
-                if (err != ST_OK) break;
-                if (index + rep > total) {
+        F16 cll = {0, 0, 0, 0};
+        F8 cd = {0, 0};
+        uint32_t dl0 = 0, dl1 = 0, dl2 = 0, dl3 = 0; // distance code lengths, 4 bits each
+        uint32_t index = 0, prev = 0, len256 = 0;
+        while (index < total) {
+            LB_REFILL_SYNC(b);
+            const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
+            if (err != ST_OK) break;
+            const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
+            uint32_t len = 0, rep = 1;
+            if (sym < 16) {
+                len = sym;
+            } else if (sym == 16) {
+                if (index == 0) {
                     err = ST_BAD_LENGTH_REPEAT;
                     break;
                 }
-                const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
-                if (len) {
-                    if (pass == 0) {
-                        f16_add(cll, len, nll);
-                        f8_add(cd, len, rep - nll);
-                    } else {
-                        for (uint32_t j = 0; j < rep; ++j) {
-                            const uint32_t s = index + j;
-                            if (s < nlen) {
-                                const uint32_t p = f16_get(oll, len);
-                                f16_add(oll, len, 1);
-                                lsym[(p < 288 ? p : 287) * LANES] = (uint16_t)s;
-                            } else {
-                                const uint32_t p = f8_get(od, len);
-                                f8_add(od, len, 1);
-                                dsym[(p < 32 ? p : 31) * LANES] = (uint8_t)(s - nlen);
-                            }
-                        }
-                    }
-                }
-                if (index <= 256 && 256 < index + rep) len256 = len;
-                prev = len;
-                index += rep;
+                len = prev;
+                rep = 3 + lb_bits(b, 2, err);
+            } else if (sym == 17) {
+                rep = 3 + lb_bits(b, 3, err);
+            } else {
+                rep = 11 + lb_bits(b, 7, err);
             }
-            if (pass == 0 && err == ST_OK) {
-                if (len256 == 0) {
-                    err = ST_BAD_TABLE;
-                    break;
+            if (err != ST_OK) break;
+            if (index + rep > total) {
+                err = ST_BAD_LENGTH_REPEAT;
+                break;
+            }
+            const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
+            if (len) {
+                f16_add(cll, len, nll);
+                f8_add(cd, len, rep - nll);
+            }
+            for (uint32_t j = 0; j < nll; ++j) lsym[(index + j) * LANES] = (uint16_t)(len << 12);
+            if (len) {
+                for (uint32_t j = nll; j < rep; ++j) {
+                    const uint32_t d = index + j - nlen, sh = 4 * (d & 7), g = d >> 3;
+                    dl0 |= g == 0 ? len << sh : 0;
+                    dl1 |= g == 1 ? len << sh : 0;
+                    dl2 |= g == 2 ? len << sh : 0;
+                    dl3 |= g == 3 ? len << sh : 0;
                 }
-                uint32_t sll = 0, sd = 0;
+            }
+            if (index <= 256 && 256 < index + rep) len256 = len;
+            prev = len;
+            index += rep;
+        }
+        if (err == ST_OK) {
+            uint32_t sll = 0, sd = 0;
 #pragma unroll
-                for (uint32_t k = 1; k < 16; ++k) {
-                    sll += f16_get(cll, k);
-                    sd += f8_get(cd, k);
+            for (uint32_t k = 1; k < 16; ++k) {
+                sll += f16_get(cll, k);
+                sd += f8_get(cd, k);
+            }
+            if (len256 == 0) err = ST_BAD_TABLE;
+            int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
+            if (e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
+            e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
+            if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
+        }
+        if (err == ST_OK) {
+            F16 oll = {0, 0, 0, 0};
+            F8 od = {0, 0};
+            uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+            for (uint32_t k = 1; k < 16; ++k) {
+                f16_add(oll, k, r0);
+                r0 += f16_get(cll, k);
+                f8_add(od, k, r1);
+                r1 += f8_get(cd, k);
+            }
+            // slot p's own length stays in its high nibble until slot p is read
+            for (uint32_t sy = 0; sy < nlen; ++sy) {
+                const uint32_t L = lsym[sy * LANES] >> 12;
+                if (L) {
+                    const uint32_t q = f16_get(oll, L);
+                    f16_add(oll, L, 1);
+                    uint16_t *slot = lsym + (q < 288 ? q : 287) * LANES;
+                    *slot = (uint16_t)((*slot & 0xF000u) | sy);
                 }
-                int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
-                if (e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
-                e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
-                if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
+            }
+            for (uint32_t d = 0; d < ndist; ++d) {
+                const uint32_t g = d >> 3;
+                const uint32_t w = g == 0 ? dl0 : g == 1 ? dl1 : g == 2 ? dl2 : dl3;
+                const uint32_t L = (w >> (4 * (d & 7))) & 15;
+                if (L) {
+                    const uint32_t q = f8_get(od, L);
+                    f8_add(od, L, 1);
+                    dsym[(q < 32 ? q : 31) * LANES] = (uint16_t)d;
+                }
             }
         }
         state = err == ST_OK ? LS_CODES : LS_DONE;
@@ -1642,14 +1659,59 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     a.nrec[m] = nrec;
 }
 
-// Phase 2: records of one member, 64 at a time, into the ring.
+// Four bytes of a partial block, skipping those at or past `end`.
+__device__ __forceinline__ uint32_t part4(const uint8_t *p, const uint8_t *end)
+{
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v |= p + j < end ? (uint32_t)p[j] << (8 * j) : 0u;
+    return v;
+}
+
+// Output window of a member for one round: lane l holds the 16-byte aligned
+// block at ((g + from) & ~15) + 16 l, so the wave covers at least positions
+// [from, from + WIN - 15). Blocks past position P are not read; the block
+// that straddles P is read byte by byte.
+constexpr uint32_t WIN = 1024;
+__device__ __forceinline__ uint4 load_window(const uint8_t *g, uint32_t from, uint32_t P, uint32_t lane)
+{
+    const uint8_t *blk = g + from - ((uintptr_t)(g + from) & 15) + 16 * lane;
+    const uint8_t *end = g + P;
+    uint4 v = {0, 0, 0, 0};
+    if (blk + 16 <= end) {
+        v = *reinterpret_cast<const uint4 *>(blk);
+    } else if (blk < end) {
+        v.x = part4(blk, end);
+        v.y = part4(blk + 4, end);
+        v.z = part4(blk + 8, end);
+        v.w = part4(blk + 12, end);
+    }
+    return v;
+}
+
+// flush() on a copy of the output state, returned by value: callers that
+// would otherwise pass their state by reference to an out-of-line call keep it
+// in registers.
+template <uint32_t RING>
+__device__ __attribute__((noinline)) Out flush_copy(const uint8_t *ring, const uint32_t *crctab, Out o, uint32_t lane,
+                                                    uint32_t lane_factor, const uint32_t *x2n)
+{
+    flush<RING>(ring, crctab, o, lane, lane_factor, x2n);
+    return o;
+}
+
+// Phase 2: records of one member, 64 at a time, into the ring. Each round's
+// records and output window are fetched one round ahead, into one of two
+// register sets used in turn (so a fetched value is never copied, which would
+// wait for it).
 template <uint32_t RING, uint32_t NW>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void fr_inflate_resolve_kernel(InflateArgs a)
 {
     using G = Geometry<RING>;
     // a round covers at most this many output bytes (one record spans <= 768)
-    constexpr uint32_t SPAN = RING / 4;
-    static_assert(SPAN >= REC_SKIP + 258, "ring too small for a record");
+    constexpr uint32_t SPAN = WIN - 16;
+    static_assert(SPAN >= REC_SKIP + 258, "window too small for a record");
+    static_assert(SPAN + WIN <= RING / 2, "window writes must stay clear of unflushed bytes");
     __shared__ uint32_t crctab[256];
     __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1688,17 +1750,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         o.raw = 0;
         o.skew = (uint32_t)((uintptr_t)o.g & 16383);
         uint32_t ri = 0;
-        while (ri < nr || o.pos < P) {
+        bool bad = false;
+        // One round: records `rc` (from ri) and window `wc` (from o.pos) are
+        // in hand; those of the next round go to rn, wn.
+        auto round = [&](const uint32_t rc, const uint4 wc, uint32_t &rn, uint4 &wn) __attribute__((always_inline)) {
             ri = uni(ri);
             o.pos = uni(o.pos);
             o.flushed = uni(o.flushed);
-            if (o.pos - o.flushed >= G::TOK_FLUSH_AT) flush<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+            if (o.pos - o.flushed >= G::TOK_FLUSH_AT) o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
             const uint32_t R0 = o.pos;
             const bool live = ri + lane < nr;
-            const uint32_t r = live ? rec[ri + lane] : 0;
-            const uint32_t gap = r >> 23;
-            const uint32_t len = !live || gap == REC_SKIP ? 0 : ((r >> 15) & 255) + 3;
-            const uint32_t dist = (r & 0x7FFFu) + 1;
+            const uint32_t gap = rc >> 23;
+            const uint32_t len = !live || gap == REC_SKIP ? 0 : ((rc >> 15) & 255) + 3;
+            const uint32_t dist = (rc & 0x7FFFu) + 1;
             uint32_t inc = live ? gap + len : 0;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
@@ -1714,16 +1778,34 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             if ((k == 0 && ri < nr) || R1 > P) {
                 // records that do not fit the member (phase 1 never writes such)
-                if (lane == 0) a.status[m] = ST_OUTPUT_OVERFLOW;
-                break;
+                bad = true;
+                return;
             }
-            // the span's literals (match bytes are overwritten below)
-            for (uint32_t i = lane; i < R1 - R0; i += 64) ring[(o.skew + R0 + i) & G::RMASK] = o.g[R0 + i];
+            // the next round's records and window
+            rn = ri + k + lane < nr ? rec[ri + k + lane] : 0;
+            wn = load_window(o.g, R1, P, lane);
+            // this round's window into the ring: whole blocks, but the first
+            // block only from R0 on (the bytes before R0 are this member's
+            // finished output, possibly not yet written back)
+            {
+                const uint32_t S = R0 - (uint32_t)((uintptr_t)(o.g + R0) & 15) + 16 * lane;
+                if (lane == 0) {
+#pragma unroll
+                    for (uint32_t j = 0; j < 16; ++j) {
+                        const uint32_t w = j < 4 ? wc.x : j < 8 ? wc.y : j < 12 ? wc.z : wc.w;
+                        if (S + j >= R0) ring[(o.skew + S + j) & G::RMASK] = (uint8_t)(w >> (8 * (j & 3)));
+                    }
+                } else {
+                    *reinterpret_cast<uint4 *>(ring + ((o.skew + S) & G::RMASK)) = wc;
+                }
+            }
             WFENCE();
+            // ring slots are good from here on: the window writes of this and
+            // earlier rounds reach at most WIN bytes past R0
+            const uint32_t lim = R0 + WIN > RING ? R0 + WIN - RING : 0;
             const bool match = lane < k && len > 0;
             const uint32_t dst = R0 + inc - len;
             const uint32_t src = dst - dist;
-            const uint32_t lim = R1 > RING ? R1 - RING : 0;
             const bool deferred = match && (src + len > R0 || len > TOK_SHORT);
             if (__ballot(match && src < lim) != 0 && o.visible < o.flushed) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1764,8 +1846,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             }
             o.pos = R1;
             ri += k;
+        };
+        uint32_t rA = lane < nr ? rec[lane] : 0, rB = 0;
+        uint4 wA = load_window(o.g, 0, P, lane), wB = {0, 0, 0, 0};
+        for (;;) {
+            if (!(ri < nr || o.pos < P) || bad) break;
+            round(rA, wA, rB, wB);
+            if (!(ri < nr || o.pos < P) || bad) break;
+            round(rB, wB, rA, wA);
         }
-        flush<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+        if (bad) {
+            if (lane == 0) a.status[m] = ST_OUTPUT_OVERFLOW;
+            continue;
+        }
+        o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
         if (lane == 0) a.crc[m] = o.raw ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(a.x2n, o.pos)) ^ 0xFFFFFFFFu;
     }
 }
