@@ -1700,6 +1700,46 @@ __device__ __attribute__((noinline)) Out flush_copy(const uint8_t *ring, const u
     return o;
 }
 
+// Inclusive prefix sum over the wave with DPP row shifts and row broadcasts
+// (no LDS round trips).
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false); // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false); // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false); // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false); // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return x;
+}
+
+// Write-back of exactly FAST_CHUNK bytes from a 16-byte aligned output
+// position: lane l moves aligned block l to HBM and takes its CRC with the
+// slice-by-16 tables (t16[k][b] = CRC of byte b followed by k zero bytes, all
+// 16 lookups independent); the blocks fold with x^(8*16*(63-l)) as in flush().
+constexpr uint32_t FAST_CHUNK = 1024;
+template <uint32_t RING>
+__device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *t16, Out &o, uint32_t lane,
+                                           uint32_t lane_factor16, uint32_t x_chunk)
+{
+    using G = Geometry<RING>;
+    WFENCE();
+    const uint32_t p = o.flushed + 16 * lane;
+    const uint4 v = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p) & G::RMASK));
+    *reinterpret_cast<uint4 *>(o.g + p) = v;
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+        r ^= t16[(15 - j) * 256 + ((w >> (8 * (j & 3))) & 0xFF)];
+    }
+    r = fr_mulmod(r, lane_factor16);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) r ^= __shfl_xor(r, off, 64);
+    o.raw = fr_mulmod(o.raw, x_chunk) ^ uni(r);
+    o.flushed += FAST_CHUNK;
+}
+
 // Phase 2: records of one member, 64 at a time, into the ring. Each round's
 // records and output window are fetched one round ahead, into one of two
 // register sets used in turn (so a fetched value is never copied, which would
@@ -1712,17 +1752,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     constexpr uint32_t SPAN = WIN - 16;
     static_assert(SPAN >= REC_SKIP + 258, "window too small for a record");
     static_assert(SPAN + WIN <= RING / 2, "window writes must stay clear of unflushed bytes");
-    __shared__ uint32_t crctab[256];
+    static_assert(FAST_CHUNK + SPAN + WIN <= RING, "fast write-back must stay clear of the window writes");
+    __shared__ uint32_t t16[16 * 256]; // t16[0] is the byte-at-a-time table
     __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
         uint32_t c = i;
         for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (FR_CRC_POLY & (0u - (c & 1u)));
-        crctab[i] = c;
+        t16[i] = c;
     }
     __syncthreads();
+    for (uint32_t k = 1; k < 16; ++k) {
+        for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
+            const uint32_t c = t16[(k - 1) * 256 + i];
+            t16[k * 256 + i] = (c >> 8) ^ t16[c & 0xFF];
+        }
+        __syncthreads();
+    }
+    const uint32_t *crctab = t16;
     uint8_t *ring = rings[wave];
     const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)G::SEG * (63 - lane));
+    const uint32_t lane_factor16 = fr_xpow8(a.x2n, (uint64_t)16 * (63 - lane));
+    const uint32_t x_chunk = fr_xpow8(a.x2n, FAST_CHUNK);
     uint32_t iter = 0;
     for (;;) {
         uint32_t m;
@@ -1757,24 +1808,30 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             ri = uni(ri);
             o.pos = uni(o.pos);
             o.flushed = uni(o.flushed);
-            if (o.pos - o.flushed >= G::TOK_FLUSH_AT) o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+            // write-back: once up to a 16-byte aligned output address, then
+            // in aligned FAST_CHUNK pieces
+            if (((uintptr_t)(o.g + o.flushed) & 15) != 0) {
+                if (o.pos - o.flushed >= FAST_CHUNK + 16) {
+                    const uint32_t keep = o.pos;
+                    o.pos -= (uint32_t)((uintptr_t)(o.g + o.pos) & 15);
+                    o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
+                    o.pos = keep;
+                }
+            } else if (o.pos - o.flushed >= FAST_CHUNK) {
+                flush_fast<RING>(ring, t16, o, lane, lane_factor16, x_chunk);
+            }
             const uint32_t R0 = o.pos;
             const bool live = ri + lane < nr;
             const uint32_t gap = rc >> 23;
             const uint32_t len = !live || gap == REC_SKIP ? 0 : ((rc >> 15) & 255) + 3;
             const uint32_t dist = (rc & 0x7FFFu) + 1;
-            uint32_t inc = live ? gap + len : 0;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(inc, d, 64);
-                if ((int)lane >= d) inc += y;
-            }
+            const uint32_t inc = wave_incl_sum(live ? gap + len : 0);
             const uint32_t k = (uint32_t)__popcll(__ballot(live && inc <= SPAN));
             uint32_t R1;
             if (k == 0) {
                 R1 = P - R0 < SPAN ? P : R0 + SPAN; // literals after the last record
             } else {
-                R1 = R0 + uni((uint32_t)__shfl(inc, (int)k - 1, 64));
+                R1 = R0 + (uint32_t)__builtin_amdgcn_readlane((int)inc, (int)k - 1);
             }
             if ((k == 0 && ri < nr) || R1 > P) {
                 // records that do not fit the member (phase 1 never writes such)
@@ -1813,10 +1870,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                 o.visible = o.flushed;
             }
             if (match && !deferred) {
-                for (uint32_t i = 0; i < len; ++i) {
-                    const uint32_t p = src + i;
-                    const uint8_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
-                    ring[(o.skew + dst + i) & G::RMASK] = v;
+                // all source bytes are read before any is written (sources lie
+                // before R0, so no write of this round can change them): one
+                // wait for the HBM reads instead of one per byte
+                uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+#pragma unroll
+                for (uint32_t i = 0; i < TOK_SHORT; ++i) {
+                    if (i < len) {
+                        const uint32_t p = src + i;
+                        const uint32_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
+                        const uint32_t sh = 8 * (i & 3);
+                        if (i < 4) w0 |= v << sh;
+                        else if (i < 8) w1 |= v << sh;
+                        else if (i < 12) w2 |= v << sh;
+                        else w3 |= v << sh;
+                    }
+                }
+#pragma unroll
+                for (uint32_t i = 0; i < TOK_SHORT; ++i) {
+                    if (i < len) {
+                        const uint32_t w = i < 4 ? w0 : i < 8 ? w1 : i < 12 ? w2 : w3;
+                        ring[(o.skew + dst + i) & G::RMASK] = (uint8_t)(w >> (8 * (i & 3)));
+                    }
                 }
             }
             WFENCE();
@@ -1951,7 +2026,7 @@ const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHED
 uint32_t inflate_lds_bytes(int schedule)
 {
     const InflateSchedule &s = INFLATE_SCHEDULES[schedule];
-    if (s.tokens == 3) return 1024 + s.waves_per_group * s.ring; // the copy phase's rings
+    if (s.tokens == 3) return 16384 + s.waves_per_group * s.ring; // the copy phase: CRC tables and rings
     uint32_t per_wave = 0;
     switch (s.ring) {
     case 32768: per_wave = sizeof(WaveMem<32768>); break;
@@ -1968,6 +2043,7 @@ uint32_t inflate_waves_per_cu(int schedule)
 {
     uint32_t groups = (160u * 1024u) / inflate_lds_bytes(schedule);
     uint32_t w = groups * INFLATE_SCHEDULES[schedule].waves_per_group;
+    // register budget: 4 waves per SIMD
     return w > 16 ? 16 : (w ? w : 1);
 }
 
