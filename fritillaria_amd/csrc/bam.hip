@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "frhip_internal.h"
+#include "pipeline_internal.h"
 
 namespace frhip {
 namespace {
@@ -290,6 +291,8 @@ __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t j)
 template <uint32_t F>
 __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
 {
+    if (!a.seg_mb && a.fast_state && a.fast_state[2]) return; // the parallel path did it
+    // (the parallel path runs only without mem_until, where s1 is the next block start)
     const uint32_t lane = threadIdx.x;
     uint32_t mb0 = 0, mb1 = a.nb;
     uint64_t *totals = a.totals;
@@ -403,6 +406,93 @@ __global__ __launch_bounds__(64) void bam_reconcile_kernel(BamReconcileArgs a)
         totals[1] = tail;
         totals[2] = status;
         totals[3] = fallbacks;
+    }
+}
+
+// Parallel reconcile for one chain, in the common case where every member is
+// entered at the previous member's speculative exit and that position is one
+// of its survivors: each member's count and mode follow from its own
+// speculation, first_index from a scan of the counts. The check kernel flags
+// the first member where the assumption fails (state[0]) and the first where
+// the chain ends inside the member (state[1]); the finish kernel writes the
+// outputs when no failure comes before the chain's end and sets state[2], and
+// the serial reconcile that runs next returns at once. Otherwise the serial
+// reconcile produces every output, as without this path.
+__global__ void bam_fast_init_kernel(uint32_t *st)
+{
+    if (threadIdx.x == 0) {
+        st[0] = ~0u;
+        st[1] = ~0u;
+        st[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void bam_fast_check_kernel(BamReconcileArgs a)
+{
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= a.nb) return;
+    const uint64_t s1 = a.block_starts[j + 1];
+    if (s1 == a.block_starts[j]) {
+        // an empty member (the EOF block): the chain passes it by
+        a.entry[j] = BAM_NONE;
+        a.count[j] = 0;
+        a.mode[j] = 0;
+        return;
+    }
+    // entered at the speculative exit of the closest non-empty member before it
+    uint64_t e = a.start_dev ? *a.start_dev : a.start;
+    bool ok = a.spec_ok[j] != 0;
+    uint32_t i = j;
+    while (i > 0 && a.block_starts[i] == a.block_starts[i - 1]) --i;
+    if (i > 0) {
+        ok = ok && a.spec_ok[i - 1] != 0;
+        e = a.exitp[i - 1];
+    }
+    uint32_t found = 0, cnt = 0;
+    if (ok && e < s1) {
+        const uint32_t ns = a.n_surv[j];
+        for (uint32_t k = 0; k < ns && !found; ++k) {
+            if (a.surv_pos[(uint64_t)j * BAM_K + k] == e) {
+                found = k + 1;
+                cnt = a.surv_cnt[(uint64_t)j * BAM_K + k] + a.cnt_tail[j];
+            }
+        }
+    }
+    a.entry[j] = e;
+    a.count[j] = cnt;
+    a.mode[j] = (uint8_t)found;
+    if (!found)
+        atomicMin(&a.fast_state[0], j);
+    else if (!a.complete[j])
+        atomicMin(&a.fast_state[1], j);
+}
+
+__global__ __launch_bounds__(256) void bam_fast_finish_kernel(BamReconcileArgs a)
+{
+    const uint32_t fail = a.fast_state[0], stop = a.fast_state[1];
+    const uint32_t last = stop == ~0u ? a.nb - 1 : stop;
+    if (fail != ~0u && fail <= last) return;
+    if (a.block_starts[a.nb] == a.block_starts[0]) return; // no member holds data
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j < a.nb) {
+        if (j <= last) {
+            a.first_index[j] = a.fast_scan[j];
+        } else {
+            a.first_index[j] = a.fast_scan[last + 1];
+            a.entry[j] = BAM_NONE;
+            a.count[j] = 0;
+            a.mode[j] = 0;
+        }
+    }
+    if (j == 0) {
+        // tail: the exit of the last non-empty member the chain reached
+        uint32_t t = last;
+        while (t > 0 && a.block_starts[t + 1] == a.block_starts[t]) --t;
+        a.totals[0] = a.fast_scan[last + 1];
+        a.totals[1] = a.exitp[t];
+        a.totals[2] = BAM_OK;
+        a.totals[3] = 0;
+        a.fast_state[2] = 1;
     }
 }
 
@@ -582,6 +672,17 @@ hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 
 hipError_t launch_bam_reconcile(const BamReconcileArgs &a, hipStream_t s)
 {
+    if (a.fast_state && a.fast_scan && a.nb > 0 && !a.mem_until) {
+        const uint32_t grid = (a.nb + 255) / 256;
+        hipLaunchKernelGGL(bam_fast_init_kernel, dim3(1), dim3(64), 0, s, a.fast_state);
+        hipLaunchKernelGGL(bam_fast_check_kernel, dim3(grid), dim3(256), 0, s, a);
+        hipError_t r = hipGetLastError();
+        if (r == hipSuccess) r = launch_scan_u32_u64(a.count, a.fast_scan, nullptr, a.nb, s);
+        if (r != hipSuccess) return r;
+        hipLaunchKernelGGL(bam_fast_finish_kernel, dim3(grid), dim3(256), 0, s, a);
+        r = hipGetLastError();
+        if (r != hipSuccess) return r;
+    }
     if (a.format == CHAIN_BCF)
         hipLaunchKernelGGL(bam_reconcile_kernel<CHAIN_BCF>, dim3(1), dim3(64), 0, s, a);
     else

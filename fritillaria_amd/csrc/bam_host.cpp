@@ -115,6 +115,8 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     ra.first_index = (uint64_t *)(s + L.o_first);
     ra.mode = s + L.o_mode;
     ra.totals = (uint64_t *)(s + L.o_totals);
+    ra.fast_scan = (uint64_t *)(s + L.o_fscan);
+    ra.fast_state = (uint32_t *)(s + L.o_fstate);
     r = launch_bam_reconcile(ra, ctx->stream);
     if (r != hipSuccess) return hip_err(err, r, "launching bam_reconcile");
     uint64_t totals[4];

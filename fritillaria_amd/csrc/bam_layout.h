@@ -11,7 +11,7 @@ namespace frhip {
 struct BamScratchLayout {
     uint64_t bytes;
     uint64_t o_bs, o_coff, o_ok, o_ns, o_spos, o_scnt, o_land, o_exit, o_ctail, o_comp, o_rel, o_entry, o_count,
-        o_first, o_mode, o_totals;
+        o_first, o_mode, o_totals, o_fscan, o_fstate;
 };
 
 inline uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
@@ -37,6 +37,8 @@ inline BamScratchLayout bam_layout(uint64_t nb)
     L.o_first = take(nb * 8);
     L.o_mode = take(nb);
     L.o_totals = take(4 * 8);
+    L.o_fscan = take((nb + 1) * 8);
+    L.o_fstate = take(4 * 4);
     L.bytes = p;
     return L;
 }

@@ -128,6 +128,8 @@ extern "C" int frhip_bcf_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     ra.first_index = (uint64_t *)(s + L.o_first);
     ra.mode = s + L.o_mode;
     ra.totals = (uint64_t *)(s + L.o_totals);
+    ra.fast_scan = (uint64_t *)(s + L.o_fscan);
+    ra.fast_state = (uint32_t *)(s + L.o_fstate);
     r = launch_bam_reconcile(ra, st);
     uint64_t totals[4];
     if (r == hipSuccess) r = hipMemcpyAsync(totals, ra.totals, sizeof totals, hipMemcpyDeviceToHost, st);

@@ -108,6 +108,10 @@ struct BamReconcileArgs {
     const uint64_t *seg_len;
     uint64_t *seg_totals;
     const uint64_t *mem_until; // as in BamSpecArgs
+    // Single-chain form only: scratch for the parallel fast path ((nb + 1)
+    // counts scanned, 4 words of state), or null for the serial reconcile alone.
+    uint64_t *fast_scan;
+    uint32_t *fast_state;
 };
 
 struct BamEmitArgs {

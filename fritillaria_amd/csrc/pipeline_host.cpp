@@ -190,6 +190,8 @@ hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
     ra.first_index = (uint64_t *)(b + L.o_first);
     ra.mode = b + L.o_mode;
     ra.totals = p->d_res->totals;
+    ra.fast_scan = (uint64_t *)(b + L.o_fscan);
+    ra.fast_state = (uint32_t *)(b + L.o_fstate);
     if (r == hipSuccess) r = launch_bam_reconcile(ra, s);
     rec(EV_REC);
     BamEmitArgs ea{};
