@@ -246,6 +246,10 @@ def main():
                "records_match": r["records"] == n_rec}
         log(f"cpu baseline: {cpu['value']:.2f} GB/s on {cores} threads ({cdt:.2f} s)")
 
+    variant = fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None
+    two_phase = variant is not None and variant[3] == 3
+    inflate_kernel = ("fr_inflate_lanes_kernel + fr_inflate_resolve_kernel (two-phase inflate: lane-per-member "
+                      "decode, then wave-per-member copies and CRC)") if two_phase else "fr_inflate_kernel"
     traffic = measured_traffic(args.records, args.schedule)
     if traffic is not None:
         log(f"inflate HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic} bytes per launch")
@@ -277,7 +281,7 @@ def main():
                         "14 reference columns + virtual offsets, one hipGraph per step",
             "records": n_rec, "members": n_members, "inflated_bytes": U, "compressed_bytes": C,
             "graph": not args.no_graph, "inflate_schedule": args.schedule,
-            "inflate_variant": fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None,
+            "inflate_variant": variant,
             "order": args.order, "bam_window": args.window or 2048,
             "autotune": None if tuned is None else {"best": tuned["best"], "sample_bytes": tuned["sample_bytes"]},
         },
@@ -287,7 +291,7 @@ def main():
         "stage_ms": {k: round(v, 3) for k, v in stages.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
-                     "kernel": "fr_inflate_kernel", "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel": inflate_kernel, "algorithmic_bytes_per_launch": alg_bytes,
                      "launch_ms": round(inflate_ms, 3)},
         "pipeline_roofline": {"algorithmic_bytes": pipe_bytes,
                               "achieved_gbps": round(pipe_bytes / (ms_per_step / 1e3) / 1e9, 2),

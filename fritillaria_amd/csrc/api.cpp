@@ -251,6 +251,11 @@ extern "C" int frhip_ctx_set_inflate_schedule(frhip_ctx *c, int schedule)
     c->inflate_schedule = schedule;
     return FRHIP_OK;
 }
+extern "C" int frhip_ctx_set_two_phase_min_members(frhip_ctx *c, uint32_t min_members)
+{
+    c->two_phase_min = min_members;
+    return FRHIP_OK;
+}
 extern "C" int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu,
                                        uint32_t *tokens, int cap)
 {
@@ -500,6 +505,7 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
         if (!d_rec) { buf_free(o); return rc; }
     }
     inflate_scratch_bind(a, d_rec, n);
+    a.two_phase_min = ctx->two_phase_min;
     r = launch_inflate(a, (uint32_t)n, ctx->stream, ctx->inflate_schedule);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
     if (t) (void)hipEventRecord(e2, ctx->stream);

@@ -23,6 +23,7 @@ struct InflateArgs {
     const uint32_t *x2n;     // x^(2^k) mod P, k = 0..31
     uint32_t *nrec;          // two-phase schedules: match records per member
     uint32_t *rec;           // two-phase schedules: match record lists
+    uint32_t two_phase_min;  // two-phase schedules: fewer members run the one-wave form
 };
 
 // Inflate kernel variants: output ring bytes per member and waves per workgroup.

@@ -57,6 +57,7 @@ struct frhip_ctx {
     uint32_t *x2n = nullptr;
     uint32_t bam_window = 2048;
     int inflate_schedule = 5; // index into INFLATE_SCHEDULES
+    uint32_t two_phase_min = 8192; // smaller batches of a two-phase schedule run the one-wave form
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
         rec_scratch;
     frhip::PinnedScratch pinned;

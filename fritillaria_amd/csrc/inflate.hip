@@ -2050,6 +2050,12 @@ uint32_t inflate_waves_per_cu(int schedule)
 hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stream, int schedule)
 {
     if (a.n == 0) return hipSuccess;
+    // The two-phase form decodes one member per lane: a batch too small to
+    // fill the GPU that way takes a member's whole decode time, so it runs the
+    // lane-speculative one-wave form with the same ring instead (same bytes).
+    if (schedule >= 0 && schedule < N_INFLATE_SCHEDULES && INFLATE_SCHEDULES[schedule].tokens == 3 &&
+        a.n < a.two_phase_min)
+        schedule = INFLATE_SCHEDULES[schedule].ring == 4096 ? 15 : 13;
     switch (schedule) {
     case 0: return launch_variant<32768, 1>(a, waves, stream);
     case 1: return launch_variant<16384, 1>(a, waves, stream);

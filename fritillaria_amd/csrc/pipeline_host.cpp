@@ -139,6 +139,7 @@ hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
         a.x2n = p->ctx->x2n;
         const int sched = p->sched;
         inflate_scratch_bind(a, p->d_rec, p->n);
+        a.two_phase_min = p->ctx->two_phase_min;
         uint32_t wpc = p->cfg.waves_per_cu ? p->cfg.waves_per_cu : inflate_waves_per_cu(sched);
         uint64_t waves = (uint64_t)p->ctx->n_cu * wpc;
         if (waves > p->n) waves = p->n;

@@ -335,6 +335,7 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
         if (!d_rec) return rc;
     }
     inflate_scratch_bind(a, d_rec, nb);
+    a.two_phase_min = ctx->two_phase_min;
     if (r == hipSuccess) {
         const int sched = ctx->inflate_schedule;
         uint64_t waves = (uint64_t)ctx->n_cu * inflate_waves_per_cu(sched);

@@ -135,6 +135,7 @@ def lib():
         "frhip_ctx_set_bam_window": (C.c_int, [_vp, C.c_uint32]),
         "frhip_ctx_synchronize": (C.c_int, [_vp, _ep]),
         "frhip_ctx_set_inflate_schedule": (C.c_int, [_vp, C.c_int]),
+        "frhip_ctx_set_two_phase_min_members": (C.c_int, [_vp, C.c_uint32]),
         "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
         "frhip_codec_name": (C.c_char_p, []),
         "frhip_device_codec_name": (C.c_char_p, []),
@@ -287,6 +288,10 @@ class HipContext:
     def set_inflate_schedule(self, schedule: int) -> None:
         if lib().frhip_ctx_set_inflate_schedule(self._h, schedule) != OK:
             raise ValueError(f"no inflate schedule {schedule}")
+
+    def set_two_phase_min_members(self, n: int) -> None:
+        """Batches of fewer members than n run a two-phase schedule's one-wave form (default 8192)."""
+        lib().frhip_ctx_set_two_phase_min_members(self._h, int(n))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         """A device buffer of nbytes (contents undefined)."""

@@ -88,9 +88,16 @@ int frhip_ctx_set_bam_window(frhip_ctx *ctx, uint32_t window);
    an index into the table frhip_inflate_schedules describes. Results do not
    depend on it. No reference counterpart. */
 int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
+/* With a two-phase schedule (token form 3: one member per lane, then one wave
+   per member), batches of fewer than min_members members run the one-wave
+   lane-speculative form of the same ring size instead, which finishes a small
+   batch sooner (default 8192). Results do not depend on it. No reference
+   counterpart. */
+int frhip_ctx_set_two_phase_min_members(frhip_ctx *ctx, uint32_t min_members);
 /* Writes up to cap entries of (ring bytes, waves per workgroup, resident waves
-   per CU, token form: 1 when the symbols are decoded into a token list and
-   resolved 64 at a time); returns the number of schedules. Any array may be NULL. */
+   per CU, token form: 0 symbols straight into the ring, 1 token lists resolved
+   64 at a time, 2 lane-speculative token lists, 3 two-phase); returns the
+   number of schedules. Any array may be NULL. */
 int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu, uint32_t *tokens,
                             int cap);
 /* Codec names of CudaCodec (lib.rs:152-294): host path and device path. */

@@ -106,3 +106,22 @@ def test_pipeline_reports_a_bad_crc_like_the_per_call_path(hip_ctx):
     with pytest.raises(fa.ChecksumMismatch) as ei:
         pipe.run()
     assert ei.value.offset == len(good) and ei.value.expected == 1
+
+
+@pytest.mark.parametrize("schedule", [16, 18])
+@pytest.mark.parametrize("order", [0, 1])
+def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order):
+    """The two-phase inflate (one member per lane, then one wave per member) in
+    the graph-replayed pipeline, each replay against the oracle."""
+    print(f"[gpu] two-phase pipeline schedule {schedule} order {order}", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    hip_ctx.set_two_phase_min_members(0)  # captured into the graph at creation
+    try:
+        s = synth.config_bam(2, n_records=60000)
+        pipe = fa.BatchedPipeline(hip_ctx, s.compressed, inflate_schedule=schedule, order=order, use_graph=True)
+    finally:
+        hip_ctx.set_two_phase_min_members(8192)
+    for rep in range(3):
+        pipe.run()
+        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0))
