@@ -1755,6 +1755,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     static_assert(FAST_CHUNK + SPAN + WIN <= RING, "fast write-back must stay clear of the window writes");
     __shared__ uint32_t t16[16 * 256]; // t16[0] is the byte-at-a-time table
     __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
+    __shared__ uint4 stage_all[NW][64][2]; // per lane: the 32 aligned bytes around a far match source
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
         uint32_t c = i;
@@ -1869,7 +1870,17 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 o.visible = o.flushed;
             }
-            if (match && !deferred) {
+            if (match && !deferred && src + len <= lim) {
+                // wholly in HBM: the one or two aligned 16-byte blocks holding
+                // it, through this lane's LDS stage (two requests, not one per byte)
+                const uint32_t head = (uint32_t)((uintptr_t)(o.g + src) & 15);
+                const uint4 *blk = reinterpret_cast<const uint4 *>(o.g + src - head);
+                uint4 *stg = stage_all[wave][lane];
+                stg[0] = blk[0];
+                if (head + len > 16) stg[1] = blk[1];
+                const uint8_t *sb = reinterpret_cast<const uint8_t *>(stg) + head;
+                for (uint32_t i = 0; i < len; ++i) ring[(o.skew + dst + i) & G::RMASK] = sb[i];
+            } else if (match && !deferred) {
                 // all source bytes are read before any is written (sources lie
                 // before R0, so no write of this round can change them): one
                 // wait for the HBM reads instead of one per byte
@@ -2026,7 +2037,7 @@ const int N_INFLATE_SCHEDULES = sizeof(INFLATE_SCHEDULES) / sizeof(INFLATE_SCHED
 uint32_t inflate_lds_bytes(int schedule)
 {
     const InflateSchedule &s = INFLATE_SCHEDULES[schedule];
-    if (s.tokens == 3) return 16384 + s.waves_per_group * s.ring; // the copy phase: CRC tables and rings
+    if (s.tokens == 3) return 16384 + s.waves_per_group * (s.ring + 2048); // the copy phase: CRC tables, rings, stages
     uint32_t per_wave = 0;
     switch (s.ring) {
     case 32768: per_wave = sizeof(WaveMem<32768>); break;
