@@ -1364,15 +1364,19 @@ template <uint32_t NW>
 __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a)
 {
     constexpr uint32_t LANES = 64 * NW;
-    // per lane: the literal/length symbol list (288 entries) then the distance
-    // list (32); entry e of lane t at [e * LANES + t]
-    __shared__ uint16_t tab_all[(288 + 32) * LANES];
+    // Per lane, in lane-minor 32-bit words (word w of lane t at [w * LANES + t]):
+    // the literal/length symbol list as bytes (symbol & 255; 72 words), one bit
+    // per entry set when the symbol is 256 or above (9 words), the distance
+    // list as bytes (8 words). 356 bytes per lane: 7 waves per CU.
+    __shared__ uint32_t tab_all[(72 + 9 + 8) * LANES];
     const uint32_t t = threadIdx.x;
     const uint32_t slot = blockIdx.x * LANES + t;
     if (slot >= a.n) return;
     const uint32_t m = a.order ? a.order[slot] : slot;
-    uint16_t *lsym = tab_all + t;
-    uint16_t *dsym = tab_all + 288 * LANES + t;
+    uint32_t *lw = tab_all + t;
+    uint32_t *hw = tab_all + 72 * LANES + t;
+    uint32_t *dw = tab_all + 81 * LANES + t;
+#define LANE_BYTE(base, e) (reinterpret_cast<uint8_t *>((base) + ((e) >> 2) * LANES)[(e) & 3])
     const uint64_t in_off = a.in_off[m];
     const uint32_t in_len = a.in_len[m];
     const uint64_t out_off = a.out_off[m];
@@ -1415,7 +1419,8 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             if (b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
             uint32_t idx = lb_decode(b, lc, ST_BAD_CODE, err);
             if (err == ST_OK) {
-                const uint32_t sym = lsym[(idx < 288 ? idx : 287) * LANES] & 0xFFFu;
+                const uint32_t e = idx < 288 ? idx : 287;
+                const uint32_t sym = LANE_BYTE(lw, e) | (((hw[(e >> 5) * LANES] >> (e & 31)) & 1u) << 8);
                 if (sym < 256) {
                     if (pos >= cap)
                         err = ST_OUTPUT_OVERFLOW;
@@ -1434,7 +1439,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                     }
                     uint32_t dist = 0;
                     if (err == ST_OK) {
-                        const uint32_t ds = dsym[(idx < 32 ? idx : 31) * LANES];
+                        const uint32_t ds = LANE_BYTE(dw, idx < 32 ? idx : 31);
                         if (ds >= 30)
                             err = ST_BAD_DISTANCE;
                         else
@@ -1502,9 +1507,20 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
         }
         if (type == 1) {
             // fixed code: sorted lists in closed form
-            for (uint32_t e = 0; e < 288; ++e)
-                lsym[e * LANES] = (uint16_t)(e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176);
-            for (uint32_t e = 0; e < 32; ++e) dsym[e * LANES] = (uint16_t)e;
+            auto fixed_sym = [](uint32_t e) -> uint32_t {
+                return e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176;
+            };
+            for (uint32_t w = 0; w < 72; ++w) {
+                uint32_t v = 0;
+                for (uint32_t j = 0; j < 4; ++j) v |= (fixed_sym(4 * w + j) & 255u) << (8 * j);
+                lw[w * LANES] = v;
+            }
+            for (uint32_t w = 0; w < 9; ++w) {
+                uint32_t v = 0;
+                for (uint32_t j = 0; j < 32; ++j) v |= (uint32_t)(fixed_sym(32 * w + j) >= 256) << j;
+                hw[w * LANES] = v;
+            }
+            for (uint32_t w = 0; w < 8; ++w) dw[w * LANES] = (4 * w) | (4 * w + 1) << 8 | (4 * w + 2) << 16 | (4 * w + 3) << 24;
             (void)lane_code(lc, 288, [](int k) -> uint32_t { return k == 7 ? 24 : k == 8 ? 152 : k == 9 ? 112 : 0; });
             (void)lane_code(dc, 32, [](int k) -> uint32_t { return k == 5 ? 32 : 0; });
             state = LS_CODES;
@@ -1553,102 +1569,92 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             state = LS_DONE;
             continue;
         }
-        // One pass over the code lengths counts them per length and keeps
-        // them: literal/length lengths in the high nibble of the lane's
-        // symbol-list slots (slot s = symbol s), distance lengths in register
-        // nibbles. The sorted lists are then placed in the slots' low bits.
+        // Two passes over the code lengths (from the same input position): the
+        // first counts them per length and checks them, the second places the
+        // symbols in the sorted lists. There is no room in LDS to keep the
+        // lengths between the two.
         const uint32_t total = nlen + ndist;
-        F16 cll = {0, 0, 0, 0};
-        F8 cd = {0, 0};
-        uint32_t dl0 = 0, dl1 = 0, dl2 = 0, dl3 = 0; // distance code lengths, 4 bits each
-        uint32_t index = 0, prev = 0, len256 = 0;
-        while (index < total) {
-            LB_REFILL_SYNC(b);
-            const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
-            if (err != ST_OK) break;
-            const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
-            uint32_t len = 0, rep = 1;
-            if (sym < 16) {
-                len = sym;
-            } else if (sym == 16) {
-                if (index == 0) {
+        LaneBits saved;
+        lb_copy(saved, b);
+        F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
+        F8 cd = {0, 0}, od = {0, 0};
+        for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
+            if (pass == 1) {
+                lb_copy(b, saved);
+                uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    f16_add(oll, k, r0);
+                    r0 += f16_get(cll, k);
+                    f8_add(od, k, r1);
+                    r1 += f8_get(cd, k);
+                }
+                for (uint32_t w = 0; w < 9; ++w) hw[w * LANES] = 0;
+            }
+            uint32_t index = 0, prev = 0, len256 = 0;
+            while (index < total) {
+                LB_REFILL_SYNC(b);
+                const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
+                if (err != ST_OK) break;
+                const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
+                uint32_t len = 0, rep = 1;
+                if (sym < 16) {
+                    len = sym;
+                } else if (sym == 16) {
+                    if (index == 0) {
+                        err = ST_BAD_LENGTH_REPEAT;
+                        break;
+                    }
+                    len = prev;
+                    rep = 3 + lb_bits(b, 2, err);
+                } else if (sym == 17) {
+                    rep = 3 + lb_bits(b, 3, err);
+                } else {
+                    rep = 11 + lb_bits(b, 7, err);
+                }
+                if (err != ST_OK) break;
+                if (index + rep > total) {
                     err = ST_BAD_LENGTH_REPEAT;
                     break;
                 }
-                len = prev;
-                rep = 3 + lb_bits(b, 2, err);
-            } else if (sym == 17) {
-                rep = 3 + lb_bits(b, 3, err);
-            } else {
-                rep = 11 + lb_bits(b, 7, err);
-            }
-            if (err != ST_OK) break;
-            if (index + rep > total) {
-                err = ST_BAD_LENGTH_REPEAT;
-                break;
-            }
-            const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
-            if (len) {
-                f16_add(cll, len, nll);
-                f8_add(cd, len, rep - nll);
-            }
-            for (uint32_t j = 0; j < nll; ++j) lsym[(index + j) * LANES] = (uint16_t)(len << 12);
-            if (len) {
-                for (uint32_t j = nll; j < rep; ++j) {
-                    const uint32_t d = index + j - nlen, sh = 4 * (d & 7), g = d >> 3;
-                    dl0 |= g == 0 ? len << sh : 0;
-                    dl1 |= g == 1 ? len << sh : 0;
-                    dl2 |= g == 2 ? len << sh : 0;
-                    dl3 |= g == 3 ? len << sh : 0;
+                const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
+                if (len) {
+                    if (pass == 0) {
+                        f16_add(cll, len, nll);
+                        f8_add(cd, len, rep - nll);
+                    } else {
+                        for (uint32_t k = 0; k < rep; ++k) {
+                            const uint32_t sy = index + k;
+                            if (sy < nlen) {
+                                uint32_t q = f16_get(oll, len);
+                                f16_add(oll, len, 1);
+                                q = q < 288 ? q : 287;
+                                LANE_BYTE(lw, q) = (uint8_t)sy;
+                                if (sy >= 256) hw[(q >> 5) * LANES] |= 1u << (q & 31);
+                            } else {
+                                uint32_t q = f8_get(od, len);
+                                f8_add(od, len, 1);
+                                LANE_BYTE(dw, q < 32 ? q : 31) = (uint8_t)(sy - nlen);
+                            }
+                        }
+                    }
                 }
+                if (index <= 256 && 256 < index + rep) len256 = len;
+                prev = len;
+                index += rep;
             }
-            if (index <= 256 && 256 < index + rep) len256 = len;
-            prev = len;
-            index += rep;
-        }
-        if (err == ST_OK) {
-            uint32_t sll = 0, sd = 0;
+            if (pass == 0 && err == ST_OK) {
+                uint32_t sll = 0, sd = 0;
 #pragma unroll
-            for (uint32_t k = 1; k < 16; ++k) {
-                sll += f16_get(cll, k);
-                sd += f8_get(cd, k);
-            }
-            if (len256 == 0) err = ST_BAD_TABLE;
-            int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
-            if (e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
-            e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
-            if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
-        }
-        if (err == ST_OK) {
-            F16 oll = {0, 0, 0, 0};
-            F8 od = {0, 0};
-            uint32_t r0 = 0, r1 = 0;
-#pragma unroll
-            for (uint32_t k = 1; k < 16; ++k) {
-                f16_add(oll, k, r0);
-                r0 += f16_get(cll, k);
-                f8_add(od, k, r1);
-                r1 += f8_get(cd, k);
-            }
-            // slot p's own length stays in its high nibble until slot p is read
-            for (uint32_t sy = 0; sy < nlen; ++sy) {
-                const uint32_t L = lsym[sy * LANES] >> 12;
-                if (L) {
-                    const uint32_t q = f16_get(oll, L);
-                    f16_add(oll, L, 1);
-                    uint16_t *slot = lsym + (q < 288 ? q : 287) * LANES;
-                    *slot = (uint16_t)((*slot & 0xF000u) | sy);
+                for (uint32_t k = 1; k < 16; ++k) {
+                    sll += f16_get(cll, k);
+                    sd += f8_get(cd, k);
                 }
-            }
-            for (uint32_t d = 0; d < ndist; ++d) {
-                const uint32_t g = d >> 3;
-                const uint32_t w = g == 0 ? dl0 : g == 1 ? dl1 : g == 2 ? dl2 : dl3;
-                const uint32_t L = (w >> (4 * (d & 7))) & 15;
-                if (L) {
-                    const uint32_t q = f8_get(od, L);
-                    f8_add(od, L, 1);
-                    dsym[(q < 32 ? q : 31) * LANES] = (uint16_t)d;
-                }
+                if (len256 == 0) err = ST_BAD_TABLE;
+                int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
+                if (err == ST_OK && e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
+                e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
+                if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
             }
         }
         state = err == ST_OK ? LS_CODES : LS_DONE;
@@ -1657,6 +1663,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     a.produced[m] = pos;
     a.crc[m] = 0;
     a.nrec[m] = nrec;
+#undef LANE_BYTE
 }
 
 // Four bytes of a partial block, skipping those at or past `end`.
