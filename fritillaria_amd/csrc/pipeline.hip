@@ -52,6 +52,35 @@ __device__ __forceinline__ uint32_t header_at(const uint8_t *f, uint64_t len, ui
 
 constexpr uint32_t SIEVE_TILE = 16384;
 constexpr uint32_t SIEVE_THREADS = 256;
+constexpr uint32_t SIEVE_CHUNK = 16; // bytes per thread per pass over a 4 KiB stretch
+
+// Bit 7 of each byte of w set where that byte is 0x1f (exact, no carries
+// between bytes).
+__device__ __forceinline__ uint32_t bytes_eq_1f(uint32_t w)
+{
+    const uint32_t t = w ^ 0x1f1f1f1fu;
+    return ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t | 0x7f7f7f7fu);
+}
+__device__ __forceinline__ uint32_t nibble_1f(uint32_t w)
+{
+    const uint32_t m = bytes_eq_1f(w);
+    return ((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u);
+}
+
+// Offsets p .. p+15 (p a multiple of 16) holding the header's first byte
+// 0x1f, as a 16-bit mask: one aligned 16-byte load (bytes past the end of the
+// file are read one at a time). Only these offsets go through header_at.
+__device__ __forceinline__ uint32_t magic_mask(const uint8_t *f, uint64_t len, uint64_t p)
+{
+    if (p >= len) return 0;
+    if (p + 16 <= len) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(f + p);
+        return nibble_1f(v.x) | (nibble_1f(v.y) << 4) | (nibble_1f(v.z) << 8) | (nibble_1f(v.w) << 12);
+    }
+    uint32_t m = 0;
+    for (uint32_t j = 0; p + j < len && j < 16; ++j) m |= (uint32_t)(f[p + j] == 0x1f) << j;
+    return m;
+}
 
 // Pass 1: candidates per tile.
 __global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_t *f, uint64_t len, uint32_t *counts,
@@ -59,12 +88,17 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_
 {
     __shared__ uint32_t wsum[SIEVE_THREADS / 64];
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        uint64_t base = t * SIEVE_TILE;
+        const uint64_t base = t * SIEVE_TILE;
         uint32_t c = 0;
-        for (uint32_t k = threadIdx.x; k < SIEVE_TILE; k += SIEVE_THREADS) {
-            uint64_t p = base + k;
-            uint32_t hs;
-            if (p < len && header_at(f, len, p, &hs)) ++c;
+        for (uint32_t k = threadIdx.x * SIEVE_CHUNK; k < SIEVE_TILE; k += SIEVE_THREADS * SIEVE_CHUNK) {
+            const uint64_t p = base + k;
+            uint32_t m = magic_mask(f, len, p);
+            while (m) {
+                const uint32_t j = (uint32_t)__ffs(m) - 1;
+                m &= m - 1;
+                uint32_t hs;
+                if (header_at(f, len, p + j, &hs)) ++c;
+            }
         }
         for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
         if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
@@ -78,7 +112,9 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_
     }
 }
 
-// Pass 2: candidates written in file order at the tile's scanned base.
+// Pass 2: candidates written in file order at the tile's scanned base. Each
+// thread takes 16 consecutive offsets; a block-wide exclusive scan of the
+// per-thread counts places them.
 __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_t *f, uint64_t len,
                                                                     const uint64_t *tile_base, uint64_t n_tiles,
                                                                     uint64_t *cand, uint32_t *cand_bs,
@@ -86,28 +122,47 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_
 {
     __shared__ uint32_t wcnt[SIEVE_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t lt = (1ull << lane) - 1;
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        uint64_t base = t * SIEVE_TILE;
+        const uint64_t base = t * SIEVE_TILE;
         uint64_t out = tile_base[t];
-        for (uint32_t k = 0; k < SIEVE_TILE; k += SIEVE_THREADS) {
-            uint64_t p = base + k + threadIdx.x;
-            uint32_t hs = 0, bs = 0;
-            if (p < len) bs = header_at(f, len, p, &hs);
-            uint64_t m = __ballot(bs != 0);
-            if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+        for (uint32_t k = 0; k < SIEVE_TILE; k += SIEVE_THREADS * SIEVE_CHUNK) {
+            const uint64_t p = base + k + threadIdx.x * SIEVE_CHUNK;
+            const uint32_t mm = magic_mask(f, len, p);
+            uint32_t m = mm, c = 0;
+            while (m) {
+                const uint32_t j = (uint32_t)__ffs(m) - 1;
+                m &= m - 1;
+                uint32_t hs;
+                if (header_at(f, len, p + j, &hs)) ++c;
+            }
+            // exclusive prefix of c over the block, in thread order
+            uint32_t inc = c;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d, 64);
+                if ((int)lane >= d) inc += y;
+            }
+            if (lane == 63) wcnt[w] = inc;
             __syncthreads();
             uint32_t before = 0, total = 0;
             for (uint32_t v = 0; v < SIEVE_THREADS / 64; ++v) {
                 if (v < w) before += wcnt[v];
                 total += wcnt[v];
             }
-            if (bs) {
-                uint64_t i = out + before + (uint64_t)__popcll(m & lt);
-                if (i < cap) {
-                    cand[i] = p;
-                    cand_bs[i] = bs;
-                    cand_hs[i] = hs;
+            uint64_t i = out + before + inc - c;
+            m = mm;
+            while (m) {
+                const uint32_t j = (uint32_t)__ffs(m) - 1;
+                m &= m - 1;
+                uint32_t hs = 0;
+                const uint32_t bs = header_at(f, len, p + j, &hs);
+                if (bs) {
+                    if (i < cap) {
+                        cand[i] = p + j;
+                        cand_bs[i] = bs;
+                        cand_hs[i] = hs;
+                    }
+                    ++i;
                 }
             }
             out += total;
