@@ -1197,16 +1197,27 @@ __device__ __forceinline__ int lane_code(LaneCode &c, uint32_t n, Cnt cnt)
 template <int MAXL = 15>
 __device__ __forceinline__ uint32_t lane_find(const LaneCode &c, uint32_t v, uint32_t &L)
 {
-    // q[] increases with k, so the entry that applies is the largest q[k] <=
-    // v16; two independent chains (even and odd k) keep the compares apart
+    // q[] increases with k, so the entry that applies is the largest of
+    // A = {1 << 12 (length 1), q[0], ..., q[MAXL-2]} at or below v16
     const uint32_t v16 = (v << 16) | 0xFFFFu;
-    uint32_t se = 1u << 12, so = 1u << 12;
+    uint32_t sel;
+    if constexpr (MAXL == 15) {
+        // binary search over A padded to 16 entries with ~0: four compares,
+        // each halving the candidates with selects
+        const bool c1 = v16 >= c.q[7];
+        const uint32_t B0 = c1 ? c.q[7] : (1u << 12), B1 = c1 ? c.q[8] : c.q[0], B2 = c1 ? c.q[9] : c.q[1],
+                       B3 = c1 ? c.q[10] : c.q[2], B4 = c1 ? c.q[11] : c.q[3], B5 = c1 ? c.q[12] : c.q[4],
+                       B6 = c1 ? c.q[13] : c.q[5], B7 = c1 ? ~0u : c.q[6];
+        const bool c2 = v16 >= B4;
+        const uint32_t C0 = c2 ? B4 : B0, C1 = c2 ? B5 : B1, C2 = c2 ? B6 : B2, C3 = c2 ? B7 : B3;
+        const bool c3 = v16 >= C2;
+        const uint32_t D0 = c3 ? C2 : C0, D1 = c3 ? C3 : C1;
+        sel = v16 >= D1 ? D1 : D0;
+    } else {
+        sel = 1u << 12;
 #pragma unroll
-    for (int k = 0; k < MAXL - 1; k += 2) {
-        se = v16 >= c.q[k] ? c.q[k] : se;
-        if (k + 1 < MAXL - 1) so = v16 >= c.q[k + 1] ? c.q[k + 1] : so;
+        for (int k = 0; k < MAXL - 1; ++k) sel = v16 >= c.q[k] ? c.q[k] : sel;
     }
-    const uint32_t sel = se > so ? se : so;
     const uint32_t len = (sel >> 12) & 15;
     const uint32_t lim = MAXL == 15 ? c.lim15 : c.q[MAXL - 1] >> 16;
     L = v < lim ? len : 0;
