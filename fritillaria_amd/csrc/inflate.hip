@@ -1864,16 +1864,24 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             // block only from R0 on (the bytes before R0 are this member's
             // finished output, possibly not yet written back)
             {
-                const uint32_t S = R0 - (uint32_t)((uintptr_t)(o.g + R0) & 15) + 16 * lane;
-                if (lane == 0) {
-#pragma unroll
-                    for (uint32_t j = 0; j < 16; ++j) {
-                        const uint32_t w = j < 4 ? wc.x : j < 8 ? wc.y : j < 12 ? wc.z : wc.w;
-                        if (S + j >= R0) ring[(o.skew + S + j) & G::RMASK] = (uint8_t)(w >> (8 * (j & 3)));
-                    }
-                } else {
-                    *reinterpret_cast<uint4 *>(ring + ((o.skew + S) & G::RMASK)) = wc;
+                const uint32_t head = (uint32_t)((uintptr_t)(o.g + R0) & 15);
+                const uint32_t S = R0 - head + 16 * lane;
+                uint4 *slot = reinterpret_cast<uint4 *>(ring + ((o.skew + S) & G::RMASK));
+                uint4 v = wc;
+                if (lane == 0 && head) {
+                    // keep the ring's bytes below R0: merge by byte masks
+                    const uint4 old = *slot;
+                    auto keep = [&](uint32_t word) -> uint32_t {
+                        const int32_t n = (int32_t)head - (int32_t)(4 * word); // bytes of this word to keep
+                        return n >= 4 ? ~0u : n <= 0 ? 0u : (1u << (8 * n)) - 1;
+                    };
+                    const uint32_t k0 = keep(0), k1 = keep(1), k2 = keep(2), k3 = keep(3);
+                    v.x = (old.x & k0) | (wc.x & ~k0);
+                    v.y = (old.y & k1) | (wc.y & ~k1);
+                    v.z = (old.z & k2) | (wc.z & ~k2);
+                    v.w = (old.w & k3) | (wc.w & ~k3);
                 }
+                *slot = v;
             }
             WFENCE();
             // ring slots are good from here on: the window writes of this and
@@ -1898,29 +1906,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                 if (head + len > 16) stg[1] = blk[1];
                 const uint8_t *sb = reinterpret_cast<const uint8_t *>(stg) + head;
                 for (uint32_t i = 0; i < len; ++i) ring[(o.skew + dst + i) & G::RMASK] = sb[i];
+            } else if (match && !deferred && src >= lim) {
+                // wholly in the ring (sources lie before R0: no write of this
+                // round changes them)
+                for (uint32_t i = 0; i < len; ++i)
+                    ring[(o.skew + dst + i) & G::RMASK] = ring[(o.skew + src + i) & G::RMASK];
             } else if (match && !deferred) {
-                // all source bytes are read before any is written (sources lie
-                // before R0, so no write of this round can change them): one
-                // wait for the HBM reads instead of one per byte
-                uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-#pragma unroll
-                for (uint32_t i = 0; i < TOK_SHORT; ++i) {
-                    if (i < len) {
-                        const uint32_t p = src + i;
-                        const uint32_t v = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
-                        const uint32_t sh = 8 * (i & 3);
-                        if (i < 4) w0 |= v << sh;
-                        else if (i < 8) w1 |= v << sh;
-                        else if (i < 12) w2 |= v << sh;
-                        else w3 |= v << sh;
-                    }
-                }
-#pragma unroll
-                for (uint32_t i = 0; i < TOK_SHORT; ++i) {
-                    if (i < len) {
-                        const uint32_t w = i < 4 ? w0 : i < 8 ? w1 : i < 12 ? w2 : w3;
-                        ring[(o.skew + dst + i) & G::RMASK] = (uint8_t)(w >> (8 * (i & 3)));
-                    }
+                // straddles lim: byte by byte from HBM or the ring
+                for (uint32_t i = 0; i < len; ++i) {
+                    const uint32_t p = src + i;
+                    ring[(o.skew + dst + i) & G::RMASK] = p >= lim ? ring[(o.skew + p) & G::RMASK] : o.g[p];
                 }
             }
             WFENCE();
