@@ -59,20 +59,19 @@ enum : uint32_t {
 // carry no extra bits (lengths 3-10), then each group of four doubles the step;
 // code 28 is length 258. Distance code d (0..29): codes 0-3 are distances 1-4,
 // then each pair doubles the step.
-__device__ __forceinline__ uint32_t len_extra(uint32_t c) { return c < 8 || c == 28 ? 0 : (c - 4) >> 2; }
+// Written as selects (no branches): codes 4-7 and 2-3 fit the general form
+// with no extra bits.
+__device__ __forceinline__ uint32_t len_extra(uint32_t c) { return c < 4 || c == 28 ? 0 : (c - 4) >> 2; }
 __device__ __forceinline__ uint32_t len_base(uint32_t c)
 {
-    if (c < 8) return c + 3;
-    if (c == 28) return 258;
-    uint32_t e = (c - 4) >> 2;
-    return ((4u | (c & 3)) << e) + 3;
+    const uint32_t g = ((4u | (c & 3)) << len_extra(c)) + 3;
+    return c < 4 ? c + 3 : c == 28 ? 258 : g;
 }
-__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 4 ? 0 : (d - 2) >> 1; }
+__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 2 ? 0 : (d - 2) >> 1; }
 __device__ __forceinline__ uint32_t dist_base(uint32_t d)
 {
-    if (d < 4) return d + 1;
-    uint32_t e = (d - 2) >> 1;
-    return ((2u | (d & 1)) << e) + 1;
+    const uint32_t g = ((2u | (d & 1)) << dist_extra(d)) + 1;
+    return d < 2 ? d + 1 : g;
 }
 // Order of the code-length code lengths (RFC 1951 3.2.7), 5 bits per entry.
 __device__ __forceinline__ uint32_t clen_order(uint32_t i)
@@ -1225,31 +1224,32 @@ __device__ __forceinline__ uint32_t lane_find(const LaneCode &c, uint32_t v, uin
 }
 
 // Per-lane LSB-first bit reader. Input is staged in two 32-byte buffers: the
-// decode takes words only from A; B holds the next 32 bytes. B is read (moved
-// into A) and refetched only at points where the whole wave does so -- every
-// few decode steps, or in the header path -- because a wait for a load is a
-// wave-wide counter: a buffer that some lane reads every step must never have
-// a load in flight.
+// decode takes words only from A, a queue whose next word is always A0.x;
+// B holds the next 32 bytes. B is read (moved into A) and refetched only at
+// points where the whole wave does so -- every few decode steps, or in the
+// header path -- because a wait for a load is a wave-wide counter: a buffer
+// that some lane reads every step must never have a load in flight.
 struct LaneBits {
     uint64_t bb;      // next bits, LSB first (may run past the payload)
     uint32_t bc;      // bits in bb
     uint32_t avail;   // payload bits not yet consumed
-    uint32_t k;       // next word of A (8: A used up)
+    uint32_t k;       // words of A used (8: A used up)
     uint4 A0, A1, B0, B1;
     const uint4 *ip;  // next 32 bytes to fetch
 };
 
-__device__ __forceinline__ uint32_t pick(bool c, uint32_t x, uint32_t y)
-{
-    const uint32_t m = 0u - (uint32_t)c;
-    return (x & m) | (y & ~m);
-}
-
-// Word k (0..7) of A, by masks: a dynamic index (or a select the compiler
-// turns into one) would put the reader in scratch memory.
-#define LB_WORD(b)                                                                                         \
-    pick((b).k & 4, pick((b).k & 2, pick((b).k & 1, (b).A1.w, (b).A1.z), pick((b).k & 1, (b).A1.y, (b).A1.x)), \
-         pick((b).k & 2, pick((b).k & 1, (b).A0.w, (b).A0.z), pick((b).k & 1, (b).A0.y, (b).A0.x)))
+// Drops the front word of A (a dynamic index into A would put the reader in
+// scratch memory).
+#define LB_SHIFT(b)                                                                                       \
+    do {                                                                                                  \
+        (b).A0.x = (b).A0.y;                                                                              \
+        (b).A0.y = (b).A0.z;                                                                              \
+        (b).A0.z = (b).A0.w;                                                                              \
+        (b).A0.w = (b).A1.x;                                                                              \
+        (b).A1.x = (b).A1.y;                                                                              \
+        (b).A1.y = (b).A1.z;                                                                              \
+        (b).A1.z = (b).A1.w;                                                                              \
+    } while (0)
 
 // Moves B into A and refetches B. The empty statement (with a memory
 // clobber) keeps the loads after the moves, so they land in B's own
@@ -1271,9 +1271,10 @@ __device__ __forceinline__ uint32_t pick(bool c, uint32_t x, uint32_t y)
 #define LB_REFILL(b)                                                                                      \
     do {                                                                                                  \
         if ((b).bc < 32 && (b).k < 8) {                                                                   \
-            (b).bb |= (uint64_t)LB_WORD(b) << (b).bc;                                                     \
+            (b).bb |= (uint64_t)(b).A0.x << (b).bc;                                                       \
             (b).bc += 32;                                                                                 \
             ++(b).k;                                                                                      \
+            LB_SHIFT(b);                                                                                  \
         }                                                                                                 \
     } while (0)
 
@@ -1406,6 +1407,9 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
         b.B1 = blk[3];
         b.ip = blk + 4;
         const uint32_t byte = (uint32_t)(in_off & 15);
+#pragma unroll
+        for (uint32_t i = 0; i < 3; ++i)
+            if (i < (byte >> 2)) LB_SHIFT(b);
         b.k = byte >> 2;
         b.bb = 0;
         b.bc = 0;
@@ -1419,256 +1423,285 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     // the member's parameters have landed: otherwise their first use inside
     // the loop would leave a wait on the memory counter in every step
     __builtin_amdgcn_s_waitcnt(0);
-    for (uint32_t step = 1; state != LS_DONE; ++step) {
-        // every 4th step (the same step in every lane): lanes that used up A
-        // take B and refetch it
-        if ((step & 3) == 0 && b.k == 8) LB_SWAP(b);
-        if (state == LS_CODES) {
-            LB_REFILL(b);
-            // a symbol needs up to 20 bits, then (a match) a refill and 28;
-            // a lane without them waits for its next swap
-            if (b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
-            uint32_t idx = lb_decode(b, lc, ST_BAD_CODE, err);
-            if (err == ST_OK) {
-                const uint32_t e = idx < 288 ? idx : 287;
-                const uint32_t sym = LANE_BYTE(lw, e) | (((hw[(e >> 5) * LANES] >> (e & 31)) & 1u) << 8);
-                if (sym < 256) {
-                    if (pos >= cap)
-                        err = ST_OUTPUT_OVERFLOW;
-                    else
-                        out[pos++] = (uint8_t)sym;
-                } else if (sym == 256) {
-                    state = last ? LS_DONE : LS_HEADER;
-                } else if (sym - 257 >= 29) {
-                    err = ST_BAD_CODE;
-                } else {
-                    const uint32_t lcode = sym - 257;
-                    const uint32_t len = len_base(lcode) + lb_bits(b, len_extra(lcode), err);
-                    if (err == ST_OK) {
-                        LB_REFILL(b);
-                        idx = lb_decode(b, dc, ST_BAD_DISTANCE, err);
-                    }
-                    uint32_t dist = 0;
-                    if (err == ST_OK) {
-                        const uint32_t ds = LANE_BYTE(dw, idx < 32 ? idx : 31);
-                        if (ds >= 30)
-                            err = ST_BAD_DISTANCE;
-                        else
-                            dist = dist_base(ds) + lb_bits(b, dist_extra(ds), err);
-                    }
-                    if (err == ST_OK) {
-                        if (dist > pos) {
-                            err = ST_BAD_DISTANCE;
-                        } else if (pos + len > cap) {
-                            err = ST_OUTPUT_OVERFLOW;
-                        } else {
-                            uint32_t gap = pos - mark;
-                            while (gap >= REC_SKIP) {
-                                rec[nrec++] = REC_SKIP << 23;
-                                gap -= REC_SKIP;
-                            }
-                            rec[nrec++] = (gap << 23) | ((len - 3) << 15) | (dist - 1);
-                            pos += len;
-                            mark = pos;
-                        }
-                    }
-                }
-            }
-            if (err != ST_OK) state = LS_DONE;
-            continue;
-        }
-        // block header
-        LB_REFILL_SYNC(b);
-        last = lb_bits(b, 1, err);
-        const uint32_t type = lb_bits(b, 2, err);
-        if (err != ST_OK) {
-            state = LS_DONE;
-            continue;
-        }
-        if (type == 0) {
-            // stored: to a byte boundary, then LEN, NLEN and the bytes
-            const uint32_t skip = (in_len * 8 - b.avail) & 7;
-            (void)lb_bits(b, skip ? 8 - skip : 0, err);
-            if (b.avail < 32) {
-                err = ST_TRUNCATED;
-            } else {
+    // Each pass of the loop: the block header for lanes at one, then four
+    // decode steps. B moves into A once per pass (the same point in every
+    // lane), so no decode step waits for memory.
+    while (state != LS_DONE) {
+        if (state == LS_HEADER) {
+                // block header
                 LB_REFILL_SYNC(b);
-                const uint32_t len = lb_bits(b, 16, err);
-                LB_REFILL_SYNC(b);
-                const uint32_t nlen = lb_bits(b, 16, err);
-                if (len != (~nlen & 0xFFFFu))
-                    err = ST_STORED_LEN;
-                else if (len * 8 > b.avail)
-                    err = ST_TRUNCATED;
-                else if (pos + len > cap)
-                    err = ST_OUTPUT_OVERFLOW;
-                else
-                    for (uint32_t i = 0; i < len; ++i) {
-                        LB_REFILL_SYNC(b);
-                        out[pos++] = (uint8_t)lb_bits(b, 8, err);
-                    }
-            }
-            state = err != ST_OK || last ? LS_DONE : LS_HEADER;
-            continue;
-        }
-        if (type == 3) {
-            err = ST_BLOCK_TYPE;
-            state = LS_DONE;
-            continue;
-        }
-        if (type == 1) {
-            // fixed code: sorted lists in closed form
-            auto fixed_sym = [](uint32_t e) -> uint32_t {
-                return e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176;
-            };
-            for (uint32_t w = 0; w < 72; ++w) {
-                uint32_t v = 0;
-                for (uint32_t j = 0; j < 4; ++j) v |= (fixed_sym(4 * w + j) & 255u) << (8 * j);
-                lw[w * LANES] = v;
-            }
-            for (uint32_t w = 0; w < 9; ++w) {
-                uint32_t v = 0;
-                for (uint32_t j = 0; j < 32; ++j) v |= (uint32_t)(fixed_sym(32 * w + j) >= 256) << j;
-                hw[w * LANES] = v;
-            }
-            for (uint32_t w = 0; w < 8; ++w) dw[w * LANES] = (4 * w) | (4 * w + 1) << 8 | (4 * w + 2) << 16 | (4 * w + 3) << 24;
-            (void)lane_code(lc, 288, [](int k) -> uint32_t { return k == 7 ? 24 : k == 8 ? 152 : k == 9 ? 112 : 0; });
-            (void)lane_code(dc, 32, [](int k) -> uint32_t { return k == 5 ? 32 : 0; });
-            state = LS_CODES;
-            continue;
-        }
-        // dynamic header
-        const uint32_t nlen = lb_bits(b, 5, err) + 257;
-        const uint32_t ndist = lb_bits(b, 5, err) + 1;
-        const uint32_t ncode = lb_bits(b, 4, err) + 4;
-        if (err == ST_OK && (nlen > 286 || ndist > 30)) err = ST_BAD_COUNTS;
-        uint64_t cl = 0; // code-length code lengths, 3 bits per symbol 0..18
-#pragma unroll
-        for (uint32_t i = 0; i < 19; ++i) {
-            if (err == ST_OK && i < ncode) {
-                LB_REFILL_SYNC(b);
-                cl |= (uint64_t)lb_bits(b, 3, err) << (3 * clen_order(i));
-            }
-        }
-        LaneCode cc;
-        uint64_t cs0 = 0, cs1 = 0; // code-length code symbols sorted, 5 bits each
-        if (err == ST_OK) {
-            F8 c8 = {0, 0};
-#pragma unroll
-            for (uint32_t s = 0; s < 19; ++s) f8_add(c8, (uint32_t)(cl >> (3 * s)) & 7, 1);
-            const int e = lane_code(cc, 19, [&](int k) -> uint32_t { return k < 8 ? f8_get(c8, (uint32_t)k) : 0; });
-            if (e != 0) err = ST_BAD_TABLE;
-            F8 o8 = {0, 0};
-            uint32_t run = 0;
-#pragma unroll
-            for (uint32_t k = 1; k < 8; ++k) {
-                f8_add(o8, k, run);
-                run += f8_get(c8, k);
-            }
-#pragma unroll
-            for (uint32_t s = 0; s < 19; ++s) {
-                const uint32_t L = (uint32_t)(cl >> (3 * s)) & 7;
-                if (L) {
-                    const uint32_t p = f8_get(o8, L);
-                    f8_add(o8, L, 1);
-                    cs0 |= p < 12 ? (uint64_t)s << (5 * p) : 0;
-                    cs1 |= p < 12 ? 0 : (uint64_t)s << (5 * (p - 12));
+                last = lb_bits(b, 1, err);
+                const uint32_t type = lb_bits(b, 2, err);
+                if (err != ST_OK) {
+                    state = LS_DONE;
+                    continue;
                 }
-            }
-        }
-        if (err != ST_OK) {
-            state = LS_DONE;
-            continue;
-        }
-        // Two passes over the code lengths (from the same input position): the
-        // first counts them per length and checks them, the second places the
-        // symbols in the sorted lists. There is no room in LDS to keep the
-        // lengths between the two.
-        const uint32_t total = nlen + ndist;
-        LaneBits saved;
-        lb_copy(saved, b);
-        F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
-        F8 cd = {0, 0}, od = {0, 0};
-        for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
-            if (pass == 1) {
-                lb_copy(b, saved);
-                uint32_t r0 = 0, r1 = 0;
-#pragma unroll
-                for (uint32_t k = 1; k < 16; ++k) {
-                    f16_add(oll, k, r0);
-                    r0 += f16_get(cll, k);
-                    f8_add(od, k, r1);
-                    r1 += f8_get(cd, k);
-                }
-                for (uint32_t w = 0; w < 9; ++w) hw[w * LANES] = 0;
-            }
-            uint32_t index = 0, prev = 0, len256 = 0;
-            while (index < total) {
-                LB_REFILL_SYNC(b);
-                const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
-                if (err != ST_OK) break;
-                const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
-                uint32_t len = 0, rep = 1;
-                if (sym < 16) {
-                    len = sym;
-                } else if (sym == 16) {
-                    if (index == 0) {
-                        err = ST_BAD_LENGTH_REPEAT;
-                        break;
-                    }
-                    len = prev;
-                    rep = 3 + lb_bits(b, 2, err);
-                } else if (sym == 17) {
-                    rep = 3 + lb_bits(b, 3, err);
-                } else {
-                    rep = 11 + lb_bits(b, 7, err);
-                }
-                if (err != ST_OK) break;
-                if (index + rep > total) {
-                    err = ST_BAD_LENGTH_REPEAT;
-                    break;
-                }
-                const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
-                if (len) {
-                    if (pass == 0) {
-                        f16_add(cll, len, nll);
-                        f8_add(cd, len, rep - nll);
+                if (type == 0) {
+                    // stored: to a byte boundary, then LEN, NLEN and the bytes
+                    const uint32_t skip = (in_len * 8 - b.avail) & 7;
+                    (void)lb_bits(b, skip ? 8 - skip : 0, err);
+                    if (b.avail < 32) {
+                        err = ST_TRUNCATED;
                     } else {
-                        for (uint32_t k = 0; k < rep; ++k) {
-                            const uint32_t sy = index + k;
-                            if (sy < nlen) {
-                                uint32_t q = f16_get(oll, len);
-                                f16_add(oll, len, 1);
-                                q = q < 288 ? q : 287;
-                                LANE_BYTE(lw, q) = (uint8_t)sy;
-                                if (sy >= 256) hw[(q >> 5) * LANES] |= 1u << (q & 31);
-                            } else {
-                                uint32_t q = f8_get(od, len);
-                                f8_add(od, len, 1);
-                                LANE_BYTE(dw, q < 32 ? q : 31) = (uint8_t)(sy - nlen);
+                        LB_REFILL_SYNC(b);
+                        const uint32_t len = lb_bits(b, 16, err);
+                        LB_REFILL_SYNC(b);
+                        const uint32_t nlen = lb_bits(b, 16, err);
+                        if (len != (~nlen & 0xFFFFu))
+                            err = ST_STORED_LEN;
+                        else if (len * 8 > b.avail)
+                            err = ST_TRUNCATED;
+                        else if (pos + len > cap)
+                            err = ST_OUTPUT_OVERFLOW;
+                        else
+                            for (uint32_t i = 0; i < len; ++i) {
+                                LB_REFILL_SYNC(b);
+                                out[pos++] = (uint8_t)lb_bits(b, 8, err);
                             }
+                    }
+                    state = err != ST_OK || last ? LS_DONE : LS_HEADER;
+                    continue;
+                }
+                if (type == 3) {
+                    err = ST_BLOCK_TYPE;
+                    state = LS_DONE;
+                    continue;
+                }
+                if (type == 1) {
+                    // fixed code: sorted lists in closed form
+                    auto fixed_sym = [](uint32_t e) -> uint32_t {
+                        return e < 24 ? 256 + e : e < 168 ? e - 24 : e < 176 ? 280 + e - 168 : 144 + e - 176;
+                    };
+                    for (uint32_t w = 0; w < 72; ++w) {
+                        uint32_t v = 0;
+                        for (uint32_t j = 0; j < 4; ++j) v |= (fixed_sym(4 * w + j) & 255u) << (8 * j);
+                        lw[w * LANES] = v;
+                    }
+                    for (uint32_t w = 0; w < 9; ++w) {
+                        uint32_t v = 0;
+                        for (uint32_t j = 0; j < 32; ++j) v |= (uint32_t)(fixed_sym(32 * w + j) >= 256) << j;
+                        hw[w * LANES] = v;
+                    }
+                    for (uint32_t w = 0; w < 8; ++w) dw[w * LANES] = (4 * w) | (4 * w + 1) << 8 | (4 * w + 2) << 16 | (4 * w + 3) << 24;
+                    (void)lane_code(lc, 288, [](int k) -> uint32_t { return k == 7 ? 24 : k == 8 ? 152 : k == 9 ? 112 : 0; });
+                    (void)lane_code(dc, 32, [](int k) -> uint32_t { return k == 5 ? 32 : 0; });
+                    state = LS_CODES;
+                    continue;
+                }
+                // dynamic header
+                const uint32_t nlen = lb_bits(b, 5, err) + 257;
+                const uint32_t ndist = lb_bits(b, 5, err) + 1;
+                const uint32_t ncode = lb_bits(b, 4, err) + 4;
+                if (err == ST_OK && (nlen > 286 || ndist > 30)) err = ST_BAD_COUNTS;
+                uint64_t cl = 0; // code-length code lengths, 3 bits per symbol 0..18
+        #pragma unroll
+                for (uint32_t i = 0; i < 19; ++i) {
+                    if (err == ST_OK && i < ncode) {
+                        LB_REFILL_SYNC(b);
+                        cl |= (uint64_t)lb_bits(b, 3, err) << (3 * clen_order(i));
+                    }
+                }
+                LaneCode cc;
+                uint64_t cs0 = 0, cs1 = 0; // code-length code symbols sorted, 5 bits each
+                if (err == ST_OK) {
+                    F8 c8 = {0, 0};
+        #pragma unroll
+                    for (uint32_t s = 0; s < 19; ++s) f8_add(c8, (uint32_t)(cl >> (3 * s)) & 7, 1);
+                    const int e = lane_code(cc, 19, [&](int k) -> uint32_t { return k < 8 ? f8_get(c8, (uint32_t)k) : 0; });
+                    if (e != 0) err = ST_BAD_TABLE;
+                    F8 o8 = {0, 0};
+                    uint32_t run = 0;
+        #pragma unroll
+                    for (uint32_t k = 1; k < 8; ++k) {
+                        f8_add(o8, k, run);
+                        run += f8_get(c8, k);
+                    }
+        #pragma unroll
+                    for (uint32_t s = 0; s < 19; ++s) {
+                        const uint32_t L = (uint32_t)(cl >> (3 * s)) & 7;
+                        if (L) {
+                            const uint32_t p = f8_get(o8, L);
+                            f8_add(o8, L, 1);
+                            cs0 |= p < 12 ? (uint64_t)s << (5 * p) : 0;
+                            cs1 |= p < 12 ? 0 : (uint64_t)s << (5 * (p - 12));
                         }
                     }
                 }
-                if (index <= 256 && 256 < index + rep) len256 = len;
-                prev = len;
-                index += rep;
-            }
-            if (pass == 0 && err == ST_OK) {
-                uint32_t sll = 0, sd = 0;
-#pragma unroll
-                for (uint32_t k = 1; k < 16; ++k) {
-                    sll += f16_get(cll, k);
-                    sd += f8_get(cd, k);
+                if (err != ST_OK) {
+                    state = LS_DONE;
+                    continue;
                 }
-                if (len256 == 0) err = ST_BAD_TABLE;
-                int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
-                if (err == ST_OK && e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
-                e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
-                if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
-            }
+                // Two passes over the code lengths (from the same input position): the
+                // first counts them per length and checks them, the second places the
+                // symbols in the sorted lists. There is no room in LDS to keep the
+                // lengths between the two.
+                const uint32_t total = nlen + ndist;
+                LaneBits saved;
+                lb_copy(saved, b);
+                F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
+                F8 cd = {0, 0}, od = {0, 0};
+                for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
+                    if (pass == 1) {
+                        lb_copy(b, saved);
+                        uint32_t r0 = 0, r1 = 0;
+        #pragma unroll
+                        for (uint32_t k = 1; k < 16; ++k) {
+                            f16_add(oll, k, r0);
+                            r0 += f16_get(cll, k);
+                            f8_add(od, k, r1);
+                            r1 += f8_get(cd, k);
+                        }
+                        for (uint32_t w = 0; w < 9; ++w) hw[w * LANES] = 0;
+                    }
+                    uint32_t index = 0, prev = 0, len256 = 0;
+                    while (index < total) {
+                        LB_REFILL_SYNC(b);
+                        const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
+                        if (err != ST_OK) break;
+                        const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
+                        uint32_t len = 0, rep = 1;
+                        if (sym < 16) {
+                            len = sym;
+                        } else if (sym == 16) {
+                            if (index == 0) {
+                                err = ST_BAD_LENGTH_REPEAT;
+                                break;
+                            }
+                            len = prev;
+                            rep = 3 + lb_bits(b, 2, err);
+                        } else if (sym == 17) {
+                            rep = 3 + lb_bits(b, 3, err);
+                        } else {
+                            rep = 11 + lb_bits(b, 7, err);
+                        }
+                        if (err != ST_OK) break;
+                        if (index + rep > total) {
+                            err = ST_BAD_LENGTH_REPEAT;
+                            break;
+                        }
+                        const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
+                        if (len) {
+                            if (pass == 0) {
+                                f16_add(cll, len, nll);
+                                f8_add(cd, len, rep - nll);
+                            } else {
+                                for (uint32_t k = 0; k < rep; ++k) {
+                                    const uint32_t sy = index + k;
+                                    if (sy < nlen) {
+                                        uint32_t q = f16_get(oll, len);
+                                        f16_add(oll, len, 1);
+                                        q = q < 288 ? q : 287;
+                                        LANE_BYTE(lw, q) = (uint8_t)sy;
+                                        if (sy >= 256) hw[(q >> 5) * LANES] |= 1u << (q & 31);
+                                    } else {
+                                        uint32_t q = f8_get(od, len);
+                                        f8_add(od, len, 1);
+                                        LANE_BYTE(dw, q < 32 ? q : 31) = (uint8_t)(sy - nlen);
+                                    }
+                                }
+                            }
+                        }
+                        if (index <= 256 && 256 < index + rep) len256 = len;
+                        prev = len;
+                        index += rep;
+                    }
+                    if (pass == 0 && err == ST_OK) {
+                        uint32_t sll = 0, sd = 0;
+        #pragma unroll
+                        for (uint32_t k = 1; k < 16; ++k) {
+                            sll += f16_get(cll, k);
+                            sd += f8_get(cd, k);
+                        }
+                        if (len256 == 0) err = ST_BAD_TABLE;
+                        int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
+                        if (err == ST_OK && e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
+                        e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
+                        if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
+                    }
+                }
+                state = err == ST_OK ? LS_CODES : LS_DONE;
         }
-        state = err == ST_OK ? LS_CODES : LS_DONE;
+        if (b.k == 8) LB_SWAP(b);
+#pragma unroll
+        for (uint32_t sub = 0; sub < 4; ++sub) {
+            LB_REFILL(b); // harmless for a lane at a header
+            // a symbol needs up to 20 bits, then (a match) a refill and 28;
+            // a lane without them waits for the next pass
+            if (state != LS_CODES || b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
+            // One symbol, and its distance when it is a length, without
+            // branches: every lane runs the same instructions, flags pick
+            // what each result means. Only the stores and errors branch.
+            const uint32_t avail0 = b.avail;
+            const uint32_t lo0 = (uint32_t)b.bb;
+            uint32_t L1;
+            const uint32_t idx1 = lane_find(lc, __builtin_bitreverse32(lo0) >> 17, L1);
+            const bool ok1 = L1 != 0 && L1 <= avail0;
+            const uint32_t e1 = idx1 < 288 ? idx1 : 287;
+            const uint32_t sym = LANE_BYTE(lw, e1) | (((hw[(e1 >> 5) * LANES] >> (e1 & 31)) & 1u) << 8);
+            const uint32_t n1 = ok1 ? L1 : 0;
+            const bool lit = ok1 && sym < 256 && pos < cap;
+            const bool eob = ok1 && sym == 256;
+            const bool lenc = ok1 && sym - 257 < 29;
+            const uint32_t lcode = lenc ? sym - 257 : 0;
+            const uint32_t xb = len_extra(lcode);
+            const bool okx = lenc && n1 + xb <= avail0;
+            const uint32_t mlen = len_base(lcode) + __builtin_amdgcn_ubfe(lo0, n1, xb);
+            const uint32_t c1 = n1 + (okx ? xb : 0);
+            b.bb >>= c1;
+            b.bc -= c1;
+            b.avail = avail0 - c1;
+            LB_REFILL(b);
+            const uint32_t avail2 = b.avail;
+            const uint32_t lo2 = (uint32_t)b.bb;
+            uint32_t L2;
+            const uint32_t idx2 = lane_find(dc, __builtin_bitreverse32(lo2) >> 17, L2);
+            const bool ok2 = okx && L2 != 0 && L2 <= avail2;
+            const uint32_t ds = LANE_BYTE(dw, idx2 < 32 ? idx2 : 31);
+            const bool okd = ok2 && ds < 30;
+            const uint32_t dcode = okd ? ds : 0;
+            const uint32_t xd = dist_extra(dcode);
+            const bool okxd = okd && L2 + xd <= avail2;
+            const uint32_t dist = dist_base(dcode) + __builtin_amdgcn_ubfe(lo2, L2, xd);
+            const bool match = okxd && dist <= pos && pos + mlen <= cap;
+            const uint32_t c2 = match ? L2 + xd : 0;
+            b.bb >>= c2;
+            b.bc -= c2;
+            b.avail = avail2 - c2;
+            if (!(lit || eob || match)) {
+                // the first check that failed, in puff's order
+                if (!ok1)
+                    err = L1 == 0 && avail0 >= 15 ? ST_BAD_CODE : ST_TRUNCATED;
+                else if (sym < 256)
+                    err = ST_OUTPUT_OVERFLOW;
+                else if (!lenc)
+                    err = ST_BAD_CODE;
+                else if (!okx)
+                    err = ST_TRUNCATED;
+                else if (!ok2)
+                    err = L2 == 0 && avail2 >= 15 ? ST_BAD_DISTANCE : ST_TRUNCATED;
+                else if (!okd)
+                    err = ST_BAD_DISTANCE;
+                else if (!okxd)
+                    err = ST_TRUNCATED;
+                else if (dist > pos)
+                    err = ST_BAD_DISTANCE;
+                else
+                    err = ST_OUTPUT_OVERFLOW;
+                state = LS_DONE;
+            }
+            if (lit) out[pos] = (uint8_t)sym;
+            if (match) {
+                uint32_t gap = pos - mark;
+                while (gap >= REC_SKIP) {
+                    rec[nrec++] = REC_SKIP << 23;
+                    gap -= REC_SKIP;
+                }
+                rec[nrec++] = (gap << 23) | ((mlen - 3) << 15) | (dist - 1);
+            }
+            pos += lit ? 1 : match ? mlen : 0;
+            mark = match ? pos : mark;
+            if (eob) state = last ? LS_DONE : LS_HEADER;
+        }
     }
     a.status[m] = err;
     a.produced[m] = pos;
