@@ -1420,13 +1420,62 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     }
     LaneCode lc, dc;
     uint32_t state = LS_HEADER, last = 0, err = ST_OK, pos = 0, nrec = 0, mark = 0;
+
+    // Output stores are held in registers and issued together in the next
+    // pass, right after B's loads: a wait for B (one pass later) then finds
+    // only stores as old as those loads in flight (the memory counter covers
+    // stores too, in issue order).
+    // Literals are stored as whole 4-byte words of the output ("windows",
+    // numbered from the aligned word at or below the member's first byte):
+    // the bytes of a word that are not literals belong to matches, which the
+    // resolve phase overwrites. Bytes outside [lo, cap) of the member (lo:
+    // the member start, or the end of its last stored block, whose bytes are
+    // written directly) are never written this way.
+    constexpr uint32_t NONE = ~0u;
+    const uint32_t h = (uint32_t)(out_off & 3);
+    uint8_t *const out_w = out - h; // word-aligned: byte r of the windows is out_w[r]
+    uint32_t lo = h;                 // first window byte a word store may cover
+    uint32_t wcur = NONE, wv = 0;    // the open window and its bytes
+    uint32_t ls_i[4], ls_v[4];       // windows closed in step j of the pass
+    uint32_t rs_i[4], rs_v[4];       // record written in step j of the pass
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+        ls_i[j] = NONE;
+        ls_v[j] = 0;
+        rs_i[j] = NONE;
+        rs_v[j] = 0;
+    }
+    // Window bytes r in [lo, min(end, h + cap)), as one word when all four are.
+    auto put_window = [&](uint32_t wi, uint32_t v, uint32_t end) __attribute__((always_inline)) {
+        const uint32_t r0 = 4 * wi;
+        if (r0 >= lo && r0 + 4 <= end && r0 - h + 4 <= cap) {
+            *reinterpret_cast<uint32_t *>(out_w + r0) = v;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                if (r0 + j >= lo && r0 + j < end && r0 + j - h < cap) out_w[r0 + j] = (uint8_t)(v >> (8 * j));
+        }
+    };
+    auto flush_pass = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if (ls_i[j] != NONE) put_window(ls_i[j], ls_v[j], NONE);
+            if (rs_i[j] != NONE) rec[rs_i[j]] = rs_v[j];
+            ls_i[j] = NONE;
+            rs_i[j] = NONE;
+        }
+    };
     // the member's parameters have landed: otherwise their first use inside
     // the loop would leave a wait on the memory counter in every step
     __builtin_amdgcn_s_waitcnt(0);
-    // Each pass of the loop: the block header for lanes at one, then four
-    // decode steps. B moves into A once per pass (the same point in every
-    // lane), so no decode step waits for memory.
+    // Each pass of the loop: B moves into A where A is used up (the same
+    // point in every lane, so no decode step waits for memory), the stores of
+    // the last pass, the block header for lanes at one, then four decode
+    // steps.
     while (state != LS_DONE) {
+        if (b.k == 8) LB_SWAP(b);
+        asm volatile("" ::: "memory"); // the stores stay after the loads
+        flush_pass();
         if (state == LS_HEADER) {
                 // block header
                 LB_REFILL_SYNC(b);
@@ -1453,11 +1502,18 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                             err = ST_TRUNCATED;
                         else if (pos + len > cap)
                             err = ST_OUTPUT_OVERFLOW;
-                        else
+                        else {
+                            // the open window's bytes before pos, then the
+                            // stored bytes; later windows start after them
+                            if (wcur != NONE) put_window(wcur, wv, pos + h);
+                            wcur = NONE;
+                            wv = 0;
                             for (uint32_t i = 0; i < len; ++i) {
                                 LB_REFILL_SYNC(b);
                                 out[pos++] = (uint8_t)lb_bits(b, 8, err);
                             }
+                            lo = pos + h;
+                        }
                     }
                     state = err != ST_OK || last ? LS_DONE : LS_HEADER;
                     continue;
@@ -1621,7 +1677,6 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                 }
                 state = err == ST_OK ? LS_CODES : LS_DONE;
         }
-        if (b.k == 8) LB_SWAP(b);
 #pragma unroll
         for (uint32_t sub = 0; sub < 4; ++sub) {
             LB_REFILL(b); // harmless for a lane at a header
@@ -1689,20 +1744,32 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                     err = ST_OUTPUT_OVERFLOW;
                 state = LS_DONE;
             }
-            if (lit) out[pos] = (uint8_t)sym;
+            {
+                // a literal in a new window closes the open one
+                const uint32_t rel = pos + h;
+                const bool close = lit && (rel >> 2) != wcur;
+                ls_i[sub] = close ? wcur : NONE;
+                ls_v[sub] = wv;
+                wv = close ? 0u : wv;
+                wcur = lit ? rel >> 2 : wcur;
+                wv |= lit ? sym << (8 * (rel & 3)) : 0u;
+            }
             if (match) {
                 uint32_t gap = pos - mark;
                 while (gap >= REC_SKIP) {
                     rec[nrec++] = REC_SKIP << 23;
                     gap -= REC_SKIP;
                 }
-                rec[nrec++] = (gap << 23) | ((mlen - 3) << 15) | (dist - 1);
+                rs_i[sub] = nrec++;
+                rs_v[sub] = (gap << 23) | ((mlen - 3) << 15) | (dist - 1);
             }
             pos += lit ? 1 : match ? mlen : 0;
             mark = match ? pos : mark;
             if (eob) state = last ? LS_DONE : LS_HEADER;
         }
     }
+    flush_pass();
+    if (wcur != NONE) put_window(wcur, wv, NONE);
     a.status[m] = err;
     a.produced[m] = pos;
     a.crc[m] = 0;
