@@ -238,10 +238,29 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
         }
         return;
     }
-    // exact walk from the agreed landing to the member end
-    uint64_t cur = L;
+    // the exact walk from the agreed landing is bam_walk_kernel's
+    if (lane == 0) {
+        a.spec_ok[b] = 1;
+        a.n_surv[b] = (uint8_t)(n_surv < BAM_K ? n_surv : BAM_K);
+        a.land[b] = L;
+        a.complete[b] = (uint8_t)Lcomplete;
+    }
+}
+
+// The exact walk of every member whose speculation agreed (spec_ok 1 from
+// bam_spec_kernel), from its landing to its end, one lane per member: the
+// walk is a chain of dependent loads, and a wave keeps 64 of them in flight.
+template <uint32_t F>
+__global__ __launch_bounds__(256) void bam_walk_kernel(BamSpecArgs a)
+{
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= a.nb || a.spec_ok[b] != 1) return;
+    const uint64_t sb = a.block_starts[b];
+    const uint64_t sb1 = a.mem_until ? a.mem_until[b] : a.block_starts[b + 1];
+    const uint64_t len = a.mem_len ? a.mem_len[b] : a.len;
+    uint64_t cur = a.land[b];
     uint32_t cnt = 0;
-    int complete = Lcomplete, valid = 1;
+    int complete = a.complete[b], valid = 1;
     uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
     if (complete) {
         for (;;) {
@@ -249,19 +268,15 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
             if (s.kind == 1) break;
             if (s.kind == 2) { complete = 0; break; }
             if (s.kind == 3) { valid = 0; break; }
-            if (lane == 0) rel[cnt] = (uint16_t)(cur - sb);
+            rel[cnt] = (uint16_t)(cur - sb);
             ++cnt;
             cur = s.next;
         }
     }
-    if (lane == 0) {
-        a.spec_ok[b] = (uint8_t)valid;
-        a.n_surv[b] = (uint8_t)(n_surv < BAM_K ? n_surv : BAM_K);
-        a.land[b] = L;
-        a.exitp[b] = cur;
-        a.cnt_tail[b] = cnt;
-        a.complete[b] = (uint8_t)complete;
-    }
+    a.spec_ok[b] = (uint8_t)valid;
+    a.exitp[b] = cur;
+    a.cnt_tail[b] = cnt;
+    a.complete[b] = (uint8_t)complete;
 }
 
 // Lane j of the result takes v, the other lanes keep old (v is wave-uniform).
@@ -663,10 +678,14 @@ hipError_t launch_bcf_decode(const BcfDecodeArgs &a, hipStream_t s)
 hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 {
     if (a.nb == 0) return hipSuccess;
-    if (a.format == CHAIN_BCF)
+    const uint32_t wgrid = (a.nb + 255) / 256;
+    if (a.format == CHAIN_BCF) {
         hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BCF>, dim3(a.nb), dim3(64), 0, s, a);
-    else
+        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BCF>, dim3(wgrid), dim3(256), 0, s, a);
+    } else {
         hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BAM>, dim3(a.nb), dim3(64), 0, s, a);
+        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BAM>, dim3(wgrid), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
