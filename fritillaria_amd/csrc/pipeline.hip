@@ -25,10 +25,28 @@ __device__ __forceinline__ uint32_t rd32(const uint8_t *p)
 }
 
 // block.rs:99-164 on the bytes at p; returns block size (0 if not a header).
+// The usual layout (one extra subfield, BC, so an 18-byte header) is checked
+// from 18 bytes loaded together; any other goes through the general walk.
 __device__ __forceinline__ uint32_t header_at(const uint8_t *f, uint64_t len, uint64_t p, uint32_t *hsize)
 {
     if (p + 12 > len) return 0;
     const uint8_t *b = f + p;
+    if (p + 18 <= len) {
+        uint32_t h[18];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) h[i] = b[i];
+        // all 18 loads issue before the first check (one wait, not one per byte)
+        asm volatile("" ::"v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(h[3]), "v"(h[4]), "v"(h[5]), "v"(h[6]), "v"(h[7]),
+                     "v"(h[8]), "v"(h[9]), "v"(h[10]), "v"(h[11]), "v"(h[12]), "v"(h[13]), "v"(h[14]), "v"(h[15]),
+                     "v"(h[16]), "v"(h[17]));
+        if (h[0] != 0x1f || h[1] != 0x8b || h[2] != 8 || (h[3] & 4) == 0) return 0;
+        if ((h[10] | h[11] << 8) == 6 && h[12] == 'B' && h[13] == 'C' && (h[14] | h[15] << 8) == 2) {
+            const uint32_t bs = (h[16] | h[17] << 8) + 1;
+            if (bs < 18 + 8 || p + bs > len) return 0;
+            *hsize = 18;
+            return bs;
+        }
+    }
     if (b[0] != 0x1f || b[1] != 0x8b || b[2] != 8 || (b[3] & 4) == 0) return 0;
     uint32_t xlen = rd16(b + 10);
     uint64_t hs = 12 + (uint64_t)xlen;
@@ -82,6 +100,26 @@ __device__ __forceinline__ uint32_t magic_mask(const uint8_t *f, uint64_t len, u
     return m;
 }
 
+// magic_mask for a thread's NK chunks of the tile at base (chunk i at
+// base + 16 * thread + 4096 * i): for a tile wholly inside the file, the NK
+// loads are all in flight before the first is used.
+template <uint32_t NK>
+__device__ __forceinline__ void tile_masks(const uint8_t *f, uint64_t len, uint64_t base, uint32_t *mk)
+{
+    const uint64_t p0 = base + threadIdx.x * SIEVE_CHUNK;
+    if (base + SIEVE_TILE <= len) {
+        uint4 v[NK];
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i) v[i] = *reinterpret_cast<const uint4 *>(f + p0 + i * SIEVE_THREADS * SIEVE_CHUNK);
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i)
+            mk[i] = nibble_1f(v[i].x) | (nibble_1f(v[i].y) << 4) | (nibble_1f(v[i].z) << 8) | (nibble_1f(v[i].w) << 12);
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i) mk[i] = magic_mask(f, len, p0 + i * SIEVE_THREADS * SIEVE_CHUNK);
+    }
+}
+
 // Pass 1: candidates per tile.
 __global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_t *f, uint64_t len, uint32_t *counts,
                                                                     uint64_t n_tiles)
@@ -90,9 +128,13 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_count_kernel(const uint8_
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const uint64_t base = t * SIEVE_TILE;
         uint32_t c = 0;
-        for (uint32_t k = threadIdx.x * SIEVE_CHUNK; k < SIEVE_TILE; k += SIEVE_THREADS * SIEVE_CHUNK) {
-            const uint64_t p = base + k;
-            uint32_t m = magic_mask(f, len, p);
+        constexpr uint32_t NK = SIEVE_TILE / (SIEVE_THREADS * SIEVE_CHUNK);
+        uint32_t mk[NK];
+        tile_masks<NK>(f, len, base, mk);
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i) {
+            const uint64_t p = base + threadIdx.x * SIEVE_CHUNK + i * SIEVE_THREADS * SIEVE_CHUNK;
+            uint32_t m = mk[i];
             while (m) {
                 const uint32_t j = (uint32_t)__ffs(m) - 1;
                 m &= m - 1;
@@ -125,9 +167,13 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const uint64_t base = t * SIEVE_TILE;
         uint64_t out = tile_base[t];
-        for (uint32_t k = 0; k < SIEVE_TILE; k += SIEVE_THREADS * SIEVE_CHUNK) {
-            const uint64_t p = base + k + threadIdx.x * SIEVE_CHUNK;
-            const uint32_t mm = magic_mask(f, len, p);
+        constexpr uint32_t NK = SIEVE_TILE / (SIEVE_THREADS * SIEVE_CHUNK);
+        uint32_t mk[NK];
+        tile_masks<NK>(f, len, base, mk);
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i) {
+            const uint64_t p = base + i * SIEVE_THREADS * SIEVE_CHUNK + threadIdx.x * SIEVE_CHUNK;
+            const uint32_t mm = mk[i];
             uint32_t m = mm, c = 0;
             while (m) {
                 const uint32_t j = (uint32_t)__ffs(m) - 1;
@@ -149,7 +195,7 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_
                 if (v < w) before += wcnt[v];
                 total += wcnt[v];
             }
-            uint64_t i = out + before + inc - c;
+            uint64_t at = out + before + inc - c;
             m = mm;
             while (m) {
                 const uint32_t j = (uint32_t)__ffs(m) - 1;
@@ -157,12 +203,12 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_
                 uint32_t hs = 0;
                 const uint32_t bs = header_at(f, len, p + j, &hs);
                 if (bs) {
-                    if (i < cap) {
-                        cand[i] = p + j;
-                        cand_bs[i] = bs;
-                        cand_hs[i] = hs;
+                    if (at < cap) {
+                        cand[at] = p + j;
+                        cand_bs[at] = bs;
+                        cand_hs[at] = hs;
                     }
-                    ++i;
+                    ++at;
                 }
             }
             out += total;
