@@ -162,58 +162,66 @@ __global__ __launch_bounds__(SIEVE_THREADS) void sieve_write_kernel(const uint8_
                                                                     uint64_t *cand, uint32_t *cand_bs,
                                                                     uint32_t *cand_hs, uint64_t cap)
 {
-    __shared__ uint32_t wcnt[SIEVE_THREADS / 64];
+    constexpr uint32_t NK = SIEVE_TILE / (SIEVE_THREADS * SIEVE_CHUNK);
+    constexpr uint32_t NW = SIEVE_THREADS / 64;
+    __shared__ uint32_t wcnt[NK][NW];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const uint64_t base = t * SIEVE_TILE;
-        uint64_t out = tile_base[t];
-        constexpr uint32_t NK = SIEVE_TILE / (SIEVE_THREADS * SIEVE_CHUNK);
+        const uint64_t out = tile_base[t];
         uint32_t mk[NK];
         tile_masks<NK>(f, len, base, mk);
+        // the headers among each chunk's candidates, and their counts scanned
+        // over the wave (file order: chunk, then thread)
+        uint32_t vm[NK], c[NK], inc[NK];
 #pragma unroll
         for (uint32_t i = 0; i < NK; ++i) {
             const uint64_t p = base + i * SIEVE_THREADS * SIEVE_CHUNK + threadIdx.x * SIEVE_CHUNK;
-            const uint32_t mm = mk[i];
-            uint32_t m = mm, c = 0;
+            uint32_t m = mk[i];
+            vm[i] = 0;
             while (m) {
                 const uint32_t j = (uint32_t)__ffs(m) - 1;
                 m &= m - 1;
                 uint32_t hs;
-                if (header_at(f, len, p + j, &hs)) ++c;
+                if (header_at(f, len, p + j, &hs)) vm[i] |= 1u << j;
             }
-            // exclusive prefix of c over the block, in thread order
-            uint32_t inc = c;
+            c[i] = (uint32_t)__popc(vm[i]);
+            inc[i] = c[i];
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t y = __shfl_up(inc, d, 64);
-                if ((int)lane >= d) inc += y;
+                const uint32_t y = __shfl_up(inc[i], d, 64);
+                if ((int)lane >= d) inc[i] += y;
             }
-            if (lane == 63) wcnt[w] = inc;
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-            for (uint32_t v = 0; v < SIEVE_THREADS / 64; ++v) {
-                if (v < w) before += wcnt[v];
-                total += wcnt[v];
+            if (lane == 63) wcnt[i][w] = inc[i];
+        }
+        __syncthreads();
+        uint64_t before = out;
+#pragma unroll
+        for (uint32_t i = 0; i < NK; ++i) {
+            uint32_t wb = 0, wt = 0;
+#pragma unroll
+            for (uint32_t v = 0; v < NW; ++v) {
+                wb += v < w ? wcnt[i][v] : 0u;
+                wt += wcnt[i][v];
             }
-            uint64_t at = out + before + inc - c;
-            m = mm;
+            const uint64_t p = base + i * SIEVE_THREADS * SIEVE_CHUNK + threadIdx.x * SIEVE_CHUNK;
+            uint64_t at = before + wb + inc[i] - c[i];
+            uint32_t m = vm[i];
             while (m) {
                 const uint32_t j = (uint32_t)__ffs(m) - 1;
                 m &= m - 1;
                 uint32_t hs = 0;
                 const uint32_t bs = header_at(f, len, p + j, &hs);
-                if (bs) {
-                    if (at < cap) {
-                        cand[at] = p + j;
-                        cand_bs[at] = bs;
-                        cand_hs[at] = hs;
-                    }
-                    ++at;
+                if (at < cap) {
+                    cand[at] = p + j;
+                    cand_bs[at] = bs;
+                    cand_hs[at] = hs;
                 }
+                ++at;
             }
-            out += total;
-            __syncthreads();
+            before += wt;
         }
+        __syncthreads();
     }
 }
 
