@@ -76,23 +76,50 @@ __device__ __forceinline__ Step cand_step(const uint8_t *buf, uint64_t len, uint
 {
     if (cur >= until) return {cur, 1};
     if (cur + 36 > len) return {cur, 3};
-    uint32_t bs = ld32(buf + cur);
+    // the 36 fixed bytes, loaded together (every check fails alike, so none
+    // needs the others' loads to wait)
+    const uint8_t *r = buf + cur;
+    uint32_t h[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) h[i] = r[i];
+    asm volatile("" ::"v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(h[3]), "v"(h[4]), "v"(h[5]), "v"(h[6]), "v"(h[7]),
+                 "v"(h[8]), "v"(h[9]), "v"(h[10]), "v"(h[11]), "v"(h[12]), "v"(h[13]), "v"(h[14]), "v"(h[15]),
+                 "v"(h[16]), "v"(h[17]), "v"(h[18]), "v"(h[19]), "v"(h[20]), "v"(h[21]), "v"(h[22]), "v"(h[23]));
+    asm volatile("" ::"v"(h[24]), "v"(h[25]), "v"(h[26]), "v"(h[27]), "v"(h[28]), "v"(h[29]), "v"(h[30]),
+                 "v"(h[31]), "v"(h[32]), "v"(h[33]), "v"(h[34]), "v"(h[35]));
+    auto w32 = [&](int o) { return h[o] | h[o + 1] << 8 | h[o + 2] << 16 | h[o + 3] << 24; };
+    const uint32_t bs = w32(0);
     if (bs < 32) return {cur, 3};
-    uint64_t end = cur + 4 + (uint64_t)bs;
+    const uint64_t end = cur + 4 + (uint64_t)bs;
     if (end > len) return {cur, 3};
-    uint32_t name_len = buf[cur + 12];
-    uint64_t n_cigar = ld16(buf + cur + 16);
-    uint64_t l_seq = ld32(buf + cur + 20);
-    uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
+    const uint32_t name_len = h[12];
+    const uint64_t n_cigar = h[16] | h[17] << 8;
+    const uint64_t l_seq = w32(20);
+    const uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
     if (name_len == 0 || cur + need > end) return {cur, 3};
-    if ((int32_t)ld32(buf + cur + 4) < -1 || (int32_t)ld32(buf + cur + 8) < -1 ||
-        (int32_t)ld32(buf + cur + 24) < -1 || (int32_t)ld32(buf + cur + 28) < -1)
+    if ((int32_t)w32(4) < -1 || (int32_t)w32(8) < -1 || (int32_t)w32(24) < -1 || (int32_t)w32(28) < -1)
         return {cur, 3};
-    // a QNAME is at least one printable character, then the NUL
-    if (name_len < 2 || buf[cur + 35 + name_len] != 0) return {cur, 3};
-    for (uint32_t i = 0; i + 1 < name_len; ++i) {
-        uint8_t c = buf[cur + 36 + i];
-        if (c < 33 || c > 126) return {cur, 3};
+    // a QNAME is at least one printable character, then the NUL (all inside
+    // the record, so inside the buffer)
+    if (name_len < 2 || r[35 + name_len] != 0) return {cur, 3};
+    const uint32_t n = name_len - 1;
+    for (uint32_t k = 0; k < n; k += 16) {
+        bool bad = false;
+        if (cur + 36 + k + 16 <= len) {
+            uint32_t c[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) c[i] = r[36 + k + i];
+            asm volatile("" ::"v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]), "v"(c[4]), "v"(c[5]), "v"(c[6]), "v"(c[7]),
+                         "v"(c[8]), "v"(c[9]), "v"(c[10]), "v"(c[11]), "v"(c[12]), "v"(c[13]), "v"(c[14]), "v"(c[15]));
+#pragma unroll
+            for (uint32_t i = 0; i < 16; ++i) bad |= k + i < n && (c[i] < 33 || c[i] > 126);
+        } else {
+            for (uint32_t i = k; i < n && i < k + 16; ++i) {
+                const uint32_t ch = r[36 + i];
+                bad |= ch < 33 || ch > 126;
+            }
+        }
+        if (bad) return {cur, 3};
     }
     return {end, 0};
 }
