@@ -1832,26 +1832,36 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
 }
 
 // Write-back of exactly FAST_CHUNK bytes from a 16-byte aligned output
-// position: lane l moves aligned block l to HBM and takes its CRC with the
-// slice-by-16 tables (t16[k][b] = CRC of byte b followed by k zero bytes, all
-// 16 lookups independent); the blocks fold with x^(8*16*(63-l)) as in flush().
-constexpr uint32_t FAST_CHUNK = 1024;
-template <uint32_t RING>
-__device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *t16, Out &o, uint32_t lane,
-                                           uint32_t lane_factor16, uint32_t x_chunk)
+// position: lane l moves the 32 bytes at 32 l to HBM and takes their CRC with
+// the slice-by-16 tables (t16[k][b] = CRC of byte b followed by k zero bytes,
+// all 16 lookups of a block independent; the second block takes the first's
+// CRC into its first four bytes); the lanes fold with x^(8*32*(63-l)) as in
+// flush().
+constexpr uint32_t FAST_CHUNK = 2048;
+__device__ __forceinline__ uint32_t slice16(const uint32_t *t16, uint4 v)
 {
-    using G = Geometry<RING>;
-    WFENCE();
-    const uint32_t p = o.flushed + 16 * lane;
-    const uint4 v = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p) & G::RMASK));
-    *reinterpret_cast<uint4 *>(o.g + p) = v;
     uint32_t r = 0;
 #pragma unroll
     for (uint32_t j = 0; j < 16; ++j) {
         const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
         r ^= t16[(15 - j) * 256 + ((w >> (8 * (j & 3))) & 0xFF)];
     }
-    r = fr_mulmod(r, lane_factor16);
+    return r;
+}
+template <uint32_t RING>
+__device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *t16, Out &o, uint32_t lane,
+                                           uint32_t lane_factor32, uint32_t x_chunk)
+{
+    using G = Geometry<RING>;
+    WFENCE();
+    const uint32_t p = o.flushed + 32 * lane;
+    const uint4 v0 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p) & G::RMASK));
+    uint4 v1 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p + 16) & G::RMASK));
+    *reinterpret_cast<uint4 *>(o.g + p) = v0;
+    *reinterpret_cast<uint4 *>(o.g + p + 16) = v1;
+    const uint32_t r0 = slice16(t16, v0);
+    v1.x ^= r0;
+    uint32_t r = fr_mulmod(slice16(t16, v1), lane_factor32);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) r ^= __shfl_xor(r, off, 64);
     o.raw = fr_mulmod(o.raw, x_chunk) ^ uni(r);
@@ -1871,6 +1881,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     static_assert(SPAN >= REC_SKIP + 258, "window too small for a record");
     static_assert(SPAN + WIN <= RING / 2, "window writes must stay clear of unflushed bytes");
     static_assert(FAST_CHUNK + SPAN + WIN <= RING, "fast write-back must stay clear of the window writes");
+    static_assert(SPAN <= 64 * G::SEG && FAST_CHUNK <= 64 * G::SEG,
+                  "the last write-back (under FAST_CHUNK bytes after a fast one) must fit flush()");
     __shared__ uint32_t t16[16 * 256]; // t16[0] is the byte-at-a-time table
     __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
     __shared__ uint4 stage_all[NW][64][2]; // per lane: the 32 aligned bytes around a far match source
@@ -1891,7 +1903,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     const uint32_t *crctab = t16;
     uint8_t *ring = rings[wave];
     const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)G::SEG * (63 - lane));
-    const uint32_t lane_factor16 = fr_xpow8(a.x2n, (uint64_t)16 * (63 - lane));
+    const uint32_t lane_factor32 = fr_xpow8(a.x2n, (uint64_t)32 * (63 - lane));
     const uint32_t x_chunk = fr_xpow8(a.x2n, FAST_CHUNK);
     uint32_t iter = 0;
     for (;;) {
@@ -1927,17 +1939,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             ri = uni(ri);
             o.pos = uni(o.pos);
             o.flushed = uni(o.flushed);
-            // write-back: once up to a 16-byte aligned output address, then
-            // in aligned FAST_CHUNK pieces
+            // write-back: the bytes up to the first 16-byte aligned output
+            // address, then aligned FAST_CHUNK pieces (fewer unflushed bytes
+            // than flush() can carry remain at the member's end)
             if (((uintptr_t)(o.g + o.flushed) & 15) != 0) {
-                if (o.pos - o.flushed >= FAST_CHUNK + 16) {
+                if (o.pos - o.flushed >= 16) {
                     const uint32_t keep = o.pos;
-                    o.pos -= (uint32_t)((uintptr_t)(o.g + o.pos) & 15);
+                    o.pos = o.flushed + 16 - (uint32_t)((uintptr_t)(o.g + o.flushed) & 15);
                     o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
                     o.pos = keep;
                 }
             } else if (o.pos - o.flushed >= FAST_CHUNK) {
-                flush_fast<RING>(ring, t16, o, lane, lane_factor16, x_chunk);
+                flush_fast<RING>(ring, t16, o, lane, lane_factor32, x_chunk);
             }
             const uint32_t R0 = o.pos;
             const bool live = ri + lane < nr;
@@ -2058,6 +2071,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             if (lane == 0) a.status[m] = ST_OUTPUT_OVERFLOW;
             continue;
         }
+        if (((uintptr_t)(o.g + o.flushed) & 15) == 0 && o.pos - o.flushed >= FAST_CHUNK)
+            flush_fast<RING>(ring, t16, o, lane, lane_factor32, x_chunk);
         o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
         if (lane == 0) a.crc[m] = o.raw ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(a.x2n, o.pos)) ^ 0xFFFFFFFFu;
     }
