@@ -1833,38 +1833,37 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
 
 // Write-back of exactly FAST_CHUNK bytes from a 16-byte aligned output
 // position: lane l moves the 32 bytes at 32 l to HBM and takes their CRC with
-// the slice-by-16 tables (t16[k][b] = CRC of byte b followed by k zero bytes,
-// all 16 lookups of a block independent; the second block takes the first's
-// CRC into its first four bytes); the lanes fold with x^(8*32*(63-l)) as in
-// flush().
+// the slice-by-8 tables (t8[k][b] = CRC of byte b followed by k zero bytes;
+// each 8-byte step takes the CRC so far into its first four bytes). Each lane
+// folds its pieces across write-backs by Horner's rule, acc = acc * X ^ crc
+// with X = x^(8 FAST_CHUNK), the product by X read from tables (tx[k][b] =
+// (b at byte k) * X); the lanes' sums fold with x^(8*32*(63-l)) once, at the
+// member's end.
 constexpr uint32_t FAST_CHUNK = 2048;
-__device__ __forceinline__ uint32_t slice16(const uint32_t *t16, uint4 v)
+__device__ __forceinline__ uint32_t slice8(const uint32_t *t8, uint32_t r, uint32_t lo, uint32_t hi)
 {
-    uint32_t r = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < 16; ++j) {
-        const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
-        r ^= t16[(15 - j) * 256 + ((w >> (8 * (j & 3))) & 0xFF)];
-    }
-    return r;
+    lo ^= r;
+    return t8[7 * 256 + (lo & 0xFF)] ^ t8[6 * 256 + ((lo >> 8) & 0xFF)] ^ t8[5 * 256 + ((lo >> 16) & 0xFF)] ^
+           t8[4 * 256 + (lo >> 24)] ^ t8[3 * 256 + (hi & 0xFF)] ^ t8[2 * 256 + ((hi >> 8) & 0xFF)] ^
+           t8[1 * 256 + ((hi >> 16) & 0xFF)] ^ t8[hi >> 24];
 }
 template <uint32_t RING>
-__device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *t16, Out &o, uint32_t lane,
-                                           uint32_t lane_factor32, uint32_t x_chunk)
+__device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *t8, const uint32_t *tx, Out &o,
+                                           uint32_t lane, uint32_t &acc, uint32_t &nfast)
 {
     using G = Geometry<RING>;
     WFENCE();
     const uint32_t p = o.flushed + 32 * lane;
     const uint4 v0 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p) & G::RMASK));
-    uint4 v1 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p + 16) & G::RMASK));
+    const uint4 v1 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p + 16) & G::RMASK));
     *reinterpret_cast<uint4 *>(o.g + p) = v0;
     *reinterpret_cast<uint4 *>(o.g + p + 16) = v1;
-    const uint32_t r0 = slice16(t16, v0);
-    v1.x ^= r0;
-    uint32_t r = fr_mulmod(slice16(t16, v1), lane_factor32);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) r ^= __shfl_xor(r, off, 64);
-    o.raw = fr_mulmod(o.raw, x_chunk) ^ uni(r);
+    uint32_t r = slice8(t8, 0, v0.x, v0.y);
+    r = slice8(t8, r, v0.z, v0.w);
+    r = slice8(t8, r, v1.x, v1.y);
+    r = slice8(t8, r, v1.z, v1.w);
+    acc = tx[acc & 0xFF] ^ tx[256 + ((acc >> 8) & 0xFF)] ^ tx[512 + ((acc >> 16) & 0xFF)] ^ tx[768 + (acc >> 24)] ^ r;
+    ++nfast;
     o.flushed += FAST_CHUNK;
 }
 
@@ -1883,28 +1882,30 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     static_assert(FAST_CHUNK + SPAN + WIN <= RING, "fast write-back must stay clear of the window writes");
     static_assert(SPAN <= 64 * G::SEG && FAST_CHUNK <= 64 * G::SEG,
                   "the last write-back (under FAST_CHUNK bytes after a fast one) must fit flush()");
-    __shared__ uint32_t t16[16 * 256]; // t16[0] is the byte-at-a-time table
+    __shared__ uint32_t t8[8 * 256]; // t8[0] is the byte-at-a-time table
+    __shared__ uint32_t tx[4 * 256]; // products by x^(8 FAST_CHUNK), per register byte
     __shared__ __attribute__((aligned(16))) uint8_t rings[NW][RING];
     __shared__ uint4 stage_all[NW][64][2]; // per lane: the 32 aligned bytes around a far match source
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
         uint32_t c = i;
         for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (FR_CRC_POLY & (0u - (c & 1u)));
-        t16[i] = c;
+        t8[i] = c;
     }
+    const uint32_t x_chunk = fr_xpow8(a.x2n, FAST_CHUNK);
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += 64 * NW) tx[i] = fr_mulmod((i & 255u) << (8 * (i >> 8)), x_chunk);
     __syncthreads();
-    for (uint32_t k = 1; k < 16; ++k) {
+    for (uint32_t k = 1; k < 8; ++k) {
         for (uint32_t i = threadIdx.x; i < 256; i += 64 * NW) {
-            const uint32_t c = t16[(k - 1) * 256 + i];
-            t16[k * 256 + i] = (c >> 8) ^ t16[c & 0xFF];
+            const uint32_t c = t8[(k - 1) * 256 + i];
+            t8[k * 256 + i] = (c >> 8) ^ t8[c & 0xFF];
         }
         __syncthreads();
     }
-    const uint32_t *crctab = t16;
+    const uint32_t *crctab = t8;
     uint8_t *ring = rings[wave];
     const uint32_t lane_factor = fr_xpow8(a.x2n, (uint64_t)G::SEG * (63 - lane));
     const uint32_t lane_factor32 = fr_xpow8(a.x2n, (uint64_t)32 * (63 - lane));
-    const uint32_t x_chunk = fr_xpow8(a.x2n, FAST_CHUNK);
     uint32_t iter = 0;
     for (;;) {
         uint32_t m;
@@ -1933,6 +1934,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         o.skew = (uint32_t)((uintptr_t)o.g & 16383);
         uint32_t ri = 0;
         bool bad = false;
+        uint32_t acc = 0, nfast = 0; // this lane's fast write-back CRCs, and their count
         // One round: records `rc` (from ri) and window `wc` (from o.pos) are
         // in hand; those of the next round go to rn, wn.
         auto round = [&](const uint32_t rc, const uint4 wc, uint32_t &rn, uint4 &wn) __attribute__((always_inline)) {
@@ -1950,7 +1952,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                     o.pos = keep;
                 }
             } else if (o.pos - o.flushed >= FAST_CHUNK) {
-                flush_fast<RING>(ring, t16, o, lane, lane_factor32, x_chunk);
+                flush_fast<RING>(ring, t8, tx, o, lane, acc, nfast);
             }
             const uint32_t R0 = o.pos;
             const bool live = ri + lane < nr;
@@ -2072,7 +2074,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             continue;
         }
         if (((uintptr_t)(o.g + o.flushed) & 15) == 0 && o.pos - o.flushed >= FAST_CHUNK)
-            flush_fast<RING>(ring, t16, o, lane, lane_factor32, x_chunk);
+            flush_fast<RING>(ring, t8, tx, o, lane, acc, nfast);
+        if (nfast) {
+            // the fast write-backs' CRC after the bytes before them
+            uint32_t r = fr_mulmod(acc, lane_factor32);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) r ^= __shfl_xor(r, off, 64);
+            o.raw = fr_mulmod(o.raw, fr_xpow8(a.x2n, (uint64_t)FAST_CHUNK * nfast)) ^ uni(r);
+        }
         o = flush_copy<RING>(ring, crctab, o, lane, lane_factor, a.x2n);
         if (lane == 0) a.crc[m] = o.raw ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(a.x2n, o.pos)) ^ 0xFFFFFFFFu;
     }
