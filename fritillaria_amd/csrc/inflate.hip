@@ -446,7 +446,7 @@ __device__ void resolve_lane_tokens(uint8_t *ring, uint32_t t_lane, Out &o, uint
         const uint32_t y = __shfl_up(inc, d, 64);
         if ((int)lane >= d) inc += y;
     }
-    const uint32_t end = chunk_pos + uni((uint32_t)__shfl(inc, 63, 64));
+    const uint32_t end = chunk_pos + (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     const uint32_t lim = end > RING ? end - RING : 0;
     const uint32_t dst = chunk_pos + inc - ln;
     const uint32_t dist = t & 0xFFFF;
@@ -472,9 +472,9 @@ __device__ void resolve_lane_tokens(uint8_t *ring, uint32_t t_lane, Out &o, uint
     while (pending) {
         const uint32_t j = (uint32_t)__ffsll((unsigned long long)pending) - 1;
         pending &= pending - 1;
-        const uint32_t d0 = uni((uint32_t)__shfl(dst, j, 64));
-        const uint32_t L = uni((uint32_t)__shfl(ln, j, 64));
-        const uint32_t D = uni((uint32_t)__shfl(dist, j, 64));
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)dst, (int)j);
+        const uint32_t L = (uint32_t)__builtin_amdgcn_readlane((int)ln, (int)j);
+        const uint32_t D = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)j);
         if (D >= L) {
             for (uint32_t off = lane; off < L; off += 64) {
                 const uint32_t p = d0 - D + off;
@@ -2038,9 +2038,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             while (pending) {
                 const uint32_t j = (uint32_t)__ffsll((unsigned long long)pending) - 1;
                 pending &= pending - 1;
-                const uint32_t d0 = uni((uint32_t)__shfl(dst, (int)j, 64));
-                const uint32_t L = uni((uint32_t)__shfl(len, (int)j, 64));
-                const uint32_t D = uni((uint32_t)__shfl(dist, (int)j, 64));
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)dst, (int)j);
+                const uint32_t L = (uint32_t)__builtin_amdgcn_readlane((int)len, (int)j);
+                const uint32_t D = (uint32_t)__builtin_amdgcn_readlane((int)dist, (int)j);
                 if (D >= L) {
                     for (uint32_t off = lane; off < L; off += 64) {
                         const uint32_t p = d0 - D + off;
