@@ -1278,6 +1278,23 @@ struct LaneBits {
         }                                                                                                 \
     } while (0)
 
+// The same as selects, for the decode steps (no branch: most steps of some
+// lane refill, so the branch would be taken anyway).
+#define LB_REFILL_SEL(b)                                                                                  \
+    do {                                                                                                  \
+        const bool rf_ = (b).bc < 32 && (b).k < 8;                                                        \
+        (b).bb |= (uint64_t)(rf_ ? (b).A0.x : 0u) << ((b).bc & 31);                                       \
+        (b).bc += rf_ ? 32u : 0u;                                                                         \
+        (b).k += rf_ ? 1u : 0u;                                                                           \
+        (b).A0.x = rf_ ? (b).A0.y : (b).A0.x;                                                             \
+        (b).A0.y = rf_ ? (b).A0.z : (b).A0.y;                                                             \
+        (b).A0.z = rf_ ? (b).A0.w : (b).A0.z;                                                             \
+        (b).A0.w = rf_ ? (b).A1.x : (b).A0.w;                                                             \
+        (b).A1.x = rf_ ? (b).A1.y : (b).A1.x;                                                             \
+        (b).A1.y = rf_ ? (b).A1.z : (b).A1.y;                                                             \
+        (b).A1.z = rf_ ? (b).A1.w : (b).A1.z;                                                             \
+    } while (0)
+
 // Same, moving B in first when A is used up (waits for B's load).
 #define LB_REFILL_SYNC(b)                                                                                 \
     do {                                                                                                  \
@@ -1679,7 +1696,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
         }
 #pragma unroll
         for (uint32_t sub = 0; sub < 4; ++sub) {
-            LB_REFILL(b); // harmless for a lane at a header
+            LB_REFILL_SEL(b); // harmless for a lane at a header
             // a symbol needs up to 20 bits, then (a match) a refill and 28;
             // a lane without them waits for the next pass
             if (state != LS_CODES || b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
@@ -1705,7 +1722,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             b.bb >>= c1;
             b.bc -= c1;
             b.avail = avail0 - c1;
-            LB_REFILL(b);
+            LB_REFILL_SEL(b);
             const uint32_t avail2 = b.avail;
             const uint32_t lo2 = (uint32_t)b.bb;
             uint32_t L2;
