@@ -7,9 +7,11 @@ each stage with HIP events. The one-wave-per-member variants reach their
 speed on any sample that fills the GPU, but the two-phase variants run one
 member per lane and need many more members in flight, so a sample
 understates them: when the file is larger than the sample, the best
-configuration of each variant family is timed again on the whole file and
-the fastest there wins. Every schedule produces the same bytes and columns;
-only time differs.
+configuration of each variant family is timed again on the whole file (the
+two-phase family with every work-queue order: on a sample, largest-first
+waves finish together, while on a whole file whose waves share SIMDs file
+order has been faster) and the fastest there wins. Every schedule produces
+the same bytes and columns; only time differs.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ def sample_prefix(compressed: np.ndarray, max_bytes: int) -> np.ndarray:
 
 
 def autotune(ctx: hip.HipContext, compressed: np.ndarray, *, sample_bytes=512 << 20, repeats=2,
-             schedules=None, orders=(0, 1), windows=(2048,), log=None) -> dict:
+             schedules=None, orders=(0, 1), windows=(2048,), whole_file_orders=(0, 1, 2), log=None) -> dict:
     sample = sample_prefix(compressed, sample_bytes)
     table = hip.inflate_schedules()
     schedules = range(len(table)) if schedules is None else schedules
@@ -60,18 +62,19 @@ def autotune(ctx: hip.HipContext, compressed: np.ndarray, *, sample_bytes=512 <<
         finals = []
         for fam in sorted({r["tokens"] for r in results}):
             r = min((x for x in results if x["tokens"] == fam), key=lambda x: x["total_ms"])
-            p = hip.BatchedPipeline(ctx, compressed, inflate_schedule=r["inflate_schedule"], order=r["order"],
-                                    bam_window=r["bam_window"])
-            full = None
-            for _ in range(repeats):
-                p.run()
-                ms = p.stage_ms()
-                full = ms if full is None or ms["total"] < full["total"] else full
-            p.close()
-            rec = dict(r, inflate_ms=full["inflate"], total_ms=full["total"], whole_file=True)
-            finals.append(rec)
-            if log:
-                log(f"autotune whole file {rec}")
+            for order in (whole_file_orders if fam == 3 else (r["order"],)):
+                p = hip.BatchedPipeline(ctx, compressed, inflate_schedule=r["inflate_schedule"], order=order,
+                                        bam_window=r["bam_window"])
+                full = None
+                for _ in range(repeats):
+                    p.run()
+                    ms = p.stage_ms()
+                    full = ms if full is None or ms["total"] < full["total"] else full
+                p.close()
+                rec = dict(r, order=order, inflate_ms=full["inflate"], total_ms=full["total"], whole_file=True)
+                finals.append(rec)
+                if log:
+                    log(f"autotune whole file {rec}")
         best = min(finals, key=lambda r: r["total_ms"])
         results += finals
     return {"best": best, "sample_bytes": int(sample.size), "seconds": time.time() - t0, "results": results}

@@ -408,7 +408,10 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
         if (r != hipSuccess) return fail(hip_err(err, r, "member lengths"));
         std::vector<uint32_t> order(p->n);
         std::iota(order.begin(), order.end(), 0u);
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+        if (p->cfg.order == 2)
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] < in_len[b]; });
+        else
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
         if ((rc = dalloc(p, (void **)&p->d_order, nn * 4, err, "member order"))) return fail(rc);
         r = hipMemcpy(p->d_order, order.data(), p->n * 4, hipMemcpyHostToDevice);
         if (r != hipSuccess) return fail(hip_err(err, r, "member order"));

@@ -109,7 +109,7 @@ def test_pipeline_reports_a_bad_crc_like_the_per_call_path(hip_ctx):
 
 
 @pytest.mark.parametrize("schedule", [16, 18])
-@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, 2])
 def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order):
     """The two-phase inflate (one member per lane, then one wave per member) in
     the graph-replayed pipeline, each replay against the oracle."""
