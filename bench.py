@@ -32,17 +32,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(records, schedule):
+def measured_traffic(records, schedule, order):
     """HBM bytes per inflate launch for this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/fr_inflate_kernel_traffic_*.json),
     each scaled by the same counter's reading for a plain copy of known size.
-    None when no pass was taken on this record count and inflate variant."""
+    None when no pass was taken on this record count, inflate variant and
+    member order."""
     path = os.path.join(ROOT, "profiles", f"fr_inflate_kernel_traffic_config2_{records}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("inflate_schedule") != schedule:
+    if t.get("inflate_schedule") != schedule or t.get("order") != order:
         return None
     return int(t["traffic_bytes_per_launch"])
 
@@ -250,7 +251,7 @@ def main():
     two_phase = variant is not None and variant[3] == 3
     inflate_kernel = ("fr_inflate_lanes_kernel + fr_inflate_resolve_kernel (two-phase inflate: lane-per-member "
                       "decode, then wave-per-member copies and CRC)") if two_phase else "fr_inflate_kernel"
-    traffic = measured_traffic(args.records, args.schedule)
+    traffic = measured_traffic(args.records, args.schedule, args.order)
     if traffic is not None:
         log(f"inflate HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic} bytes per launch")
 
