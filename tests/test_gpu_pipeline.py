@@ -81,6 +81,44 @@ def test_schedules_do_not_change_results(hip_ctx, oracle_mod, opts):
     check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=False)
 
 
+def bgzf_member_with_extra(payload: bytes, before: bytes = b"", after: bytes = b"") -> bytes:
+    """A BGZF member whose FEXTRA holds other subfields before and/or after BC
+    (RFC 1952 allows any; the BC one gives the block size)."""
+    import struct
+    import zlib
+    from helpers import deflate_raw
+    cdata = deflate_raw(payload, 6)
+    extra = before + b"BC" + struct.pack("<H", 2) + b"\0\0" + after
+    bsize = 12 + len(extra) + len(cdata) + 8
+    at = 12 + len(before) + 4
+    hdr = bytes([0x1F, 0x8B, 8, 4, 0, 0, 0, 0, 0, 0xFF]) + struct.pack("<H", len(extra)) + extra
+    hdr = hdr[:at] + struct.pack("<H", bsize - 1) + hdr[at + 2:]
+    return hdr + cdata + struct.pack("<II", zlib.crc32(payload) & 0xFFFFFFFF, len(payload))
+
+
+def test_headers_with_other_subfields_are_found_on_the_device(hip_ctx, oracle_mod):
+    """Device discovery checks the usual 18-byte header from one load; members
+    whose FEXTRA carries other subfields take its general walk, with the same
+    result as the oracle."""
+    print("[gpu] pipeline: headers with other extra subfields", flush=True)
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(43)
+    data = bam_header() + b"".join(random_records(rng, 600))
+    layouts = [(b"", b""), (b"XY\x02\x00ab", b""), (b"", b"ZZ\x01\x00q"), (b"AB\x00\x00", b"CD\x03\x00xyz")]
+    members, pos, i = [], 0, 0
+    while pos < len(data):
+        before, after = layouts[i % len(layouts)]
+        members.append(bgzf_member_with_extra(data[pos:pos + 7000], before, after))
+        pos += 7000
+        i += 1
+    blob = b"".join(members) + bgzf(b"", eof=True)
+    pipe = fa.BatchedPipeline(hip_ctx, blob)
+    assert pipe.device_discovery()
+    pipe.run()
+    assert pipe.members() == len(oracle_mod.discover(blob).spans)
+    check_pipeline(pipe, oracle_mod, blob)
+
+
 def test_false_header_in_payload_falls_back_to_host_walk(hip_ctx, oracle_mod):
     import fritillaria_amd as fa
     rng = np.random.default_rng(41)
@@ -109,7 +147,7 @@ def test_pipeline_reports_a_bad_crc_like_the_per_call_path(hip_ctx):
 
 
 @pytest.mark.parametrize("schedule", [16, 18])
-@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
 def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order):
     """The two-phase inflate (one member per lane, then one wave per member) in
     the graph-replayed pipeline, each replay against the oracle."""
