@@ -9,8 +9,9 @@ member per lane and need many more members in flight, so a sample
 understates them: when the file is larger than the sample, the best
 configuration of each variant family is timed again on the whole file (the
 two-phase family with every work-queue order: on a sample, largest-first
-waves finish together, while on a whole file whose waves share SIMDs file
-order has been faster) and the fastest there wins. Every schedule produces
+waves finish together, while on a whole file, whose waves share SIMDs, the
+order that wins depends on which waves end up sharing one) and the fastest
+there wins. Every schedule produces
 the same bytes and columns; only time differs.
 """
 from __future__ import annotations
@@ -35,7 +36,7 @@ def sample_prefix(compressed: np.ndarray, max_bytes: int) -> np.ndarray:
 
 
 def autotune(ctx: hip.HipContext, compressed: np.ndarray, *, sample_bytes=512 << 20, repeats=2,
-             schedules=None, orders=(0, 1), windows=(2048,), whole_file_orders=(0, 1, 2), log=None) -> dict:
+             schedules=None, orders=(0, 1), windows=(2048,), whole_file_orders=(0, 1, 2, 3), log=None) -> dict:
     sample = sample_prefix(compressed, sample_bytes)
     table = hip.inflate_schedules()
     schedules = range(len(table)) if schedules is None else schedules

@@ -1525,7 +1525,22 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                             if (wcur != NONE) put_window(wcur, wv, pos + h);
                             wcur = NONE;
                             wv = 0;
-                            for (uint32_t i = 0; i < len; ++i) {
+                            // four bytes per step while they last (the reader
+                            // is at a byte boundary, and len * 8 <= avail)
+                            uint32_t i = 0;
+                            for (; i + 4 <= len; i += 4) {
+                                LB_REFILL_SYNC(b);
+                                const uint32_t v = (uint32_t)b.bb;
+                                b.bb >>= 32;
+                                b.bc -= 32;
+                                b.avail -= 32;
+                                out[pos] = (uint8_t)v;
+                                out[pos + 1] = (uint8_t)(v >> 8);
+                                out[pos + 2] = (uint8_t)(v >> 16);
+                                out[pos + 3] = (uint8_t)(v >> 24);
+                                pos += 4;
+                            }
+                            for (; i < len; ++i) {
                                 LB_REFILL_SYNC(b);
                                 out[pos++] = (uint8_t)lb_bits(b, 8, err);
                             }

@@ -408,10 +408,20 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
         if (r != hipSuccess) return fail(hip_err(err, r, "member lengths"));
         std::vector<uint32_t> order(p->n);
         std::iota(order.begin(), order.end(), 0u);
-        if (p->cfg.order == 2)
+        if (p->cfg.order == 2) {
             std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] < in_len[b]; });
-        else
+        } else if (p->cfg.order == 3) {
+            // largest first within each run of 1024 file-order members: a
+            // wave's members are alike and still come from one stretch of
+            // the file
+            for (size_t i = 0; i < order.size(); i += 1024) {
+                const size_t e = std::min(order.size(), i + 1024);
+                std::stable_sort(order.begin() + i, order.begin() + e,
+                                 [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+            }
+        } else {
             std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+        }
         if ((rc = dalloc(p, (void **)&p->d_order, nn * 4, err, "member order"))) return fail(rc);
         r = hipMemcpy(p->d_order, order.data(), p->n * 4, hipMemcpyHostToDevice);
         if (r != hipSuccess) return fail(hip_err(err, r, "member order"));
