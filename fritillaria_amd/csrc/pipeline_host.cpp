@@ -411,13 +411,19 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
         if (p->cfg.order == 2) {
             std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] < in_len[b]; });
         } else if (p->cfg.order == 3) {
-            // largest first within each run of 1024 file-order members: a
-            // wave's members are alike and still come from one stretch of
-            // the file
-            for (size_t i = 0; i < order.size(); i += 1024) {
-                const size_t e = std::min(order.size(), i + 1024);
-                std::stable_sort(order.begin() + i, order.begin() + e,
-                                 [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
+            // members whose payload is no smaller than their ISIZE (stored
+            // blocks: a lane copies them, which would hold up a wave of
+            // decoding lanes) first, together; then the rest largest first
+            // within each run of 1024 file-order members: a wave's members
+            // are alike and still come from one stretch of the file
+            std::vector<uint32_t> isz(p->n);
+            r = hipMemcpy(isz.data(), p->m.isize, p->n * 4, hipMemcpyDeviceToHost);
+            if (r != hipSuccess) return fail(hip_err(err, r, "member sizes"));
+            const auto mid = std::stable_partition(order.begin(), order.end(),
+                                                   [&](uint32_t a) { return in_len[a] >= isz[a]; });
+            for (auto i = mid; i < order.end(); i += std::min<std::ptrdiff_t>(1024, order.end() - i)) {
+                const auto e = i + std::min<std::ptrdiff_t>(1024, order.end() - i);
+                std::stable_sort(i, e, [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });
             }
         } else {
             std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return in_len[a] > in_len[b]; });

@@ -25,7 +25,9 @@ typedef struct frhip_pipeline frhip_pipeline;
 typedef struct {
     uint32_t waves_per_cu;     /* persistent inflate grid = CUs x this; 0 = the schedule's resident waves */
     uint32_t order;            /* member order on the work queue: 0 file order, 1 largest payload first,
-                                  2 smallest payload first */
+                                  2 smallest payload first, 3 members whose payload is no smaller than
+                                  their ISIZE first, then the rest largest first within each run of
+                                  1024 in file order */
     uint32_t bam_window;       /* speculation window of the boundary scan; 0 = 2048 */
     uint32_t use_graph;        /* 1: capture the whole run once and replay it as one hipGraph */
     uint32_t host_discovery;   /* 1: walk member headers on the host once at create; 0: sieve on the device every run */
