@@ -1387,7 +1387,20 @@ __device__ __forceinline__ void lb_copy(LaneBits &d, const LaneBits &s)
     d.ip = s.ip;
 }
 
-enum : uint32_t { LS_HEADER = 0, LS_CODES = 1, LS_DONE = 2 };
+enum : uint32_t { LS_HEADER = 0, LS_CODES = 1, LS_DONE = 2, LS_LENS = 3, LS_TABLES = 4 };
+
+// Counts per code length of the ndist distance code lengths held 4 bits
+// each in d0..d3.
+__device__ __forceinline__ F8 dist_counts(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, uint32_t ndist)
+{
+    F8 c = {0, 0};
+#pragma unroll
+    for (uint32_t j = 0; j < 30; ++j) {
+        const uint32_t dn = j < 8 ? d0 : j < 16 ? d1 : j < 24 ? d2 : d3;
+        if (j < ndist) f8_add(c, (dn >> (4 * (j & 7))) & 15, 1);
+    }
+    return c;
+}
 
 template <uint32_t NW>
 __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a)
@@ -1482,6 +1495,39 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             rs_i[j] = NONE;
         }
     };
+    // A dynamic block's code lengths (LS_LENS) take two passes over the same
+    // input, lbit bits into the payload: pass 0 counts the literal/length
+    // lengths per code length (cll) and keeps the distance lengths, 4 bits
+    // each, in dn0..dn3; pass 1 turns the counts into list entries and
+    // places the literal/length symbols. There is no room in LDS to keep all
+    // the lengths between the passes.
+    uint32_t pass = 0, nlen = 0, ndist = 0, total = 0, index = 0, prev = 0, len256 = 0, lbit = 0;
+    uint32_t dn0 = 0, dn1 = 0, dn2 = 0, dn3 = 0;
+    F16 cll = {0, 0, 0, 0};
+    // Puts the reader `bit` bits into the payload. It waits for its loads:
+    // rare (twice per dynamic block), and no later step then waits for them.
+    auto lb_seek = [&](uint32_t bit) __attribute__((always_inline)) {
+        const uint64_t at = in_off + (bit >> 3);
+        const uint4 *blk = reinterpret_cast<const uint4 *>(a.in + (at & ~15ull));
+        b.A0 = blk[0];
+        b.A1 = blk[1];
+        b.B0 = blk[2];
+        b.B1 = blk[3];
+        b.ip = blk + 4;
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t byte = (uint32_t)(at & 15);
+#pragma unroll
+        for (uint32_t i = 0; i < 3; ++i)
+            if (i < (byte >> 2)) LB_SHIFT(b);
+        b.k = byte >> 2;
+        b.bb = 0;
+        b.bc = 0;
+        b.avail = in_len * 8 - bit;
+        LB_REFILL(b);
+        const uint32_t drop = (byte & 3) * 8 + (bit & 7);
+        b.bb >>= drop;
+        b.bc -= drop;
+    };
     // the member's parameters have landed: otherwise their first use inside
     // the loop would leave a wait on the memory counter in every step
     __builtin_amdgcn_s_waitcnt(0);
@@ -1512,8 +1558,8 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                         LB_REFILL_SYNC(b);
                         const uint32_t len = lb_bits(b, 16, err);
                         LB_REFILL_SYNC(b);
-                        const uint32_t nlen = lb_bits(b, 16, err);
-                        if (len != (~nlen & 0xFFFFu))
+                        const uint32_t ncomp = lb_bits(b, 16, err);
+                        if (len != (~ncomp & 0xFFFFu))
                             err = ST_STORED_LEN;
                         else if (len * 8 > b.avail)
                             err = ST_TRUNCATED;
@@ -1576,42 +1622,44 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                     state = LS_CODES;
                     continue;
                 }
-                // dynamic header
-                const uint32_t nlen = lb_bits(b, 5, err) + 257;
-                const uint32_t ndist = lb_bits(b, 5, err) + 1;
+                // dynamic header up to the code-length code; the lengths are
+                // read in the decode steps (LS_LENS), in parallel with the
+                // other lanes' symbols
+                nlen = lb_bits(b, 5, err) + 257;
+                ndist = lb_bits(b, 5, err) + 1;
                 const uint32_t ncode = lb_bits(b, 4, err) + 4;
                 if (err == ST_OK && (nlen > 286 || ndist > 30)) err = ST_BAD_COUNTS;
                 uint64_t cl = 0; // code-length code lengths, 3 bits per symbol 0..18
-        #pragma unroll
+#pragma unroll
                 for (uint32_t i = 0; i < 19; ++i) {
                     if (err == ST_OK && i < ncode) {
                         LB_REFILL_SYNC(b);
                         cl |= (uint64_t)lb_bits(b, 3, err) << (3 * clen_order(i));
                     }
                 }
-                LaneCode cc;
-                uint64_t cs0 = 0, cs1 = 0; // code-length code symbols sorted, 5 bits each
                 if (err == ST_OK) {
+                    // the code goes in dc (its limits flatten past length 7),
+                    // its symbols sorted by (length, symbol) in the distance
+                    // list's bytes
                     F8 c8 = {0, 0};
-        #pragma unroll
+#pragma unroll
                     for (uint32_t s = 0; s < 19; ++s) f8_add(c8, (uint32_t)(cl >> (3 * s)) & 7, 1);
-                    const int e = lane_code(cc, 19, [&](int k) -> uint32_t { return k < 8 ? f8_get(c8, (uint32_t)k) : 0; });
+                    const int e = lane_code(dc, 19, [&](int k) -> uint32_t { return k < 8 ? f8_get(c8, (uint32_t)k) : 0; });
                     if (e != 0) err = ST_BAD_TABLE;
                     F8 o8 = {0, 0};
                     uint32_t run = 0;
-        #pragma unroll
+#pragma unroll
                     for (uint32_t k = 1; k < 8; ++k) {
                         f8_add(o8, k, run);
                         run += f8_get(c8, k);
                     }
-        #pragma unroll
+#pragma unroll
                     for (uint32_t s = 0; s < 19; ++s) {
                         const uint32_t L = (uint32_t)(cl >> (3 * s)) & 7;
                         if (L) {
                             const uint32_t p = f8_get(o8, L);
                             f8_add(o8, L, 1);
-                            cs0 |= p < 12 ? (uint64_t)s << (5 * p) : 0;
-                            cs1 |= p < 12 ? 0 : (uint64_t)s << (5 * (p - 12));
+                            LANE_BYTE(dw, p < 32 ? p : 31) = (uint8_t)s;
                         }
                     }
                 }
@@ -1619,110 +1667,96 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                     state = LS_DONE;
                     continue;
                 }
-                // Two passes over the code lengths (from the same input position): the
-                // first counts them per length and checks them, the second places the
-                // symbols in the sorted lists. There is no room in LDS to keep the
-                // lengths between the two.
-                const uint32_t total = nlen + ndist;
-                LaneBits saved;
-                lb_copy(saved, b);
-                F16 cll = {0, 0, 0, 0}, oll = {0, 0, 0, 0};
-                F8 cd = {0, 0}, od = {0, 0};
-                for (int pass = 0; pass < 2 && err == ST_OK; ++pass) {
-                    if (pass == 1) {
-                        lb_copy(b, saved);
-                        uint32_t r0 = 0, r1 = 0;
-        #pragma unroll
-                        for (uint32_t k = 1; k < 16; ++k) {
-                            f16_add(oll, k, r0);
-                            r0 += f16_get(cll, k);
-                            f8_add(od, k, r1);
-                            r1 += f8_get(cd, k);
-                        }
-                        for (uint32_t w = 0; w < 9; ++w) hw[w * LANES] = 0;
-                    }
-                    uint32_t index = 0, prev = 0, len256 = 0;
-                    while (index < total) {
-                        LB_REFILL_SYNC(b);
-                        const uint32_t ci = lb_decode<7>(b, cc, ST_BAD_CODE, err);
-                        if (err != ST_OK) break;
-                        const uint32_t sym = (uint32_t)((ci < 12 ? cs0 >> (5 * ci) : cs1 >> (5 * (ci - 12))) & 31);
-                        uint32_t len = 0, rep = 1;
-                        if (sym < 16) {
-                            len = sym;
-                        } else if (sym == 16) {
-                            if (index == 0) {
-                                err = ST_BAD_LENGTH_REPEAT;
-                                break;
-                            }
-                            len = prev;
-                            rep = 3 + lb_bits(b, 2, err);
-                        } else if (sym == 17) {
-                            rep = 3 + lb_bits(b, 3, err);
-                        } else {
-                            rep = 11 + lb_bits(b, 7, err);
-                        }
-                        if (err != ST_OK) break;
-                        if (index + rep > total) {
-                            err = ST_BAD_LENGTH_REPEAT;
-                            break;
-                        }
-                        const uint32_t nll = index < nlen ? (rep < nlen - index ? rep : nlen - index) : 0;
-                        if (len) {
-                            if (pass == 0) {
-                                f16_add(cll, len, nll);
-                                f8_add(cd, len, rep - nll);
-                            } else {
-                                for (uint32_t k = 0; k < rep; ++k) {
-                                    const uint32_t sy = index + k;
-                                    if (sy < nlen) {
-                                        uint32_t q = f16_get(oll, len);
-                                        f16_add(oll, len, 1);
-                                        q = q < 288 ? q : 287;
-                                        LANE_BYTE(lw, q) = (uint8_t)sy;
-                                        if (sy >= 256) hw[(q >> 5) * LANES] |= 1u << (q & 31);
-                                    } else {
-                                        uint32_t q = f8_get(od, len);
-                                        f8_add(od, len, 1);
-                                        LANE_BYTE(dw, q < 32 ? q : 31) = (uint8_t)(sy - nlen);
-                                    }
-                                }
-                            }
-                        }
-                        if (index <= 256 && 256 < index + rep) len256 = len;
-                        prev = len;
-                        index += rep;
-                    }
-                    if (pass == 0 && err == ST_OK) {
-                        uint32_t sll = 0, sd = 0;
-        #pragma unroll
-                        for (uint32_t k = 1; k < 16; ++k) {
-                            sll += f16_get(cll, k);
-                            sd += f8_get(cd, k);
-                        }
-                        if (len256 == 0) err = ST_BAD_TABLE;
-                        int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
-                        if (err == ST_OK && e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
-                        e = lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
-                        if (err == ST_OK && e != 0 && (e < 0 || sd != 1)) err = ST_BAD_TABLE;
+                total = nlen + ndist;
+                index = 0;
+                prev = 0;
+                len256 = 0;
+                pass = 0;
+                cll = F16{0, 0, 0, 0};
+                dn0 = dn1 = dn2 = dn3 = 0;
+                lbit = in_len * 8 - b.avail;
+                state = LS_LENS;
+        } else if (state == LS_TABLES) {
+            // the end of a pass over the code lengths
+            if (pass == 0) {
+                // puff's table checks, in its order
+                if (len256 == 0) err = ST_BAD_TABLE;
+                uint32_t sll = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) sll += f16_get(cll, k);
+                const int e = lane_code(lc, nlen, [&](int k) -> uint32_t { return f16_get(cll, (uint32_t)k); });
+                if (err == ST_OK && e != 0 && (e < 0 || sll != 1)) err = ST_BAD_TABLE;
+                const F8 cd = dist_counts(dn0, dn1, dn2, dn3, ndist);
+                uint32_t sd = 0;
+                int dleft = 1;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    sd += f8_get(cd, k);
+                    dleft = (dleft << 1) - (int)f8_get(cd, k);
+                }
+                if (sd == 0) dleft = 0;
+                if (err == ST_OK && dleft != 0 && (dleft < 0 || sd != 1)) err = ST_BAD_TABLE;
+                if (err != ST_OK) {
+                    state = LS_DONE;
+                    continue;
+                }
+                // counts -> first list entry per code length, for pass 1
+                F16 o = {0, 0, 0, 0};
+                uint32_t r = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    f16_add(o, k, r);
+                    r += f16_get(cll, k);
+                }
+                cll = o;
+                for (uint32_t w = 0; w < 9; ++w) hw[w * LANES] = 0;
+                lb_seek(lbit);
+                index = 0;
+                prev = 0;
+                pass = 1;
+                state = LS_LENS;
+            } else {
+                // the distance list from the lengths in dn0..dn3 (the
+                // code-length symbols in dw are no longer needed)
+                const F8 cd = dist_counts(dn0, dn1, dn2, dn3, ndist);
+                F8 od = {0, 0};
+                uint32_t r = 0;
+#pragma unroll
+                for (uint32_t k = 1; k < 16; ++k) {
+                    f8_add(od, k, r);
+                    r += f8_get(cd, k);
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < 30; ++j) {
+                    const uint32_t dn = j < 8 ? dn0 : j < 16 ? dn1 : j < 24 ? dn2 : dn3;
+                    const uint32_t L = (dn >> (4 * (j & 7))) & 15;
+                    if (j < ndist && L) {
+                        const uint32_t q = f8_get(od, L);
+                        f8_add(od, L, 1);
+                        LANE_BYTE(dw, q < 32 ? q : 31) = (uint8_t)j;
                     }
                 }
-                state = err == ST_OK ? LS_CODES : LS_DONE;
+                (void)lane_code(dc, ndist, [&](int k) -> uint32_t { return f8_get(cd, (uint32_t)k); });
+                state = LS_CODES;
+            }
         }
 #pragma unroll
         for (uint32_t sub = 0; sub < 4; ++sub) {
             LB_REFILL_SEL(b); // harmless for a lane at a header
             // a symbol needs up to 20 bits, then (a match) a refill and 28;
             // a lane without them waits for the next pass
-            if (state != LS_CODES || b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
+            if ((state != LS_CODES && state != LS_LENS) || b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
             // One symbol, and its distance when it is a length, without
             // branches: every lane runs the same instructions, flags pick
-            // what each result means. Only the stores and errors branch.
+            // what each result means. Only the stores and errors branch. A
+            // lane reading code lengths uses only the second code (dc holds
+            // the code-length code, dw its symbols).
+            const bool lens = state == LS_LENS;
             const uint32_t avail0 = b.avail;
             const uint32_t lo0 = (uint32_t)b.bb;
             uint32_t L1;
             const uint32_t idx1 = lane_find(lc, __builtin_bitreverse32(lo0) >> 17, L1);
-            const bool ok1 = L1 != 0 && L1 <= avail0;
+            const bool ok1 = !lens && L1 != 0 && L1 <= avail0;
             const uint32_t e1 = idx1 < 288 ? idx1 : 287;
             const uint32_t sym = LANE_BYTE(lw, e1) | (((hw[(e1 >> 5) * LANES] >> (e1 & 31)) & 1u) << 8);
             const uint32_t n1 = ok1 ? L1 : 0;
@@ -1742,38 +1776,53 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             const uint32_t lo2 = (uint32_t)b.bb;
             uint32_t L2;
             const uint32_t idx2 = lane_find(dc, __builtin_bitreverse32(lo2) >> 17, L2);
-            const bool ok2 = okx && L2 != 0 && L2 <= avail2;
+            const bool ok2 = (okx || lens) && L2 != 0 && L2 <= avail2;
             const uint32_t ds = LANE_BYTE(dw, idx2 < 32 ? idx2 : 31);
-            const bool okd = ok2 && ds < 30;
-            const uint32_t dcode = okd ? ds : 0;
-            const uint32_t xd = dist_extra(dcode);
-            const bool okxd = okd && L2 + xd <= avail2;
-            const uint32_t dist = dist_base(dcode) + __builtin_amdgcn_ubfe(lo2, L2, xd);
-            const bool match = okxd && dist <= pos && pos + mlen <= cap;
-            const uint32_t c2 = match ? L2 + xd : 0;
+            // a distance code, or a code-length symbol (0..18): its extra
+            // bits and base give the distance or the repeat count
+            const bool okd = ok2 && (lens || ds < 30);
+            const uint32_t dcode = okd && !lens ? ds : 0;
+            const uint32_t xd = lens ? (ds == 16 ? 2u : ds == 17 ? 3u : ds == 18 ? 7u : 0u) : dist_extra(dcode);
+            const uint32_t vbase = lens ? (ds == 18 ? 11u : ds >= 16 ? 3u : 1u) : dist_base(dcode);
+            const bool rep0 = lens && ds == 16 && index == 0; // checked before the extra bits
+            const bool okxd = okd && !rep0 && L2 + xd <= avail2;
+            const uint32_t val = vbase + __builtin_amdgcn_ubfe(lo2, L2, xd);
+            const bool match = !lens && okxd && val <= pos && pos + mlen <= cap;
+            const bool clok = lens && okxd && index + val <= total;
+            const uint32_t c2 = match || clok ? L2 + xd : 0;
             b.bb >>= c2;
             b.bc -= c2;
             b.avail = avail2 - c2;
-            if (!(lit || eob || match)) {
+            if (!(lit || eob || match || clok)) {
                 // the first check that failed, in puff's order
-                if (!ok1)
+                if (lens) {
+                    if (!ok2)
+                        err = L2 == 0 && avail2 >= 15 ? ST_BAD_CODE : ST_TRUNCATED;
+                    else if (rep0)
+                        err = ST_BAD_LENGTH_REPEAT;
+                    else if (!okxd)
+                        err = ST_TRUNCATED;
+                    else
+                        err = ST_BAD_LENGTH_REPEAT;
+                } else if (!ok1) {
                     err = L1 == 0 && avail0 >= 15 ? ST_BAD_CODE : ST_TRUNCATED;
-                else if (sym < 256)
+                } else if (sym < 256) {
                     err = ST_OUTPUT_OVERFLOW;
-                else if (!lenc)
+                } else if (!lenc) {
                     err = ST_BAD_CODE;
-                else if (!okx)
+                } else if (!okx) {
                     err = ST_TRUNCATED;
-                else if (!ok2)
+                } else if (!ok2) {
                     err = L2 == 0 && avail2 >= 15 ? ST_BAD_DISTANCE : ST_TRUNCATED;
-                else if (!okd)
+                } else if (!okd) {
                     err = ST_BAD_DISTANCE;
-                else if (!okxd)
+                } else if (!okxd) {
                     err = ST_TRUNCATED;
-                else if (dist > pos)
+                } else if (val > pos) {
                     err = ST_BAD_DISTANCE;
-                else
+                } else {
                     err = ST_OUTPUT_OVERFLOW;
+                }
                 state = LS_DONE;
             }
             {
@@ -1793,11 +1842,43 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                     gap -= REC_SKIP;
                 }
                 rs_i[sub] = nrec++;
-                rs_v[sub] = (gap << 23) | ((mlen - 3) << 15) | (dist - 1);
+                rs_v[sub] = (gap << 23) | ((mlen - 3) << 15) | (val - 1);
             }
             pos += lit ? 1 : match ? mlen : 0;
             mark = match ? pos : mark;
             if (eob) state = last ? LS_DONE : LS_HEADER;
+            if (clok) {
+                // val code lengths of clen; the first nll are literal/length
+                // symbols, the rest distance symbols
+                const uint32_t clen = ds < 16 ? ds : ds == 16 ? prev : 0;
+                const uint32_t nll = index < nlen ? (val < nlen - index ? val : nlen - index) : 0;
+                if (clen) {
+                    const uint32_t q = f16_get(cll, clen);
+                    f16_add(cll, clen, nll); // pass 0: counts; pass 1: next entries
+                    if (pass) {
+                        // symbols of one length take consecutive list entries
+                        for (uint32_t k = 0; k < nll; ++k) {
+                            const uint32_t e = q + k < 288 ? q + k : 287;
+                            const uint32_t sy = index + k;
+                            LANE_BYTE(lw, e) = (uint8_t)sy;
+                            if (sy >= 256) atomicOr(&hw[(e >> 5) * LANES], 1u << (e & 31));
+                        }
+                    } else {
+                        for (uint32_t k = nll; k < val; ++k) {
+                            const uint32_t j = index + k - nlen;
+                            const uint32_t v = clen << (4 * (j & 7));
+                            dn0 |= j < 8 ? v : 0u;
+                            dn1 |= j - 8 < 8 ? v : 0u;
+                            dn2 |= j - 16 < 8 ? v : 0u;
+                            dn3 |= j - 24 < 8 ? v : 0u;
+                        }
+                    }
+                }
+                if (index <= 256 && 256 < index + val) len256 = clen;
+                prev = clen;
+                index += val;
+                if (index == total) state = LS_TABLES;
+            }
         }
     }
     flush_pass();
