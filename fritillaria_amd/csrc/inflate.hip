@@ -59,19 +59,22 @@ enum : uint32_t {
 // carry no extra bits (lengths 3-10), then each group of four doubles the step;
 // code 28 is length 258. Distance code d (0..29): codes 0-3 are distances 1-4,
 // then each pair doubles the step.
-// Written as selects (no branches): codes 4-7 and 2-3 fit the general form
-// with no extra bits.
-__device__ __forceinline__ uint32_t len_extra(uint32_t c) { return c < 4 || c == 28 ? 0 : (c - 4) >> 2; }
+// Written as arithmetic and selects of computed values (no branches): codes
+// 0-7 and 0-3 take the general form with no extra bits and a fixed offset.
+__device__ __forceinline__ uint32_t len_extra(uint32_t c)
+{
+    const uint32_t x = (max(c, 4u) - 4) >> 2;
+    return c == 28 ? 0 : x;
+}
 __device__ __forceinline__ uint32_t len_base(uint32_t c)
 {
-    const uint32_t g = ((4u | (c & 3)) << len_extra(c)) + 3;
-    return c < 4 ? c + 3 : c == 28 ? 258 : g;
+    const uint32_t g = ((4u | (c & 3)) << ((max(c, 4u) - 4) >> 2)) + 3 - (c < 4 ? 4u : 0u);
+    return c == 28 ? 258 : g;
 }
-__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return d < 2 ? 0 : (d - 2) >> 1; }
+__device__ __forceinline__ uint32_t dist_extra(uint32_t d) { return (max(d, 2u) - 2) >> 1; }
 __device__ __forceinline__ uint32_t dist_base(uint32_t d)
 {
-    const uint32_t g = ((2u | (d & 1)) << dist_extra(d)) + 1;
-    return d < 2 ? d + 1 : g;
+    return ((2u | (d & 1)) << dist_extra(d)) + 1 - (d < 2 ? 2u : 0u);
 }
 // Order of the code-length code lengths (RFC 1951 3.2.7), 5 bits per entry.
 __device__ __forceinline__ uint32_t clen_order(uint32_t i)
@@ -1743,10 +1746,12 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
 #pragma unroll
         for (uint32_t sub = 0; sub < 4; ++sub) {
             LB_REFILL_SEL(b); // harmless for a lane at a header
-            // a symbol needs up to 20 bits, then (a match) a refill and 28;
-            // a lane without them waits for the next pass
+            // a symbol needs up to 20 bits, then (a match) a refill and 28
+            // (a second literal 15); a lane without them waits for the next
+            // pass
             if ((state != LS_CODES && state != LS_LENS) || b.bc < 32 || (b.bc < 48 && b.k == 8)) continue;
-            // One symbol, and its distance when it is a length, without
+            // One symbol, and its distance when it is a length or a second
+            // literal after a literal, without
             // branches: every lane runs the same instructions, flags pick
             // what each result means. Only the stores and errors branch. A
             // lane reading code lengths uses only the second code (dc holds
@@ -1774,22 +1779,42 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             LB_REFILL_SEL(b);
             const uint32_t avail2 = b.avail;
             const uint32_t lo2 = (uint32_t)b.bb;
+            // The second code: after a length its distance, in LS_LENS a
+            // code-length symbol (both from dc and dw), after a literal the
+            // next literal/length code (lc, lw), taken only when it is a
+            // literal whose byte shares the first literal's output word (one
+            // word closes per step at most)
+            const bool tb = lit;
+            LaneCode c2c;
+#pragma unroll
+            for (uint32_t i = 0; i < LQ; ++i) c2c.q[i] = tb ? lc.q[i] : dc.q[i];
+            c2c.lim15 = tb ? lc.lim15 : dc.lim15;
             uint32_t L2;
-            const uint32_t idx2 = lane_find(dc, __builtin_bitreverse32(lo2) >> 17, L2);
+            const uint32_t idx2 = lane_find(c2c, __builtin_bitreverse32(lo2) >> 17, L2);
             const bool ok2 = (okx || lens) && L2 != 0 && L2 <= avail2;
-            const uint32_t ds = LANE_BYTE(dw, idx2 < 32 ? idx2 : 31);
+            const uint32_t e2 = min(idx2, tb ? 287u : 31u);
+            const uint32_t ds = LANE_BYTE(tb ? lw : dw, e2);
+            const bool lit2 = tb && L2 != 0 && L2 <= avail2 && ((hw[(e2 >> 5) * LANES] >> (e2 & 31)) & 1u) == 0 &&
+                              pos + 1 < cap && ((pos + h + 1) & 3) != 0;
             // a distance code, or a code-length symbol (0..18): its extra
             // bits and base give the distance or the repeat count
             const bool okd = ok2 && (lens || ds < 30);
             const uint32_t dcode = okd && !lens ? ds : 0;
-            const uint32_t xd = lens ? (ds == 16 ? 2u : ds == 17 ? 3u : ds == 18 ? 7u : 0u) : dist_extra(dcode);
-            const uint32_t vbase = lens ? (ds == 18 ? 11u : ds >= 16 ? 3u : 1u) : dist_base(dcode);
+            // repeat codes 16, 17, 18: 2, 3, 7 extra bits on 3, 3, 11; the
+            // literal lengths 0-15 none on 1 (nibble tables, then a bitwise
+            // select on `lens`, which the compiler keeps branch-free)
+            const uint32_t t16 = ds - 16;
+            const uint32_t xl = t16 < 3 ? (0x732u >> (4 * t16)) & 15u : 0u;
+            const uint32_t bl = t16 < 3 ? (0xB33u >> (4 * t16)) & 15u : 1u;
+            const uint32_t lm = 0u - (uint32_t)lens;
+            const uint32_t xd = (xl & lm) | (dist_extra(dcode) & ~lm);
+            const uint32_t vbase = (bl & lm) | (dist_base(dcode) & ~lm);
             const bool rep0 = lens && ds == 16 && index == 0; // checked before the extra bits
             const bool okxd = okd && !rep0 && L2 + xd <= avail2;
             const uint32_t val = vbase + __builtin_amdgcn_ubfe(lo2, L2, xd);
             const bool match = !lens && okxd && val <= pos && pos + mlen <= cap;
             const bool clok = lens && okxd && index + val <= total;
-            const uint32_t c2 = match || clok ? L2 + xd : 0;
+            const uint32_t c2 = match || clok ? L2 + xd : lit2 ? L2 : 0;
             b.bb >>= c2;
             b.bc -= c2;
             b.avail = avail2 - c2;
@@ -1834,6 +1859,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                 wv = close ? 0u : wv;
                 wcur = lit ? rel >> 2 : wcur;
                 wv |= lit ? sym << (8 * (rel & 3)) : 0u;
+                wv |= lit2 ? ds << (8 * ((rel + 1) & 3)) : 0u;
             }
             if (match) {
                 uint32_t gap = pos - mark;
@@ -1844,7 +1870,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
                 rs_i[sub] = nrec++;
                 rs_v[sub] = (gap << 23) | ((mlen - 3) << 15) | (val - 1);
             }
-            pos += lit ? 1 : match ? mlen : 0;
+            pos += lit ? 1 + (uint32_t)lit2 : match ? mlen : 0;
             mark = match ? pos : mark;
             if (eob) state = last ? LS_DONE : LS_HEADER;
             if (clok) {
