@@ -398,8 +398,25 @@ __device__ void build_pair_lut(uint32_t *lut, const uint16_t *count, const uint1
     WFENCE();
 }
 
+// Global-memory pointers keep their address space through structs and
+// calls, so their loads and stores are global (vector memory counter only),
+// not flat: a flat access also counts on the LDS counter, and every wait for
+// an LDS read would then wait for it too.
+#define FR_GLOBAL __attribute__((address_space(1)))
+// 16-byte global loads and stores through a native vector type (uint4's
+// assignment operators take generic-space references only)
+typedef uint32_t fr_v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 gload16(const FR_GLOBAL void *p)
+{
+    const fr_v4 x = *reinterpret_cast<const FR_GLOBAL fr_v4 *>(p);
+    return make_uint4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void gstore16(FR_GLOBAL void *p, uint4 v)
+{
+    *reinterpret_cast<FR_GLOBAL fr_v4 *>(p) = fr_v4{v.x, v.y, v.z, v.w};
+}
 struct Out {
-    uint8_t *g;         // member's output in HBM
+    FR_GLOBAL uint8_t *g; // member's output in HBM
     uint32_t pos;       // bytes produced
     uint32_t flushed;   // bytes written back to HBM (and folded into the CRC)
     uint32_t visible;   // bytes whose HBM stores are known complete
@@ -521,7 +538,7 @@ __device__ void flush(const uint8_t *ring, const uint32_t *crctab, Out &o, uint3
     const uint32_t c = o.pos - o.flushed;
     if (c == 0) return;
     WFENCE();
-    uint8_t *dst = o.g + o.flushed;
+    FR_GLOBAL uint8_t *dst = o.g + o.flushed;
     const uint32_t s0 = o.skew + o.flushed; // slot (unmasked) of the first byte
     const uint32_t head = (16u - (uint32_t)((uintptr_t)dst & 15)) & 15;
     if (c < head + 16) {
@@ -534,7 +551,7 @@ __device__ void flush(const uint8_t *ring, const uint32_t *crctab, Out &o, uint3
             const uint32_t off = head + (w << 4);
             // an aligned output word sits at a 16-byte aligned ring slot
             const uint4 v = *reinterpret_cast<const uint4 *>(ring + ((s0 + off) & G::RMASK));
-            *reinterpret_cast<uint4 *>(dst + off) = v;
+            gstore16(dst + off, v);
         }
         for (uint32_t i = tail0 + lane; i < c; i += 64) dst[i] = ring[(s0 + i) & G::RMASK];
     }
@@ -617,7 +634,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
     b.lead = (uint32_t)(in_off & (CHUNK - 1));
     b.stage_end = 0;
     Out o;
-    o.g = gout;
+    o.g = (FR_GLOBAL uint8_t *)gout;
     o.pos = 0;
     o.flushed = 0;
     o.visible = 0;
@@ -1070,7 +1087,7 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     o.visible = o.flushed;
                 }
-                const uint8_t *src = o.g + (o.pos - dist);
+                const FR_GLOBAL uint8_t *src = o.g + (o.pos - dist);
                 for (uint32_t off = lane; off < len; off += 64) sm.ring[(d0 + off) & G::RMASK] = src[off];
             }
             WFENCE();
@@ -1917,7 +1934,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
 }
 
 // Four bytes of a partial block, skipping those at or past `end`.
-__device__ __forceinline__ uint32_t part4(const uint8_t *p, const uint8_t *end)
+__device__ __forceinline__ uint32_t part4(const FR_GLOBAL uint8_t *p, const FR_GLOBAL uint8_t *end)
 {
     uint32_t v = 0;
 #pragma unroll
@@ -1930,13 +1947,13 @@ __device__ __forceinline__ uint32_t part4(const uint8_t *p, const uint8_t *end)
 // [from, from + WIN - 15). Blocks past position P are not read; the block
 // that straddles P is read byte by byte.
 constexpr uint32_t WIN = 1024;
-__device__ __forceinline__ uint4 load_window(const uint8_t *g, uint32_t from, uint32_t P, uint32_t lane)
+__device__ __forceinline__ uint4 load_window(const FR_GLOBAL uint8_t *g, uint32_t from, uint32_t P, uint32_t lane)
 {
-    const uint8_t *blk = g + from - ((uintptr_t)(g + from) & 15) + 16 * lane;
-    const uint8_t *end = g + P;
+    const FR_GLOBAL uint8_t *blk = g + from - ((uintptr_t)(g + from) & 15) + 16 * lane;
+    const FR_GLOBAL uint8_t *end = g + P;
     uint4 v = {0, 0, 0, 0};
     if (blk + 16 <= end) {
-        v = *reinterpret_cast<const uint4 *>(blk);
+        v = gload16(blk);
     } else if (blk < end) {
         v.x = part4(blk, end);
         v.y = part4(blk + 4, end);
@@ -1995,8 +2012,8 @@ __device__ __forceinline__ void flush_fast(const uint8_t *ring, const uint32_t *
     const uint32_t p = o.flushed + 32 * lane;
     const uint4 v0 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p) & G::RMASK));
     const uint4 v1 = *reinterpret_cast<const uint4 *>(ring + ((o.skew + p + 16) & G::RMASK));
-    *reinterpret_cast<uint4 *>(o.g + p) = v0;
-    *reinterpret_cast<uint4 *>(o.g + p + 16) = v1;
+    gstore16(o.g + p, v0);
+    gstore16(o.g + p + 16, v1);
     uint32_t r = slice8(t8, 0, v0.x, v0.y);
     r = slice8(t8, r, v0.z, v0.w);
     r = slice8(t8, r, v1.x, v1.y);
@@ -2064,7 +2081,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         const uint64_t out_off = uni64(a.out_off[m]);
         const uint32_t *rec = a.rec + rec_base(out_off, m);
         Out o;
-        o.g = a.out + out_off;
+        o.g = (FR_GLOBAL uint8_t *)(a.out + out_off);
         o.pos = 0;
         o.flushed = 0;
         o.visible = 0;
@@ -2114,6 +2131,34 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
             // the next round's records and window
             rn = ri + k + lane < nr ? rec[ri + k + lane] : 0;
             wn = load_window(o.g, R1, P, lane);
+            // This round's matches. A source wholly in HBM is read as the one
+            // or two aligned 16-byte blocks holding it (two requests, not one
+            // per byte), issued here and used after the window and the ring
+            // copies, which hide their latency.
+            // ring slots are good from R0 + WIN - RING on: the window writes
+            // of this and earlier rounds reach at most WIN bytes past R0
+            const uint32_t lim = R0 + WIN > RING ? R0 + WIN - RING : 0;
+            const bool match = lane < k && len > 0;
+            const uint32_t dst = R0 + inc - len;
+            const uint32_t src = dst - dist;
+            const bool deferred = match && (src + len > R0 || len > TOK_SHORT);
+            const bool far = match && !deferred && src + len <= lim;
+            const bool near = match && !deferred && src >= lim;
+            const uint32_t fhead = (uint32_t)((uintptr_t)(o.g + src) & 15);
+            // this wave's write-backs, before it reads them from HBM
+            if (__ballot(match && src < lim) != 0 && o.visible < o.flushed) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                o.visible = o.flushed;
+            }
+            // left unset where not loaded: a zero fill of the registers would
+            // have to wait for the loads of other lanes into them
+            uint4 f0, f1;
+            if (far) {
+                const FR_GLOBAL uint8_t *blk = o.g + src - fhead;
+                f0 = gload16(blk);
+                if (fhead + len > 16) f1 = gload16(blk + 16);
+            }
             // this round's window into the ring: whole blocks, but the first
             // block only from R0 on (the bytes before R0 are this member's
             // finished output, possibly not yet written back)
@@ -2138,34 +2183,35 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                 *slot = v;
             }
             WFENCE();
-            // ring slots are good from here on: the window writes of this and
-            // earlier rounds reach at most WIN bytes past R0
-            const uint32_t lim = R0 + WIN > RING ? R0 + WIN - RING : 0;
-            const bool match = lane < k && len > 0;
-            const uint32_t dst = R0 + inc - len;
-            const uint32_t src = dst - dist;
-            const bool deferred = match && (src + len > R0 || len > TOK_SHORT);
-            if (__ballot(match && src < lim) != 0 && o.visible < o.flushed) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                o.visible = o.flushed;
+            // Bytes from `B` at q (masked by M) to the ring at d: up to an
+            // aligned ring word, whole words (two aligned source words
+            // funnel-shifted), then the last bytes. Only bytes of this
+            // match's span are written.
+            auto copy = [&](const uint8_t *B, uint32_t M, uint32_t q) __attribute__((always_inline)) {
+                const uint32_t d = o.skew + dst;
+                const uint32_t pre = min((0u - d) & 3u, len);
+                uint32_t i = 0;
+                for (; i < pre; ++i) ring[(d + i) & G::RMASK] = B[(q + i) & M];
+                for (; i + 4 <= len; i += 4) {
+                    const uint32_t sp = q + i, al = sp & ~3u;
+                    const uint32_t w0 = *reinterpret_cast<const uint32_t *>(B + (al & M));
+                    const uint32_t w1 = *reinterpret_cast<const uint32_t *>(B + ((al + 4) & M));
+                    *reinterpret_cast<uint32_t *>(ring + ((d + i) & G::RMASK)) =
+                        __builtin_amdgcn_alignbit(w1, w0, (sp & 3) * 8);
+                }
+                for (; i < len; ++i) ring[(d + i) & G::RMASK] = B[(q + i) & M];
+            };
+            // sources in the ring lie before R0, which no write of this round
+            // changes
+            if (near) copy(ring, G::RMASK, o.skew + src);
+            // sources in HBM, through this lane's 32-byte stage
+            uint4 *stg = stage_all[wave][lane];
+            if (far) {
+                stg[0] = f0;
+                stg[1] = f1;
+                copy(reinterpret_cast<const uint8_t *>(stg), ~0u, fhead);
             }
-            if (match && !deferred && src + len <= lim) {
-                // wholly in HBM: the one or two aligned 16-byte blocks holding
-                // it, through this lane's LDS stage (two requests, not one per byte)
-                const uint32_t head = (uint32_t)((uintptr_t)(o.g + src) & 15);
-                const uint4 *blk = reinterpret_cast<const uint4 *>(o.g + src - head);
-                uint4 *stg = stage_all[wave][lane];
-                stg[0] = blk[0];
-                if (head + len > 16) stg[1] = blk[1];
-                const uint8_t *sb = reinterpret_cast<const uint8_t *>(stg) + head;
-                for (uint32_t i = 0; i < len; ++i) ring[(o.skew + dst + i) & G::RMASK] = sb[i];
-            } else if (match && !deferred && src >= lim) {
-                // wholly in the ring (sources lie before R0: no write of this
-                // round changes them)
-                for (uint32_t i = 0; i < len; ++i)
-                    ring[(o.skew + dst + i) & G::RMASK] = ring[(o.skew + src + i) & G::RMASK];
-            } else if (match && !deferred) {
+            if (match && !deferred && !far && !near) {
                 // straddles lim: byte by byte from HBM or the ring
                 for (uint32_t i = 0; i < len; ++i) {
                     const uint32_t p = src + i;
