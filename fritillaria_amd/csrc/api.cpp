@@ -423,7 +423,8 @@ namespace frhip {
 
 // Plan + upload + launch + verify. On success *out owns the output.
 int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
-                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t)
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t,
+                   uint64_t align_at)
 {
     *out = nullptr;
     (void)hipSetDevice(ctx->ordinal);
@@ -443,6 +444,7 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
             return set_err(err, FRHIP_E_INVALID_BLOCK, s.compressed_offset,
                            "payload extends past the end of the batch");
         in_off[i] = s.payload_start;
+        if (i == align_at) total = (total + 4095) & ~4095ull;
         out_off[i] = total;
         inl[i] = (uint32_t)s.payload_len;
         caps[i] = s.isize;
@@ -527,7 +529,6 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     // verify_and_offsets (backend.rs:221-256): status, then size, then CRC
     const uint32_t *st = h_st, *produced = h_st + n, *crc = h_st + 2 * n;
     offsets[0] = 0;
-    uint64_t cursor = 0;
     for (uint64_t i = 0; i < n; ++i) {
         if (st[i] != 0) {
             buf_free(o);
@@ -549,8 +550,7 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
             if (err) { err->expected = spans[i].crc32; err->actual = crc[i]; }
             return k;
         }
-        cursor += produced[i];
-        offsets[i + 1] = cursor;
+        offsets[i + 1] = out_off[i] + produced[i];
     }
     *out = o;
     return FRHIP_OK;

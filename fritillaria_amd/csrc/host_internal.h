@@ -33,8 +33,12 @@ int hip_err(frhip_error *e, hipError_t r, const char *what);
 const char *inflate_status_message(uint32_t status);
 int discover(const uint8_t *buf, uint64_t len, uint64_t base, std::vector<frhip_block_span> &spans,
              uint64_t *position, frhip_error *err);
+// Members [0, align_at) are placed back to back from 0; member align_at and
+// those after it from the next 4 KiB boundary (align_at >= n: no gap).
+// offsets[i + 1] is the end of member i's output.
 int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
-                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t);
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t,
+                   uint64_t align_at = ~0ull);
 hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n,
                                uint32_t *crc, const uint32_t *x2n, hipStream_t s);
 

@@ -171,6 +171,9 @@ def lib():
         "frhip_bcf_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_uint32), _ep]),
         "frhip_fastq_decode": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_bcf_fastq_decode": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, C.POINTER(_vp), _vp,
+                                             C.POINTER(_u64), C.POINTER(_u64), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32), C.POINTER(_vp), C.POINTER(_vp), _ep]),
         "frhip_fastq_batch_len": (_u64, [_vp]),
         "frhip_fastq_batch_tail": (_u64, [_vp]),
         "frhip_fastq_batch_survivors": (_u64, [_vp]),
@@ -807,6 +810,61 @@ class BcfScanner:
         """(record offsets, tail)."""
         d = self.decode(batch, start, samples, contigs)
         return d.column_to_host("record_offsets"), d.tail()
+
+
+class BcfFastqBatch:
+    """A BCF stream and a bgzipped FASTQ stream decoded together
+    (frhip_bcf_fastq_decode): `bcf` and `fastq` are the two scanners' column
+    batches, `arena` the inflated bytes of both (the FASTQ part from
+    `fastq_base`), which the column batches keep alive."""
+
+    def __init__(self, arena, bcf, fastq, fastq_base, bcf_start, samples, contigs):
+        self.arena = arena
+        self.bcf = bcf
+        self.fastq = fastq
+        self.fastq_base = fastq_base
+        self.bcf_start = bcf_start
+        self.samples = samples
+        self.contigs = contigs
+
+
+def decode_bcf_fastq(context: "HipContext", bcf_compressed, bcf_spans, fastq_compressed,
+                     fastq_spans) -> BcfFastqBatch:
+    """One inflate launch over both streams' members, then the BCF and FASTQ
+    boundary scans and columns (one BcfScanner::decode plus one
+    FastqScanner::decode, crates/fritillaria-cuda/src/{bcf,fastq}.rs)."""
+    both, sp, n_bcf = join_bcf_fastq(bcf_compressed, bcf_spans, fastq_compressed, fastq_spans)
+    return decode_bcf_fastq_joined(context, both, sp, n_bcf)
+
+
+def join_bcf_fastq(bcf_compressed, bcf_spans, fastq_compressed, fastq_spans):
+    """(bytes, spans, n_bcf): the two BGZF streams back to back, with the FASTQ
+    members' spans moved past the BCF bytes."""
+    b = _u8(bcf_compressed)
+    f = _u8(fastq_compressed)
+    sb = _spans_array(bcf_spans).copy()
+    sf = _spans_array(fastq_spans).copy()
+    for k in ("compressed_offset", "payload_start"):
+        sf[k] += np.uint64(b.size)
+    return np.concatenate([b, f]), np.concatenate([sb, sf]), int(sb.size)
+
+
+def decode_bcf_fastq_joined(context: "HipContext", both, spans, n_bcf: int) -> BcfFastqBatch:
+    """decode_bcf_fastq over streams already joined by join_bcf_fastq."""
+    a = _u8(both)
+    sp = _spans_array(spans)
+    offs = np.zeros(sp.size + 1, dtype=np.uint64)
+    arena, bh, fh = _vp(), _vp(), _vp()
+    base, start = _u64(), _u64()
+    ns, nc = C.c_uint32(), C.c_uint32()
+    e = _Err()
+    rc = lib().frhip_bcf_fastq_decode(context.handle, _ptr(a), a.size, _ptr(sp), n_bcf, sp.size,
+                                      C.byref(arena), _ptr(offs), C.byref(base), C.byref(start), C.byref(ns),
+                                      C.byref(nc), C.byref(bh), C.byref(fh), C.byref(e))
+    _raise(rc, e)
+    src = DeviceInflateBatch(DeviceBuffer(arena), offs, sp["compressed_offset"].copy())
+    return BcfFastqBatch(src, BcfRecordBatch(bh, src), FastqRecordBatch(fh, src), base.value, start.value,
+                         ns.value, nc.value)
 
 
 def bcf_parse_header(buf):

@@ -305,6 +305,30 @@ void frhip_bcf_batch_free(frhip_bcf_batch *b);
 int frhip_bcf_parse_header(const uint8_t *buf, uint64_t len, uint64_t *records_start, uint32_t *samples,
                            uint32_t *contigs, frhip_error *err);
 
+/* A BCF stream and a bgzipped FASTQ stream decoded together. No single
+   counterpart in the reference: it stands for one BcfScanner::decode
+   (crates/fritillaria-cuda/src/bcf.rs:94-143) plus one FastqScanner::decode
+   (crates/fritillaria-cuda/src/fastq.rs:94-135), each over its stream inflated
+   as one batch (crates/fritillaria-cuda/src/backend.rs:536-582).
+   batch[0..batch_len) holds both streams' compressed bytes; spans[0..n_bcf)
+   are the BCF members and spans[n_bcf..n) the FASTQ members (BGZF members are
+   independent, so the two files may simply be concatenated). All n members are
+   inflated in ONE launch of the persistent inflate and verified in member
+   order (status, then ISIZE, then CRC, as frhip_inflate_batch_device) into one
+   arena: the BCF bytes at 0, the FASTQ bytes from *fastq_base (the BCF size
+   rounded up to 4 KiB). offsets[n+1] receives each member's output end
+   (offsets[0] = 0). The BCF header is parsed from the inflated bytes
+   (*bcf_start, *samples, *contigs); then the BCF boundaries and columns from
+   *bcf_start and the FASTQ boundaries and columns from the FASTQ part's first
+   byte are found as by frhip_bcf_decode and frhip_fastq_decode, with their
+   errors (positions relative to each part). On success the caller owns
+   *arena, *bcf and *fastq; the columns index their part of the arena. */
+int frhip_bcf_fastq_decode(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                           const frhip_block_span *spans, uint64_t n_bcf, uint64_t n,
+                           frhip_buf **arena, uint64_t *offsets, uint64_t *fastq_base,
+                           uint64_t *bcf_start, uint32_t *samples, uint32_t *contigs,
+                           frhip_bcf_batch **bcf, frhip_fastq_batch **fastq, frhip_error *err);
+
 /* ---- DeviceBgzfReader (crates/fritillaria-bgzf/src/device_read.rs:47-280) ---- */
 
 /* A reader over a BGZF stream held in host memory [data, data+len) (borrowed for
