@@ -43,6 +43,9 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t *p)
 }
 __device__ __forceinline__ uint32_t ld16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
 
+// Four little-endian words at a 4-byte aligned address: one 16-byte load.
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+
 // One step of the chain from `cur` (a record start candidate); the walk stops
 // at `until`. Returns 0 advanced, 1 reached until, 2 ran out of buffer
 // (partial record: the tail), 3 malformed.
@@ -54,6 +57,25 @@ struct Step {
 __device__ __forceinline__ Step step(const uint8_t *buf, uint64_t len, uint64_t cur, uint64_t until)
 {
     if (cur >= until) return {cur, 1};
+    const uint64_t a4 = cur & ~3ull;
+    if (a4 + 32 <= len) {
+        // the fields the step reads: two 16-byte loads from the aligned word
+        // at or below the record, funnel-shifted (one wait per record, and a
+        // walk over members far apart touches two lines, not one per byte)
+        const u32x4_a4 *q = reinterpret_cast<const u32x4_a4 *>(buf + a4);
+        const u32x4_a4 v0 = q[0], v1 = q[1];
+        const uint32_t sh = (uint32_t)(cur & 3) * 8;
+        const uint32_t bs = __builtin_amdgcn_alignbit(v0.y, v0.x, sh);
+        if (bs < 32) return {cur, 3};
+        const uint64_t end = cur + 4 + (uint64_t)bs;
+        if (end > len) return {cur, 2};
+        const uint32_t name_len = __builtin_amdgcn_alignbit(v1.x, v0.w, sh) & 0xFF;
+        const uint64_t n_cigar = __builtin_amdgcn_alignbit(v1.y, v1.x, sh) & 0xFFFF;
+        const uint64_t l_seq = __builtin_amdgcn_alignbit(v1.z, v1.y, sh);
+        const uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
+        if (name_len == 0 || cur + need > end) return {cur, 3};
+        return {end, 0};
+    }
     if (cur + 24 <= len) {
         // the fields the step reads, loaded together (one wait per record)
         const uint8_t *r = buf + cur;
@@ -606,19 +628,37 @@ __global__ __launch_bounds__(256) void bam_decode_kernel(BamDecodeArgs a)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t r = a.c.record_offsets[i];
-        const uint8_t *p = a.buf + r;
-        uint32_t bs = ld32(p);
-        uint32_t name_len = p[12];
-        uint32_t nc = ld16(p + 16);
-        uint32_t ls = ld32(p + 20);
-        a.c.ref_id[i] = (int32_t)ld32(p + 4);
-        a.c.pos[i] = (int32_t)ld32(p + 8);
-        a.c.mapq[i] = p[13];
-        a.c.flags[i] = (uint16_t)ld16(p + 18);
+        // the record's 36 fixed bytes as nine words w[k] = bytes 4k..4k+3:
+        // records lie at any byte offset, so three 16-byte loads from the
+        // aligned word at or below the record, funnel-shifted (neighbouring
+        // records are hundreds of bytes apart, and one load per byte would
+        // touch a cache line per byte); near the end of the buffer, bytes
+        uint32_t w[9];
+        const uint64_t a4 = r & ~3ull;
+        if (a4 + 48 <= a.len) {
+            const u32x4_a4 *q = reinterpret_cast<const u32x4_a4 *>(a.buf + a4);
+            const u32x4_a4 v0 = q[0], v1 = q[1], v2 = q[2];
+            const uint32_t d[12] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+            const uint32_t sh = (uint32_t)(r & 3) * 8;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) w[k] = __builtin_amdgcn_alignbit(d[k + 1], d[k], sh);
+        } else {
+            const uint8_t *p = a.buf + r;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) w[k] = ld32(p + 4 * k);
+        }
+        const uint32_t bs = w[0];
+        const uint32_t name_len = w[3] & 0xFF;
+        const uint32_t nc = w[4] & 0xFFFF;
+        const uint32_t ls = w[5];
+        a.c.ref_id[i] = (int32_t)w[1];
+        a.c.pos[i] = (int32_t)w[2];
+        a.c.mapq[i] = (uint8_t)(w[3] >> 8);
+        a.c.flags[i] = (uint16_t)(w[4] >> 16);
         a.c.l_seq[i] = ls;
-        a.c.next_ref_id[i] = (int32_t)ld32(p + 24);
-        a.c.next_pos[i] = (int32_t)ld32(p + 28);
-        a.c.tlen[i] = (int32_t)ld32(p + 32);
+        a.c.next_ref_id[i] = (int32_t)w[6];
+        a.c.next_pos[i] = (int32_t)w[7];
+        a.c.tlen[i] = (int32_t)w[8];
         uint32_t c = 36u + name_len;
         uint32_t s = c + 4u * nc;
         uint32_t q = s + (ls + 1u) / 2u;

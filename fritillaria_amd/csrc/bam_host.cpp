@@ -165,6 +165,7 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     }
     BamDecodeArgs da{};
     da.buf = sa.buf;
+    da.len = len;
     da.n = n;
     da.n_dev = nullptr;
     da.c.ref_id = (int32_t *)b->cols[0]->ptr;

@@ -152,6 +152,7 @@ struct BamColumns {
 
 struct BamDecodeArgs {
     const uint8_t *buf;
+    uint64_t len; // bytes of buf that may be read (records lie inside it)
     uint64_t n;
     const uint64_t *n_dev; // device-side record count (batched form), or null
     BamColumns c;

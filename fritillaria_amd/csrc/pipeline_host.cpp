@@ -211,6 +211,7 @@ hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
     if (r == hipSuccess) r = launch_bam_emit(ea, s);
     BamDecodeArgs da{};
     da.buf = p->d_u;
+    da.len = p->U;
     da.n = 0;
     da.n_dev = &p->d_res->totals[0];
     da.c.ref_id = (int32_t *)p->d_cols[0];

@@ -406,6 +406,7 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
     if (r == hipSuccess) r = launch_bam_emit(ea, st);
     BamDecodeArgs da{};
     da.buf = du;
+    da.len = total;
     da.n = 0;
     da.n_dev = seg_base + ns;
     da.c.ref_id = (int32_t *)dc[0];
