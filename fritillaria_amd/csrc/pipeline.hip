@@ -72,28 +72,38 @@ constexpr uint32_t SIEVE_TILE = 16384;
 constexpr uint32_t SIEVE_THREADS = 256;
 constexpr uint32_t SIEVE_CHUNK = 16; // bytes per thread per pass over a 4 KiB stretch
 
-// Bit 7 of each byte of w set where that byte is 0x1f (exact, no carries
-// between bytes).
-__device__ __forceinline__ uint32_t bytes_eq_1f(uint32_t w)
+// Bit 7 of each byte of w set where that byte equals the byte of x4 (x4 holds
+// it four times; exact, no carries between bytes).
+__device__ __forceinline__ uint32_t bytes_eq(uint32_t w, uint32_t x4)
 {
-    const uint32_t t = w ^ 0x1f1f1f1fu;
+    const uint32_t t = w ^ x4;
     return ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t | 0x7f7f7f7fu);
 }
-__device__ __forceinline__ uint32_t nibble_1f(uint32_t w)
+// Offsets k of a 16-byte chunk (words v) where the gzip magic 1f 8b starts, as
+// a 16-bit mask: each word against 0x1f, the same word shifted one byte (the
+// bytes after, via a funnel shift) against 0x8b. The chunk's last byte keeps a
+// 0x1f alone (its successor is in the next chunk): a superset, header_at
+// decides.
+__device__ __forceinline__ uint32_t magic_nibbles(const uint4 v)
 {
-    const uint32_t m = bytes_eq_1f(w);
-    return ((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u);
+    auto nib = [](uint32_t w, uint32_t after) {
+        const uint32_t m = bytes_eq(w, 0x1f1f1f1fu) & bytes_eq(after, 0x8b8b8b8bu);
+        return ((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u);
+    };
+    return nib(v.x, __builtin_amdgcn_alignbit(v.y, v.x, 8)) | (nib(v.y, __builtin_amdgcn_alignbit(v.z, v.y, 8)) << 4) |
+           (nib(v.z, __builtin_amdgcn_alignbit(v.w, v.z, 8)) << 8) |
+           (nib(v.w, __builtin_amdgcn_alignbit(0x8b8b8b8bu, v.w, 8)) << 12);
 }
 
-// Offsets p .. p+15 (p a multiple of 16) holding the header's first byte
-// 0x1f, as a 16-bit mask: one aligned 16-byte load (bytes past the end of the
-// file are read one at a time). Only these offsets go through header_at.
+// Offsets p .. p+15 (p a multiple of 16) where a header may start, as a 16-bit
+// mask: one aligned 16-byte load through magic_nibbles (bytes past the end of
+// the file are read one at a time and checked for 0x1f). Only these offsets go
+// through header_at.
 __device__ __forceinline__ uint32_t magic_mask(const uint8_t *f, uint64_t len, uint64_t p)
 {
     if (p >= len) return 0;
     if (p + 16 <= len) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(f + p);
-        return nibble_1f(v.x) | (nibble_1f(v.y) << 4) | (nibble_1f(v.z) << 8) | (nibble_1f(v.w) << 12);
+        return magic_nibbles(*reinterpret_cast<const uint4 *>(f + p));
     }
     uint32_t m = 0;
     for (uint32_t j = 0; p + j < len && j < 16; ++j) m |= (uint32_t)(f[p + j] == 0x1f) << j;
@@ -112,8 +122,7 @@ __device__ __forceinline__ void tile_masks(const uint8_t *f, uint64_t len, uint6
 #pragma unroll
         for (uint32_t i = 0; i < NK; ++i) v[i] = *reinterpret_cast<const uint4 *>(f + p0 + i * SIEVE_THREADS * SIEVE_CHUNK);
 #pragma unroll
-        for (uint32_t i = 0; i < NK; ++i)
-            mk[i] = nibble_1f(v[i].x) | (nibble_1f(v[i].y) << 4) | (nibble_1f(v[i].z) << 8) | (nibble_1f(v[i].w) << 12);
+        for (uint32_t i = 0; i < NK; ++i) mk[i] = magic_nibbles(v[i]);
     } else {
 #pragma unroll
         for (uint32_t i = 0; i < NK; ++i) mk[i] = magic_mask(f, len, p0 + i * SIEVE_THREADS * SIEVE_CHUNK);
