@@ -46,6 +46,31 @@ __device__ __forceinline__ uint32_t ld16(const uint8_t *p) { return (uint32_t)p[
 // Four little-endian words at a 4-byte aligned address: one 16-byte load.
 typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
 
+// A record's 36 fixed bytes at cur (cur + 36 <= len) as nine little-endian
+// words w[k] = bytes 4k..4k+3: records lie at any byte offset, so three
+// 16-byte loads from the aligned word at or below cur, funnel-shifted (one
+// load per byte would touch a cache line per byte where records are far
+// apart, and costs ~36 instructions anyway); within 48 bytes of len, bytes.
+__device__ __forceinline__ void fixed_words(const uint8_t *buf, uint64_t len, uint64_t cur, uint32_t *w)
+{
+    const uint64_t a4 = cur & ~3ull;
+    if (a4 + 48 <= len) {
+        const u32x4_a4 *q = reinterpret_cast<const u32x4_a4 *>(buf + a4);
+        const u32x4_a4 v0 = q[0], v1 = q[1], v2 = q[2];
+        const uint32_t d[12] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+        const uint32_t sh = (uint32_t)(cur & 3) * 8;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = __builtin_amdgcn_alignbit(d[k + 1], d[k], sh);
+    } else {
+        const uint8_t *p = buf + cur;
+        uint32_t h[36];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) h[i] = p[i];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w[k] = h[4 * k] | h[4 * k + 1] << 8 | h[4 * k + 2] << 16 | h[4 * k + 3] << 24;
+    }
+}
+
 // One step of the chain from `cur` (a record start candidate); the walk stops
 // at `until`. Returns 0 advanced, 1 reached until, 2 ran out of buffer
 // (partial record: the tail), 3 malformed.
@@ -120,25 +145,18 @@ __device__ __forceinline__ Step cand_step(const uint8_t *buf, uint64_t len, uint
     // the 36 fixed bytes, loaded together (every check fails alike, so none
     // needs the others' loads to wait)
     const uint8_t *r = buf + cur;
-    uint32_t h[36];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) h[i] = r[i];
-    asm volatile("" ::"v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(h[3]), "v"(h[4]), "v"(h[5]), "v"(h[6]), "v"(h[7]),
-                 "v"(h[8]), "v"(h[9]), "v"(h[10]), "v"(h[11]), "v"(h[12]), "v"(h[13]), "v"(h[14]), "v"(h[15]),
-                 "v"(h[16]), "v"(h[17]), "v"(h[18]), "v"(h[19]), "v"(h[20]), "v"(h[21]), "v"(h[22]), "v"(h[23]));
-    asm volatile("" ::"v"(h[24]), "v"(h[25]), "v"(h[26]), "v"(h[27]), "v"(h[28]), "v"(h[29]), "v"(h[30]),
-                 "v"(h[31]), "v"(h[32]), "v"(h[33]), "v"(h[34]), "v"(h[35]));
-    auto w32 = [&](int o) { return h[o] | h[o + 1] << 8 | h[o + 2] << 16 | h[o + 3] << 24; };
-    const uint32_t bs = w32(0);
+    uint32_t w[9];
+    fixed_words(buf, len, cur, w);
+    const uint32_t bs = w[0];
     if (bs < 32) return {cur, 3};
     const uint64_t end = cur + 4 + (uint64_t)bs;
     if (end > len) return {cur, 3};
-    const uint32_t name_len = h[12];
-    const uint64_t n_cigar = h[16] | h[17] << 8;
-    const uint64_t l_seq = w32(20);
+    const uint32_t name_len = w[3] & 0xFF;
+    const uint64_t n_cigar = w[4] & 0xFFFF;
+    const uint64_t l_seq = w[5];
     const uint64_t need = 36ull + name_len + 4ull * n_cigar + (l_seq + 1) / 2 + l_seq;
     if (name_len == 0 || cur + need > end) return {cur, 3};
-    if ((int32_t)w32(4) < -1 || (int32_t)w32(8) < -1 || (int32_t)w32(24) < -1 || (int32_t)w32(28) < -1)
+    if ((int32_t)w[1] < -1 || (int32_t)w[2] < -1 || (int32_t)w[6] < -1 || (int32_t)w[7] < -1)
         return {cur, 3};
     // a QNAME is at least one printable character, then the NUL (all inside
     // the record, so inside the buffer)
@@ -628,25 +646,8 @@ __global__ __launch_bounds__(256) void bam_decode_kernel(BamDecodeArgs a)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t r = a.c.record_offsets[i];
-        // the record's 36 fixed bytes as nine words w[k] = bytes 4k..4k+3:
-        // records lie at any byte offset, so three 16-byte loads from the
-        // aligned word at or below the record, funnel-shifted (neighbouring
-        // records are hundreds of bytes apart, and one load per byte would
-        // touch a cache line per byte); near the end of the buffer, bytes
         uint32_t w[9];
-        const uint64_t a4 = r & ~3ull;
-        if (a4 + 48 <= a.len) {
-            const u32x4_a4 *q = reinterpret_cast<const u32x4_a4 *>(a.buf + a4);
-            const u32x4_a4 v0 = q[0], v1 = q[1], v2 = q[2];
-            const uint32_t d[12] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
-            const uint32_t sh = (uint32_t)(r & 3) * 8;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) w[k] = __builtin_amdgcn_alignbit(d[k + 1], d[k], sh);
-        } else {
-            const uint8_t *p = a.buf + r;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) w[k] = ld32(p + 4 * k);
-        }
+        fixed_words(a.buf, a.len, r, w);
         const uint32_t bs = w[0];
         const uint32_t name_len = w[3] & 0xFF;
         const uint32_t nc = w[4] & 0xFFFF;
