@@ -337,9 +337,9 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
 // bam_spec_kernel), from its landing to its end, one lane per member: the
 // walk is a chain of dependent loads, and a wave keeps 64 of them in flight.
 template <uint32_t F>
-__global__ __launch_bounds__(256) void bam_walk_kernel(BamSpecArgs a)
+__global__ __launch_bounds__(64) void bam_walk_kernel(BamSpecArgs a)
 {
-    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t b = blockIdx.x * 64 + threadIdx.x;
     if (b >= a.nb || a.spec_ok[b] != 1) return;
     const uint64_t sb = a.block_starts[b];
     const uint64_t sb1 = a.mem_until ? a.mem_until[b] : a.block_starts[b + 1];
@@ -765,13 +765,14 @@ hipError_t launch_bcf_decode(const BcfDecodeArgs &a, hipStream_t s)
 hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 {
     if (a.nb == 0) return hipSuccess;
-    const uint32_t wgrid = (a.nb + 255) / 256;
+    // one-wave workgroups: the walks are latency-bound chains, spread over every CU
+    const uint32_t wgrid = (a.nb + 63) / 64;
     if (a.format == CHAIN_BCF) {
         hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BCF>, dim3(a.nb), dim3(64), 0, s, a);
-        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BCF>, dim3(wgrid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BCF>, dim3(wgrid), dim3(64), 0, s, a);
     } else {
         hipLaunchKernelGGL(bam_spec_kernel<CHAIN_BAM>, dim3(a.nb), dim3(64), 0, s, a);
-        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BAM>, dim3(wgrid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(bam_walk_kernel<CHAIN_BAM>, dim3(wgrid), dim3(64), 0, s, a);
     }
     return hipGetLastError();
 }
