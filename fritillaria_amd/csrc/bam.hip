@@ -349,14 +349,29 @@ __global__ __launch_bounds__(64) void bam_walk_kernel(BamSpecArgs a)
     int complete = a.complete[b], valid = 1;
     uint16_t *rel = a.rel_offs + (uint64_t)b * BAM_CAPB;
     if (complete) {
+        // The offsets go out eight at a time as one 16-byte store (the slab
+        // is 16-byte aligned): a store in every step would make every step's
+        // load wait for it too (the memory counter covers both). q0:q1 is a
+        // shift register of eight u16, the newest at the top.
+        uint64_t q0 = 0, q1 = 0;
         for (;;) {
             Step s = chain_step<F>(a.buf, len, cur, sb1);
             if (s.kind == 1) break;
             if (s.kind == 2) { complete = 0; break; }
             if (s.kind == 3) { valid = 0; break; }
-            rel[cnt] = (uint16_t)(cur - sb);
+            q0 = (q0 >> 16) | (q1 << 48);
+            q1 = (q1 >> 16) | ((uint64_t)(uint16_t)(cur - sb) << 48);
             ++cnt;
+            if ((cnt & 7) == 0)
+                *reinterpret_cast<uint4 *>(rel + cnt - 8) =
+                    make_uint4((uint32_t)q0, (uint32_t)(q0 >> 32), (uint32_t)q1, (uint32_t)(q1 >> 32));
             cur = s.next;
+        }
+        // the last cnt % 8 offsets sit in the top slots of the register
+        const uint32_t k = cnt & 7;
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t slot = 8 - k + j;
+            rel[cnt - k + j] = (uint16_t)(slot < 4 ? q0 >> (16 * slot) : q1 >> (16 * (slot - 4)));
         }
     }
     a.spec_ok[b] = (uint8_t)valid;

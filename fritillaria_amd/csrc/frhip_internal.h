@@ -49,7 +49,9 @@ uint32_t inflate_waves_per_cu(int schedule);
 // Record-boundary discovery for BAM over an inflated batch.
 constexpr uint32_t BAM_K = 8;                 // survivors kept per block
 constexpr uint64_t BAM_NONE = ~0ull;          // block has no entry
-constexpr uint32_t BAM_CAPB = 65536 / 32 + 1; // most records that can start in one member (BCF's are >= 32 B)
+// most records that can start in one member (BCF's are >= 32 B), rounded up
+// to a multiple of 8 so that each member's u16 offsets start 16-byte aligned
+constexpr uint32_t BAM_CAPB = (65536 / 32 + 1 + 7) & ~7u;
 constexpr uint32_t BAM_MODE_FALLBACK = 255;
 
 // Length-prefixed record chains the boundary scan handles.
