@@ -208,7 +208,7 @@ def main():
     t0 = time.time()
     s = synth.config_bam(2, n_records=args.records)
     C = int(s.compressed.size)
-    log(f"generated config 2: {args.records} records, {C} compressed / {s.inflated_len} inflated bytes "
+    log(f"generated the batched large BAM: {args.records} records, {C} compressed / {s.inflated_len} inflated bytes "
         f"in {time.time() - t0:.1f}s")
 
     ctx = fa.HipContext(0)
@@ -298,7 +298,7 @@ def main():
         cdt = time.perf_counter() - t0
         assert rc == 0
         cpu = {"value": r["inflated"] / cdt / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
-               "sample": f"whole config-2 file ({args.records} records, {r['inflated']} inflated bytes), "
+               "sample": f"whole batched large BAM ({args.records} records, {r['inflated']} inflated bytes), "
                          f"noodles-structured reader with zlib 1.2.11, {cdt:.2f} s",
                "records_match": r["records"] == n_rec}
         log(f"cpu baseline: {cpu['value']:.2f} GB/s on {cores} threads ({cdt:.2f} s)")

@@ -58,7 +58,7 @@ def run_pair(hip_ctx, oracle_mod, bcf_raw: bytes, fq_raw: bytes):
 
 @pytest.mark.parametrize("two_phase", [False, True])
 def test_synthetic_bcf_and_fastq_together(hip_ctx, codec, oracle_mod, two_phase):
-    """The config-4 shapes (2,504 diploid samples, GT only; 150 bp FASTQ), scaled
+    """The mixed BCF + FASTQ shapes (2,504 diploid samples, GT only; 150 bp reads), scaled
     down, on the one-wave inflate and on the two-phase inflate; the columns are
     also those of each stream decoded alone by its own scanner."""
     print(f"[gpu] bcf+fastq together, two_phase={two_phase}", flush=True)
