@@ -6,8 +6,16 @@ an indel in the CIGAR with p = 0.01, NM/RG aux), framed the way noodles writes
 it (65,495-byte member payloads, no record-boundary flush), members at mixed
 zlib levels 1/6/9 with 1% stored, EOF member. The compressed file is resident
 in HBM when the timed region starts; a step is member discovery, inflate with
-CRC32/ISIZE verification, BAM record boundaries and the SoA columns, replayed
-as one hipGraph.
+CRC32/ISIZE verification, BAM record boundaries, the 14 reference columns with
+virtual offsets, and the SoA extras (resolved CIGAR ops, packed 4-bit
+sequence, aux tag index), replayed as one hipGraph.
+
+Parity is enforced, not just reported: the batched run's record count and
+column checksums (positions, flags, sequence lengths, record offsets, CIGAR
+ops, packed sequence bytes, aux entries) must equal those of the CPU reader
+over the same file, the inflated bytes left in HBM must match every member's
+trailer CRC32, the per-call path must find the same records, and no served
+query may fail; otherwise the bench exits non-zero before printing a line.
 
 Beside it: the plain per-call path (DeviceBgzfReader + BamDecoder, 256 members
 per batch, host-fed like the reference's drop-in) over the same file, the
@@ -30,6 +38,17 @@ HBM_PEAK_GBPS = 8000.0
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class ParityError(SystemExit):
+    pass
+
+
+def require(ok, what):
+    """Exits non-zero (before any metric line is printed) when a parity check fails."""
+    if not ok:
+        log(f"PARITY FAILURE: {what}")
+        raise ParityError(3)
 
 
 def measured_traffic(records, schedule, order):
@@ -62,6 +81,7 @@ def serve_pass(fa, ctx, compressed, voffs, n_requests, log):
     burst = sv.make_requests(voffs, co, n_requests, rate_hz=1e9, seed=0x5EED0005)
     st = sv.serve(server, burst, budget)
     cap = st.summary()
+    require(cap["failures"] == 0, f"{cap['failures']} served queries failed at capacity")
     bytes_per_req = st.inflated_bytes / max(1, len(burst))
     req_per_s = len(burst) / st.makespan_s
     log(f"serving capacity: {cap['throughput_gbps']} GB/s, {req_per_s:.1f} requests/s "
@@ -72,6 +92,7 @@ def serve_pass(fa, ctx, compressed, voffs, n_requests, log):
     for load in (0.3, 0.6, 0.9):
         reqs = sv.make_requests(voffs, co, n_requests, rate_hz=load * req_per_s, seed=0x5EED0005 + int(load * 10))
         r = sv.serve(server, reqs, budget).summary()
+        require(r["failures"] == 0, f"{r['failures']} served queries failed at {load:.0%} load")
         r["rate_hz"] = round(load * req_per_s, 2)
         out["loads"][str(load)] = r
         log(f"serving at {load:.0%} load: p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, {r['throughput_gbps']} GB/s, "
@@ -79,9 +100,32 @@ def serve_pass(fa, ctx, compressed, voffs, n_requests, log):
     server.close()
     reqs = sv.make_requests(voffs, co, max(20, n_requests // 4), rate_hz=0.6 * req_per_s, seed=0x5EED0005 + 6)
     plain = sv.serve_plain(ctx, compressed, reqs, co).summary()
+    require(plain["failures"] == 0, f"{plain['failures']} plain per-call queries failed")
     plain["rate_hz"] = round(0.6 * req_per_s, 2)
     out["plain_per_call_at_0.6"] = plain
     log(f"plain per-call at 60% of batched capacity: p50 {plain['p50_ms']} ms, p99 {plain['p99_ms']} ms")
+    return out
+
+
+def column_checksums(pipe, np):
+    """Record count and sums (mod 2^64) of the batched run's columns, the same
+    quantities oracle.mt_bam_columns(soa=True) reports."""
+    m = (1 << 64) - 1
+    out = {"records": pipe.records()}
+    out["sum_pos"] = int(pipe.column_to_host("position").astype(np.int64).sum()) & m
+    out["sum_flag"] = int(pipe.column_to_host("flags").astype(np.uint64).sum())
+    out["sum_lseq"] = int(pipe.column_to_host("sequence_len").astype(np.uint64).sum())
+    out["sum_off"] = int(pipe.column_to_host("record_offsets").sum(dtype=np.uint64))
+    soa = {}
+    for name in ("cigar_offsets", "sequence_offsets", "aux_offsets"):
+        soa[name] = int(pipe.soa_column(name).to_vec().view(np.uint64)[-1])
+    out["ops"] = soa["cigar_offsets"]
+    out["sum_ops"] = int(pipe.soa_column("cigar_ops").to_vec().view(np.uint32).sum(dtype=np.uint64))
+    out["seq_bytes"] = soa["sequence_offsets"]
+    out["sum_seq"] = int(pipe.soa_column("sequence_packed").to_vec().sum(dtype=np.uint64))
+    out["aux"] = soa["aux_offsets"]
+    aux = pipe.soa_column("aux_entries").to_vec().view(np.uint32).reshape(-1, 4)
+    out["sum_aux_value_offset"] = int(aux[:, 1].sum(dtype=np.uint64))
     return out
 
 
@@ -221,7 +265,7 @@ def main():
         log(f"autotune picked {tuned['best']} in {tuned['seconds']:.1f}s")
     t0 = time.time()
     pipe = fa.BatchedPipeline(ctx, s.compressed, use_graph=not args.no_graph, waves_per_cu=args.waves_per_cu,
-                              order=args.order, bam_window=args.window, inflate_schedule=args.schedule)
+                              order=args.order, bam_window=args.window, inflate_schedule=args.schedule, soa=True)
     U = pipe.inflated_len()
     n_members = pipe.members()
     log(f"pipeline ready ({n_members} members, device discovery {pipe.device_discovery()}) in {time.time() - t0:.1f}s")
@@ -246,15 +290,15 @@ def main():
     crc_written = fa.HipCodec(ctx).crc32_blocks(pipe.inflated(), offs[:-1], np.diff(offs).astype(np.uint32))
     bad_members = int(np.count_nonzero(crc_written != spans["crc32"][: len(crc_written)]))
     log(f"inflated bytes in HBM re-checked: {bad_members} member CRC mismatches")
-    pos = pipe.column_to_host("position")
-    lseq = pipe.column_to_host("sequence_len")
-    batched_sig = (n_rec, int(pos.astype(np.int64).sum()), int(lseq.astype(np.int64).sum()))
+    require(bad_members == 0 and len(crc_written) == len(spans), "inflated bytes in HBM differ from the member CRCs")
+    batched_sig = column_checksums(pipe, np)
+    log(f"batched column checksums: {batched_sig}")
     voffs = pipe.column_to_host("virtual_offsets") if not args.no_serve else None
-    del pos, lseq, spans, crc_written
+    del spans, crc_written
 
     # inflate-kernel roofline: one run without a graph, HIP events around the launch
     timed = fa.BatchedPipeline(ctx, s.compressed, use_graph=False, waves_per_cu=args.waves_per_cu,
-                               order=args.order, bam_window=args.window, inflate_schedule=args.schedule)
+                               order=args.order, bam_window=args.window, inflate_schedule=args.schedule, soa=True)
     stage_runs = []
     for i in range(3):
         timed.run()
@@ -266,7 +310,10 @@ def main():
     alg_bytes = payload_bytes + U
     achieved = alg_bytes / (inflate_ms / 1e3) / 1e9
     log(f"stages (ms): {json.dumps({k: round(v, 3) for k, v in stages.items()})}")
-    k_out = 71 * n_rec
+    # column bytes written per step: 55 B/record of reference columns (record
+    # offsets included), 8 B of virtual offset, 3 x 8 B of SoA offsets, 4 B per
+    # CIGAR op, the packed sequence, 16 B per aux entry
+    k_out = (55 + 8 + 24) * n_rec + 4 * batched_sig["ops"] + batched_sig["seq_bytes"] + 16 * batched_sig["aux"]
     pipe_bytes = C + U + k_out
 
     per_call = None
@@ -282,24 +329,31 @@ def main():
         per_call = {"gbps": U / pdt / 1e9, "seconds": pdt, "records": pc_rec, "batches": pc_batches,
                     "blocks_per_batch": 256, "host_fed": True, "matches_batched_records": pc_rec == n_rec}
         log(f"per-call: {pdt:.2f} s, {per_call['gbps']:.2f} GB/s, {pc_rec} records in {pc_batches} batches")
+        require(pc_rec == n_rec, f"per-call path found {pc_rec} records, batched {n_rec}")
 
     serving = None
     if not args.no_serve:
         serving = serve_pass(fa, ctx, s.compressed, voffs, args.serve_requests, log)
         del voffs
 
+    # the CPU reader over the same file: the timed baseline, and the checksums
+    # the batched columns must equal
+    import oracle
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = max(1, min(cores, 64))
+    t0 = time.perf_counter()
+    rc, r = oracle.mt_bam_columns(s.compressed, cores, soa=True)
+    cdt = time.perf_counter() - t0
+    require(rc == 0, f"CPU reader failed with {rc}")
+    mism = [k for k in batched_sig if batched_sig[k] != r[k]]
+    require(not mism, f"batched columns differ from the CPU reader's in {mism}: {batched_sig} vs {r}")
+    log(f"batched columns equal the CPU reader's checksums ({len(batched_sig)} sums)")
     cpu = None
     if not args.no_cpu:
-        import oracle
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        cores = max(1, min(cores, 64))
-        t0 = time.perf_counter()
-        rc, r = oracle.mt_bam_columns(s.compressed, cores)
-        cdt = time.perf_counter() - t0
-        assert rc == 0
         cpu = {"value": r["inflated"] / cdt / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
                "sample": f"whole batched large BAM ({args.records} records, {r['inflated']} inflated bytes), "
-                         f"noodles-structured reader with zlib 1.2.11, {cdt:.2f} s",
+                         f"noodles-structured reader with zlib 1.2.11 building the same columns and SoA extras, "
+                         f"{cdt:.2f} s",
                "records_match": r["records"] == n_rec}
         log(f"cpu baseline: {cpu['value']:.2f} GB/s on {cores} threads ({cdt:.2f} s)")
 
@@ -335,7 +389,8 @@ def main():
         "config": {
             "workload": "batched large BAM: 20M records, read length 100-250, indel p=0.01, noodles framing, "
                         "members at levels 1/6/9 + 1% stored; discovery + inflate + verify + boundaries + "
-                        "14 reference columns + virtual offsets, one hipGraph per step",
+                        "14 reference columns + virtual offsets + SoA extras (resolved CIGAR ops, 4-bit "
+                        "packed sequence, aux tag index), one hipGraph per step",
             "records": n_rec, "members": n_members, "inflated_bytes": U, "compressed_bytes": C,
             "graph": not args.no_graph, "inflate_schedule": args.schedule,
             "inflate_variant": variant,
@@ -357,8 +412,8 @@ def main():
         "p99_latency_ms": serving["loads"]["0.6"]["p99_ms"] if serving else None,
         "p50_latency_ms": serving["loads"]["0.6"]["p50_ms"] if serving else None,
         "serving": serving,
-        "batched_signature": list(batched_sig),
-        "inflated_crc_recheck_mismatches": bad_members,
+        "parity": {"column_checksums": batched_sig, "equal_to_cpu_reader": True,
+                   "inflated_crc_recheck_mismatches": bad_members},
     }
     print(json.dumps(out), flush=True)
 

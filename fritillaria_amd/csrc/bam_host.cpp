@@ -416,40 +416,38 @@ extern "C" int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const 
         *out = o;
         return FRHIP_OK;
     }
-    // the three offset columns are outputs; the per-record counts live in scratch
+    // the three offset columns are outputs; per-record counts and tile sums live in scratch
     for (int c : {FRHIP_SOA_CIGAR_OFFSETS, FRHIP_SOA_SEQUENCE_OFFSETS, FRHIP_SOA_AUX_OFFSETS}) {
         o->cols[c] = buf_alloc(ctx, (n + 1) * 8, err);
         if (!o->cols[c]) return fail(err ? err->kind : FRHIP_E_DEVICE);
     }
     int rc;
-    const uint64_t cnt_bytes = ((n * 4 + 255) / 256) * 256;
-    uint8_t *s = (uint8_t *)ctx->soa_scratch.get(4 * cnt_bytes + 256, err, &rc);
+    const SoaScratchLayout L = soa_scratch_layout(n);
+    uint8_t *s = (uint8_t *)ctx->soa_scratch.get(L.bytes, err, &rc);
     if (!s) return fail(rc);
     SoaArgs a{};
+    soa_bind_scratch(a, s, L);
     a.buf = (const uint8_t *)data->ptr;
-    a.record_offsets = (const uint64_t *)records->cols[FRHIP_BAM_RECORD_OFFSETS]->ptr;
     a.n = n;
-    a.n_ops = (uint32_t *)s;
-    a.seq_bytes = (uint32_t *)(s + cnt_bytes);
-    a.n_aux = (uint32_t *)(s + 2 * cnt_bytes);
-    a.cg_at = (uint32_t *)(s + 3 * cnt_bytes);
-    a.first_bad = (unsigned long long *)(s + 4 * cnt_bytes);
-    a.op_off = (const uint64_t *)o->cols[FRHIP_SOA_CIGAR_OFFSETS]->ptr;
-    a.seq_off = (const uint64_t *)o->cols[FRHIP_SOA_SEQUENCE_OFFSETS]->ptr;
-    a.aux_off = (const uint64_t *)o->cols[FRHIP_SOA_AUX_OFFSETS]->ptr;
+    a.cap_rec = n;
+    a.cap_ops = a.cap_seq = a.cap_aux = 0;
+    soa_bind_columns(a, (const uint64_t *)records->cols[FRHIP_BAM_RECORD_OFFSETS]->ptr,
+                     (const uint32_t *)records->cols[FRHIP_BAM_SEQUENCE_LEN]->ptr,
+                     (const uint32_t *)records->cols[FRHIP_BAM_CIGAR_START]->ptr,
+                     (const uint32_t *)records->cols[FRHIP_BAM_SEQUENCE_START]->ptr,
+                     (const uint32_t *)records->cols[FRHIP_BAM_AUX_START]->ptr,
+                     (const uint32_t *)records->cols[FRHIP_BAM_RECORD_END]->ptr);
+    a.op_off = (uint64_t *)o->cols[FRHIP_SOA_CIGAR_OFFSETS]->ptr;
+    a.seq_off = (uint64_t *)o->cols[FRHIP_SOA_SEQUENCE_OFFSETS]->ptr;
+    a.aux_off = (uint64_t *)o->cols[FRHIP_SOA_AUX_OFFSETS]->ptr;
     hipError_t r = launch_fill_u64((uint64_t *)a.first_bad, ~0ull, 1, st);
     if (r == hipSuccess) r = launch_soa_count(a, st);
-    if (r == hipSuccess) r = launch_scan_u32_u64(a.n_ops, (uint64_t *)a.op_off, nullptr, n, st);
-    if (r == hipSuccess) r = launch_scan_u32_u64(a.seq_bytes, (uint64_t *)a.seq_off, nullptr, n, st);
-    if (r == hipSuccess) r = launch_scan_u32_u64(a.n_aux, (uint64_t *)a.aux_off, nullptr, n, st);
-    uint64_t tot[3];
-    unsigned long long bad = ~0ull;
-    if (r == hipSuccess) r = hipMemcpyAsync(&tot[0], a.op_off + n, 8, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(&tot[1], a.seq_off + n, 8, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(&tot[2], a.aux_off + n, 8, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(&bad, a.first_bad, 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = launch_soa_scan(a, st);
+    uint64_t tot[SOA_N_RESULT + 1];
+    if (r == hipSuccess) r = hipMemcpyAsync(tot, a.result, sizeof(tot), hipMemcpyDeviceToHost, st);
     if (r == hipSuccess) r = hipStreamSynchronize(st);
     if (r != hipSuccess) return fail(hip_err(err, r, "SoA counts"));
+    const uint64_t bad = tot[SOA_N_RESULT];
     if (bad != ~0ull) {
         uint64_t at = 0;
         (void)hipMemcpy(&at, a.record_offsets + bad, 8, hipMemcpyDeviceToHost);
@@ -459,7 +457,7 @@ extern "C" int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const 
         if (err) std::strcpy(err->format, "bam");
         return fail(k);
     }
-    const uint64_t sizes[3] = {tot[0] * 4, tot[1], tot[2] * 16};
+    const uint64_t sizes[3] = {tot[SOA_TOT_OPS] * 4, tot[SOA_TOT_SEQ], tot[SOA_TOT_AUX] * 16};
     const int data_cols[3] = {FRHIP_SOA_CIGAR_OPS, FRHIP_SOA_SEQUENCE_PACKED, FRHIP_SOA_AUX_ENTRIES};
     for (int k = 0; k < 3; ++k) {
         o->cols[data_cols[k]] = buf_alloc(ctx, sizes[k], err);
@@ -468,6 +466,9 @@ extern "C" int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const 
     a.ops = (uint32_t *)o->cols[FRHIP_SOA_CIGAR_OPS]->ptr;
     a.seq = (uint8_t *)o->cols[FRHIP_SOA_SEQUENCE_PACKED]->ptr;
     a.aux = (AuxEntry *)o->cols[FRHIP_SOA_AUX_ENTRIES]->ptr;
+    a.cap_ops = tot[SOA_TOT_OPS];
+    a.cap_seq = tot[SOA_TOT_SEQ];
+    a.cap_aux = tot[SOA_TOT_AUX];
     r = launch_soa_fill(a, st);
     if (r == hipSuccess) r = hipStreamSynchronize(st);
     if (r != hipSuccess) return fail(hip_err(err, r, "SoA fill"));

@@ -20,12 +20,14 @@
 #include "frhip_internal.h"
 #include "host_internal.h"
 #include "pipeline_internal.h"
+#include "soa_internal.h"
 
 using namespace frhip;
 
 namespace {
 
 const int WIDTHS[FRHIP_BAM_N_COLUMNS] = {4, 4, 2, 1, 4, 4, 4, 4, 8, 4, 4, 4, 4, 4, 8};
+const int SOA_WIDTHS[FRHIP_SOA_N_COLUMNS] = {8, 4, 8, 1, 8, 16};
 
 // Small block read back once per run.
 struct ResultBlock {
@@ -34,9 +36,11 @@ struct ResultBlock {
     uint64_t hdr_status;     // BAM header walk
     uint64_t records_start;
     uint64_t totals[4];      // records, tail, status, fallbacks
+    uint64_t soa[SOA_N_RESULT + 1]; // SoA totals, overflow, first malformed record
 };
 
-enum { EV_START, EV_DISC, EV_INFL, EV_VERIFY, EV_SPEC, EV_REC, EV_END, N_EV };
+enum { EV_START, EV_DISC, EV_INFL, EV_VERIFY, EV_SPEC, EV_REC, EV_COLS, EV_END, N_EV };
+constexpr int N_STAGES = N_EV; // the stages between events, then the total
 
 } // namespace
 
@@ -68,15 +72,22 @@ struct frhip_pipeline {
     ResultBlock *d_res = nullptr;
     ResultBlock res{};
     ResultBlock *h_res = nullptr; // pinned
+    // SoA extras (cfg.soa): sized from the records of a run at create
+    bool soa = false;
+    uint8_t *d_soa_scratch = nullptr;
+    void *d_soa_cols[FRHIP_SOA_N_COLUMNS] = {};
+    uint64_t soa_cap_rec = 0, soa_cap[3] = {};
     // views
     frhip_buf u_view{};
     frhip_buf col_view[FRHIP_BAM_N_COLUMNS] = {};
+    frhip_buf soa_view[FRHIP_SOA_N_COLUMNS] = {};
     // graph and timing
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[N_EV] = {};
     bool timed = false;
-    double stage_ms[7] = {};
+    bool last_ok = false;
+    double stage_ms[N_STAGES] = {};
     std::vector<void *> allocs;
 };
 
@@ -108,6 +119,45 @@ hipError_t enqueue_discovery(frhip_pipeline *p, hipStream_t s)
                          p->cand_cap, p->m, (uint64_t *)&p->d_res->chain_bad, s);
     if (r == hipSuccess)
         r = launch_scan_u32_u64(p->m.isize, p->m.out_off, p->d_tile_base + p->n_tiles, 0, s);
+    return r;
+}
+
+SoaArgs soa_args(frhip_pipeline *p)
+{
+    SoaArgs a{};
+    soa_bind_scratch(a, p->d_soa_scratch, soa_scratch_layout(p->soa_cap_rec));
+    a.result = p->d_res->soa;
+    a.first_bad = (unsigned long long *)&p->d_res->soa[SOA_N_RESULT];
+    a.buf = p->d_u;
+    a.n = p->soa_cap_rec;
+    a.n_dev = &p->d_res->totals[0];
+    a.cap_rec = p->soa_cap_rec;
+    soa_bind_columns(a, (const uint64_t *)p->d_cols[FRHIP_BAM_RECORD_OFFSETS],
+                     (const uint32_t *)p->d_cols[FRHIP_BAM_SEQUENCE_LEN],
+                     (const uint32_t *)p->d_cols[FRHIP_BAM_CIGAR_START],
+                     (const uint32_t *)p->d_cols[FRHIP_BAM_SEQUENCE_START],
+                     (const uint32_t *)p->d_cols[FRHIP_BAM_AUX_START],
+                     (const uint32_t *)p->d_cols[FRHIP_BAM_RECORD_END]);
+    a.cap_ops = p->soa_cap[0];
+    a.cap_seq = p->soa_cap[1];
+    a.cap_aux = p->soa_cap[2];
+    a.op_off = (uint64_t *)p->d_soa_cols[FRHIP_SOA_CIGAR_OFFSETS];
+    a.seq_off = (uint64_t *)p->d_soa_cols[FRHIP_SOA_SEQUENCE_OFFSETS];
+    a.aux_off = (uint64_t *)p->d_soa_cols[FRHIP_SOA_AUX_OFFSETS];
+    a.ops = (uint32_t *)p->d_soa_cols[FRHIP_SOA_CIGAR_OPS];
+    a.seq = (uint8_t *)p->d_soa_cols[FRHIP_SOA_SEQUENCE_PACKED];
+    a.aux = (AuxEntry *)p->d_soa_cols[FRHIP_SOA_AUX_ENTRIES];
+    return a;
+}
+
+// The SoA counts and their prefix; with `fill`, the offsets and data too.
+hipError_t enqueue_soa(frhip_pipeline *p, hipStream_t s, bool fill)
+{
+    const SoaArgs a = soa_args(p);
+    hipError_t r = launch_fill_u64((uint64_t *)a.first_bad, ~0ull, 1, s);
+    if (r == hipSuccess) r = launch_soa_count(a, s);
+    if (r == hipSuccess) r = launch_soa_scan(a, s);
+    if (r == hipSuccess && fill) r = launch_soa_fill(a, s);
     return r;
 }
 
@@ -229,11 +279,19 @@ hipError_t enqueue_run(frhip_pipeline *p, hipStream_t s, bool timed)
     da.c.aux_start = (uint32_t *)p->d_cols[12];
     da.c.record_end = (uint32_t *)p->d_cols[13];
     if (r == hipSuccess) r = launch_bam_decode(da, p->cap_rec, s);
+    rec(EV_COLS);
+    if (r == hipSuccess && p->soa && p->d_soa_scratch) r = enqueue_soa(p, s, true);
     rec(EV_END);
     return r;
 }
 
-int check_result(frhip_pipeline *p, frhip_error *err)
+void clear_views(frhip_pipeline *p)
+{
+    for (auto &v : p->col_view) v.len = 0;
+    for (auto &v : p->soa_view) v.len = 0;
+}
+
+int check_result_inner(frhip_pipeline *p, frhip_error *err)
 {
     const ResultBlock &R = p->res;
     if (p->device_discovery && R.chain_bad)
@@ -276,8 +334,40 @@ int check_result(frhip_pipeline *p, frhip_error *err)
         }
         return k;
     }
-    for (int c = 0; c < FRHIP_BAM_N_COLUMNS; ++c) p->col_view[c].len = R.totals[0] * WIDTHS[c];
+    if (p->soa) {
+        const uint64_t *S = R.soa;
+        if (S[SOA_N_RESULT] != ~0ull) {
+            uint64_t at = 0;
+            (void)hipMemcpy(&at, (const uint64_t *)p->d_cols[FRHIP_BAM_RECORD_OFFSETS] + S[SOA_N_RESULT], 8,
+                            hipMemcpyDeviceToHost);
+            int k = set_err(err, FRHIP_E_MALFORMED, at,
+                            "record %llu: malformed aux field, or a long-CIGAR placeholder without CG:B:I",
+                            (unsigned long long)S[SOA_N_RESULT]);
+            if (err) std::strcpy(err->format, "bam");
+            return k;
+        }
+        if (S[SOA_OVERFLOW] || S[SOA_TOT_OPS] > p->soa_cap[0] || S[SOA_TOT_SEQ] > p->soa_cap[1] ||
+            S[SOA_TOT_AUX] > p->soa_cap[2])
+            return set_err(err, FRHIP_E_DEVICE, 0, "SoA columns outgrew the sizes found at create");
+    }
+    const uint64_t n = R.totals[0];
+    for (int c = 0; c < FRHIP_BAM_N_COLUMNS; ++c) p->col_view[c].len = n * WIDTHS[c];
+    if (p->soa) {
+        const uint64_t lens[FRHIP_SOA_N_COLUMNS] = {(n + 1) * 8, R.soa[SOA_TOT_OPS] * 4, (n + 1) * 8,
+                                                    R.soa[SOA_TOT_SEQ], (n + 1) * 8, R.soa[SOA_TOT_AUX] * 16};
+        for (int c = 0; c < FRHIP_SOA_N_COLUMNS; ++c) p->soa_view[c].len = lens[c];
+    }
     return FRHIP_OK;
+}
+
+// A failed run leaves every column view empty (the reference leaves a failed
+// batch's output empty: crates/fritillaria-cuda/tests/device_resident.rs:208-236).
+int check_result(frhip_pipeline *p, frhip_error *err)
+{
+    clear_views(p);
+    const int rc = check_result_inner(p, err);
+    p->last_ok = rc == FRHIP_OK;
+    return rc;
 }
 
 } // namespace
@@ -446,6 +536,36 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
         r = hipEventCreate(&e);
         if (r != hipSuccess) return fail(hip_err(err, r, "hipEventCreate"));
     }
+    p->soa = p->cfg.soa != 0;
+    if (p->soa && p->n > 0) {
+        // One run learns the record count, and the SoA counts of those records
+        // the totals the three data columns are sized for; the resident file
+        // gives the same counts on every later run.
+        r = enqueue_run(p, s, false);
+        if (r == hipSuccess) r = hipMemcpyAsync(p->h_res, p->d_res, sizeof(ResultBlock), hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipStreamSynchronize(s);
+        if (r != hipSuccess) return fail(hip_err(err, r, "sizing run"));
+        p->soa_cap_rec = std::min<uint64_t>(p->h_res->totals[0], p->cap_rec);
+        if ((rc = dalloc(p, (void **)&p->d_soa_scratch, soa_scratch_layout(p->soa_cap_rec).bytes, err,
+                         "SoA scratch")))
+            return fail(rc);
+        for (int c : {FRHIP_SOA_CIGAR_OFFSETS, FRHIP_SOA_SEQUENCE_OFFSETS, FRHIP_SOA_AUX_OFFSETS})
+            if ((rc = dalloc(p, &p->d_soa_cols[c], (p->soa_cap_rec + 1) * 8, err, "SoA offsets"))) return fail(rc);
+        r = enqueue_soa(p, s, false);
+        if (r == hipSuccess) r = hipMemcpyAsync(p->h_res, p->d_res, sizeof(ResultBlock), hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipStreamSynchronize(s);
+        if (r != hipSuccess) return fail(hip_err(err, r, "sizing the SoA columns"));
+        const uint64_t *S = p->h_res->soa;
+        p->soa_cap[0] = S[SOA_TOT_OPS];
+        p->soa_cap[1] = S[SOA_TOT_SEQ];
+        p->soa_cap[2] = S[SOA_TOT_AUX];
+        const int data_cols[3] = {FRHIP_SOA_CIGAR_OPS, FRHIP_SOA_SEQUENCE_PACKED, FRHIP_SOA_AUX_ENTRIES};
+        for (int k = 0; k < 3; ++k)
+            if ((rc = dalloc(p, &p->d_soa_cols[data_cols[k]], p->soa_cap[k] * SOA_WIDTHS[data_cols[k]], err,
+                             "SoA columns")))
+                return fail(rc);
+        for (int c = 0; c < FRHIP_SOA_N_COLUMNS; ++c) p->soa_view[c] = frhip_buf{p->d_soa_cols[c], 0, ctx->ordinal};
+    }
     if (p->cfg.use_graph && p->n > 0) {
         r = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (r == hipSuccess) {
@@ -498,6 +618,8 @@ extern "C" int frhip_pipeline_finish(frhip_pipeline *p, frhip_error *err)
     if (p->n == 0) {
         p->res = ResultBlock{};
         p->res.first_bad = ~0ull;
+        p->last_ok = false;
+        clear_views(p);
         return set_err(err, FRHIP_E_MALFORMED, 0, "no BGZF members");
     }
     hipError_t r = hipStreamSynchronize(p->ctx->stream);
@@ -505,12 +627,12 @@ extern "C" int frhip_pipeline_finish(frhip_pipeline *p, frhip_error *err)
     p->res = *p->h_res;
     if (p->timed) {
         float ms = 0;
-        for (int i = 0; i < 6; ++i) {
+        for (int i = 0; i + 1 < N_EV; ++i) {
             (void)hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]);
             p->stage_ms[i] = ms;
         }
         (void)hipEventElapsedTime(&ms, p->ev[EV_START], p->ev[EV_END]);
-        p->stage_ms[6] = ms;
+        p->stage_ms[N_STAGES - 1] = ms;
     }
     return check_result(p, err);
 }
@@ -525,7 +647,7 @@ extern "C" int frhip_pipeline_run(frhip_pipeline *p, frhip_error *err)
 extern "C" uint64_t frhip_pipeline_members(const frhip_pipeline *p) { return p->n; }
 extern "C" uint64_t frhip_pipeline_inflated_len(const frhip_pipeline *p) { return p->U; }
 extern "C" uint64_t frhip_pipeline_compressed_len(const frhip_pipeline *p) { return p->file_len; }
-extern "C" uint64_t frhip_pipeline_records(const frhip_pipeline *p) { return p->res.totals[0]; }
+extern "C" uint64_t frhip_pipeline_records(const frhip_pipeline *p) { return p->last_ok ? p->res.totals[0] : 0; }
 extern "C" uint64_t frhip_pipeline_records_start(const frhip_pipeline *p) { return p->res.records_start; }
 extern "C" uint64_t frhip_pipeline_tail(const frhip_pipeline *p) { return p->res.totals[1]; }
 extern "C" uint64_t frhip_pipeline_fallbacks(const frhip_pipeline *p) { return p->res.totals[3]; }
@@ -535,6 +657,12 @@ extern "C" const frhip_buf *frhip_pipeline_column(const frhip_pipeline *p, int c
 {
     if (column < 0 || column >= FRHIP_BAM_N_COLUMNS) return nullptr;
     return &p->col_view[column];
+}
+
+extern "C" const frhip_buf *frhip_pipeline_soa_column(const frhip_pipeline *p, int column)
+{
+    if (!p->soa || column < 0 || column >= FRHIP_SOA_N_COLUMNS) return nullptr;
+    return &p->soa_view[column];
 }
 
 extern "C" int frhip_pipeline_member_offsets(const frhip_pipeline *p, uint64_t *offsets, uint64_t *coffsets,
@@ -549,7 +677,7 @@ extern "C" int frhip_pipeline_member_offsets(const frhip_pipeline *p, uint64_t *
 
 extern "C" int frhip_pipeline_stage_ms(const frhip_pipeline *p, double *ms, int n)
 {
-    int k = n < 7 ? n : 7;
+    int k = n < N_STAGES ? n : N_STAGES;
     for (int i = 0; i < k; ++i) ms[i] = p->timed ? p->stage_ms[i] : -1.0;
     return k;
 }

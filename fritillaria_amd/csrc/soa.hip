@@ -5,9 +5,20 @@
 // (crates/fritillaria-bam/src/columnar/record.rs:220-361, aux.rs): cigar_raw /
 // cigar_resolved (the long-CIGAR placeholder "<l_seq>S <n>N" replaced by the
 // CG:B:I array), sequence_packed, and the aux field walk of aux.rs. Here they
-// become device columns: a count pass (one thread per record), three prefix
-// sums, and a fill pass that copies ops and sequence bytes and writes one
-// 16-byte index entry per aux field.
+// become device columns in three launches, each sized by the record count
+// alone so that the batched form captures them in its graph:
+//
+//   count  one thread per record, in tiles of 256: op and aux-field counts,
+//          the CG:B:I source of a placeholder, and each tile's three sums;
+//   scan   one workgroup: exclusive prefix of the tile sums and the totals;
+//   fill   the same tiles again: a workgroup prefix of its records' counts
+//          plus the tile's base gives every offset, then each thread copies
+//          its record's ops and packed sequence (aligned 4-byte stores,
+//          source words funnel-shifted) and writes its 16-byte aux entries.
+//
+// A record's field bounds come from the decoded columns (cigar_start,
+// seq_start, aux_start, record_end, l_seq: coalesced 4-byte loads), not from
+// its header again.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -16,11 +27,19 @@
 namespace frhip {
 namespace {
 
-__device__ __forceinline__ uint32_t ld32(const uint8_t *p)
+static_assert(SOA_TILE == 256, "the workgroup scans below assume 4 waves of 64");
+
+// The aligned word at or below p, and the next one only when p is not
+// aligned: neither reaches past the last byte of [p, p + 4).
+__device__ __forceinline__ uint32_t ldu32(const uint8_t *p)
 {
-    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t sh = (uint32_t)(a & 3);
+    const uint32_t *w = (const uint32_t *)(a - sh);
+    const uint32_t lo = w[0];
+    if (!sh) return lo;
+    return __builtin_amdgcn_alignbyte(w[1], lo, sh);
 }
-__device__ __forceinline__ uint32_t ld16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
 
 __device__ __forceinline__ bool is_hex(uint8_t c)
 {
@@ -58,7 +77,7 @@ __device__ uint64_t aux_field(const uint8_t *r, uint64_t p, uint64_t end, AuxEnt
         const uint32_t w = sub == 'c' || sub == 'C' ? 1 : sub == 's' || sub == 'S' ? 2
                          : sub == 'i' || sub == 'I' || sub == 'f' ? 4 : 0;
         if (!w) return 0;
-        count = ld32(r + v + 1);
+        count = ldu32(r + v + 1);
         v += 5;
         size = count * w;
         break;
@@ -76,91 +95,244 @@ __device__ uint64_t aux_field(const uint8_t *r, uint64_t p, uint64_t end, AuxEnt
     return v + size;
 }
 
-struct Shape {
-    uint64_t c0, s0, a0, end;
-    uint32_t nc, ls;
-};
-
-__device__ __forceinline__ Shape shape(const uint8_t *r)
+// Record::has_long_cigar_placeholder (record.rs:316-327): two ops, the first
+// <l_seq>S, the second an N.
+__device__ __forceinline__ bool placeholder(const uint8_t *r, uint32_t c0, uint32_t nc, uint32_t ls)
 {
-    Shape s;
-    s.end = 4 + (uint64_t)ld32(r);
-    s.nc = ld16(r + 16);
-    s.ls = ld32(r + 20);
-    s.c0 = 36 + (uint64_t)r[12];
-    s.s0 = s.c0 + 4ull * s.nc;
-    s.a0 = s.s0 + (s.ls + 1) / 2 + s.ls;
-    return s;
+    if (nc != 2) return false;
+    const uint32_t o0 = ldu32(r + c0), o1 = ldu32(r + c0 + 4);
+    return (o0 & 15) == 4 && (o0 >> 4) == ls && (o1 & 15) == 3;
 }
 
-// Record::has_long_cigar_placeholder (record.rs:316-327)
-__device__ __forceinline__ bool placeholder(const uint8_t *r, const Shape &s)
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 {
-    if (s.nc != 2) return false;
-    const uint32_t o0 = ld32(r + s.c0), o1 = ld32(r + s.c0 + 4);
-    return (o0 & 15) == 4 && (o0 >> 4) == s.ls && (o1 & 15) == 3;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, uint32_t lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t u = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t record_count(const SoaArgs &a)
+{
+    const uint64_t n = a.n_dev ? *a.n_dev : a.n;
+    return n < a.cap_rec ? n : a.cap_rec;
 }
 
 __global__ __launch_bounds__(256) void soa_count_kernel(SoaArgs a)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint8_t *r = a.buf + a.record_offsets[i];
-        const Shape s = shape(r);
-        uint32_t k = 0, cg = 0, cg_count = 0, cg_ok = 0;
-        bool ok = true;
-        for (uint64_t p = s.a0; p < s.end;) {
-            AuxEntry e;
-            const uint64_t nx = aux_field(r, p, s.end, e);
-            if (!nx) {
-                ok = false;
-                break;
+    __shared__ uint64_t red[3][4];
+    const uint64_t n = record_count(a);
+    const uint64_t tiles = soa_tiles(n);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t i = t * SOA_TILE + threadIdx.x;
+        uint32_t ops = 0, sb = 0, k = 0;
+        if (i < n) {
+            const uint8_t *r = a.buf + a.record_offsets[i];
+            const uint32_t c0 = a.cigar_start[i], s0 = a.seq_start[i], a0 = a.aux_start[i], end = a.record_end[i];
+            const uint32_t ls = a.l_seq[i];
+            const uint32_t nc = (s0 - c0) >> 2;
+            uint32_t cg = 0, cg_count = 0, cg_ok = 0;
+            bool ok = true;
+            for (uint64_t p = a0; p < end;) {
+                AuxEntry e;
+                const uint64_t nx = aux_field(r, p, end, e);
+                if (!nx) {
+                    ok = false;
+                    break;
+                }
+                if (!cg && e.tag[0] == 'C' && e.tag[1] == 'G') {
+                    cg = e.value_offset;
+                    cg_count = e.count;
+                    cg_ok = e.type == 'B' && e.subtype == 'I';
+                }
+                ++k;
+                p = nx;
             }
-            if (!cg && e.tag[0] == 'C' && e.tag[1] == 'G') {
-                cg = e.value_offset;
-                cg_count = e.count;
-                cg_ok = e.type == 'B' && e.subtype == 'I';
-            }
-            ++k;
-            p = nx;
+            const bool ph = placeholder(r, c0, nc, ls);
+            if (ph && !(cg && cg_ok)) ok = false;
+            // a malformed record copies nothing: its counts stay inside the record
+            ops = ph ? (ok ? cg_count : 0) : nc;
+            sb = (ls + 1) / 2;
+            a.n_ops[i] = ops;
+            a.n_aux[i] = k;
+            a.cg_at[i] = ph && ok ? cg : 0;
+            if (!ok) atomicMin(a.first_bad, (unsigned long long)i);
         }
-        const bool ph = placeholder(r, s);
-        if (ph && (!cg || !cg_ok)) ok = false;
-        a.n_aux[i] = k;
-        a.n_ops[i] = ph ? cg_count : s.nc;
-        a.cg_at[i] = ph ? cg : 0;
-        a.seq_bytes[i] = (s.ls + 1) / 2;
-        if (!ok) atomicMin(a.first_bad, (unsigned long long)i);
+        const uint64_t s0 = wave_sum(ops), s1 = wave_sum(sb), s2 = wave_sum(k);
+        if (lane == 0) {
+            red[0][wave] = s0;
+            red[1][wave] = s1;
+            red[2][wave] = s2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const uint32_t c = threadIdx.x;
+            a.tile_sums[3 * t + c] = red[c][0] + red[c][1] + red[c][2] + red[c][3];
+        }
+        __syncthreads();
     }
+}
+
+// Exclusive prefix of the per-tile sums by one workgroup of 1024 threads, four
+// tiles per thread per step; the totals and the record overflow go to result.
+__global__ __launch_bounds__(1024) void soa_scan_kernel(SoaArgs a)
+{
+    __shared__ uint64_t wsum[3][16];
+    __shared__ uint64_t carry[3];
+    const uint64_t n_all = a.n_dev ? *a.n_dev : a.n;
+    const uint64_t n = record_count(a);
+    const uint64_t tiles = soa_tiles(n);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 3) carry[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < tiles; base += 1024 * 4) {
+        const uint64_t t0 = base + (uint64_t)threadIdx.x * 4;
+        uint64_t v[4][3], s[3] = {0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[j][c] = t0 + j < tiles ? a.tile_sums[3 * (t0 + j) + c] : 0;
+                s[c] += v[j][c];
+            }
+        uint64_t incl[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) incl[c] = wave_incl_scan(s[c], lane);
+        if (lane == 63)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) wsum[c][wave] = incl[c];
+        __syncthreads();
+        uint64_t run[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            uint64_t pre = carry[c];
+            for (uint32_t w = 0; w < wave; ++w) pre += wsum[c][w];
+            run[c] = pre + incl[c] - s[c];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (t0 + j < tiles) a.tile_base[3 * (t0 + j) + c] = run[c];
+                run[c] += v[j][c];
+            }
+        __syncthreads();
+        if (threadIdx.x == 1023)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[c] = run[c];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.result[SOA_TOT_OPS] = carry[0];
+        a.result[SOA_TOT_SEQ] = carry[1];
+        a.result[SOA_TOT_AUX] = carry[2];
+        a.result[SOA_OVERFLOW] = n_all > a.cap_rec ? 1 : 0;
+    }
+}
+
+// dst[0, len) = src[0, len): head bytes up to dst's 4-byte alignment, aligned
+// 4-byte stores of funnel-shifted source words, tail bytes. No source word
+// read lies past the last source byte's word.
+__device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uint32_t len)
+{
+    uint32_t head = (uint32_t)((4 - ((uintptr_t)dst & 3)) & 3);
+    if (head > len) head = len;
+    for (uint32_t j = 0; j < head; ++j) dst[j] = src[j];
+    const uint32_t nw = (len - head) >> 2;
+    if (nw) {
+        const uintptr_t sa = (uintptr_t)(src + head);
+        const uint32_t sh = (uint32_t)(sa & 3);
+        const uint32_t *sw = (const uint32_t *)(sa - sh);
+        uint32_t *dw = (uint32_t *)(dst + head);
+        if (sh == 0) {
+            for (uint32_t k = 0; k < nw; ++k) dw[k] = sw[k];
+        } else {
+            uint32_t lo = sw[0];
+            for (uint32_t k = 0; k < nw; ++k) {
+                const uint32_t hi = sw[k + 1];
+                dw[k] = __builtin_amdgcn_alignbyte(hi, lo, sh);
+                lo = hi;
+            }
+        }
+    }
+    for (uint32_t j = head + 4 * nw; j < len; ++j) dst[j] = src[j];
 }
 
 __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    __shared__ uint64_t wsum[3][4];
+    const uint64_t n_all = a.n_dev ? *a.n_dev : a.n;
+    if (n_all > a.cap_rec) return;
+    const uint64_t n = n_all;
+    const uint64_t *R = a.result;
+    const bool fits = R[SOA_TOT_OPS] <= a.cap_ops && R[SOA_TOT_SEQ] <= a.cap_seq && R[SOA_TOT_AUX] <= a.cap_aux;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.op_off[n] = R[SOA_TOT_OPS];
+        a.seq_off[n] = R[SOA_TOT_SEQ];
+        a.aux_off[n] = R[SOA_TOT_AUX];
+    }
+    const uint64_t tiles = soa_tiles(n);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint64_t i = t * SOA_TILE + threadIdx.x;
+        uint32_t ops = 0, sb = 0, k = 0;
+        if (i < n) {
+            ops = a.n_ops[i];
+            sb = (a.l_seq[i] + 1) / 2;
+            k = a.n_aux[i];
+        }
+        const uint64_t i0 = wave_incl_scan(ops, lane), i1 = wave_incl_scan(sb, lane), i2 = wave_incl_scan(k, lane);
+        if (lane == 63) {
+            wsum[0][wave] = i0;
+            wsum[1][wave] = i1;
+            wsum[2][wave] = i2;
+        }
+        __syncthreads();
+        uint64_t o0 = a.tile_base[3 * t] + i0 - ops, o1 = a.tile_base[3 * t + 1] + i1 - sb,
+                 o2 = a.tile_base[3 * t + 2] + i2 - k;
+        for (uint32_t w = 0; w < wave; ++w) {
+            o0 += wsum[0][w];
+            o1 += wsum[1][w];
+            o2 += wsum[2][w];
+        }
+        __syncthreads();
+        if (i >= n) continue;
+        a.op_off[i] = o0;
+        a.seq_off[i] = o1;
+        a.aux_off[i] = o2;
+        if (!fits) continue;
         const uint8_t *r = a.buf + a.record_offsets[i];
-        const Shape s = shape(r);
-        // CIGAR: inline ops, or the CG array the count pass found
-        const uint64_t src = a.cg_at[i] ? a.cg_at[i] : s.c0;
-        uint32_t *ops = a.ops + a.op_off[i];
-        const uint32_t nops = (uint32_t)(a.op_off[i + 1] - a.op_off[i]);
-        for (uint32_t j = 0; j < nops; ++j) ops[j] = ld32(r + src + 4ull * j);
-        uint8_t *seq = a.seq + a.seq_off[i];
-        const uint32_t nb = (s.ls + 1) / 2;
-        for (uint32_t j = 0; j < nb; ++j) seq[j] = r[s.s0 + j];
-        AuxEntry *aux = a.aux + a.aux_off[i];
-        uint32_t k = 0;
-        for (uint64_t p = s.a0; p < s.end;) {
+        // CIGAR: the inline ops, or the CG:B:I array the count pass found
+        const uint32_t cg = a.cg_at[i];
+        const uint32_t src = cg ? cg : a.cigar_start[i];
+        copy_bytes((uint8_t *)(a.ops + o0), r + src, 4 * ops);
+        copy_bytes(a.seq + o1, r + a.seq_start[i], sb);
+        const uint32_t end = a.record_end[i];
+        AuxEntry *aux = a.aux + o2;
+        uint32_t m = 0;
+        for (uint64_t p = a.aux_start[i]; p < end && m < k;) {
             AuxEntry e;
-            const uint64_t nx = aux_field(r, p, s.end, e);
+            const uint64_t nx = aux_field(r, p, end, e);
             if (!nx) break;
-            aux[k++] = e;
+            aux[m++] = e;
             p = nx;
         }
     }
 }
 
-uint32_t grid_for(uint64_t n)
+uint32_t grid_for_tiles(uint64_t n)
 {
-    uint64_t g = (n + 255) / 256;
+    uint64_t g = soa_tiles(n);
     if (g > 8192) g = 8192;
     return g ? (uint32_t)g : 1;
 }
@@ -169,15 +341,22 @@ uint32_t grid_for(uint64_t n)
 
 hipError_t launch_soa_count(const SoaArgs &a, hipStream_t s)
 {
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(soa_count_kernel, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+    const uint64_t n = a.n < a.cap_rec ? a.n : a.cap_rec;
+    if (n == 0 && !a.n_dev) return hipSuccess;
+    hipLaunchKernelGGL(soa_count_kernel, dim3(grid_for_tiles(n)), dim3(SOA_TILE), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_scan(const SoaArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(soa_scan_kernel, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_soa_fill(const SoaArgs &a, hipStream_t s)
 {
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(soa_fill_kernel, dim3(grid_for(a.n)), dim3(256), 0, s, a);
+    const uint64_t n = a.n < a.cap_rec ? a.n : a.cap_rec;
+    hipLaunchKernelGGL(soa_fill_kernel, dim3(grid_for_tiles(n)), dim3(SOA_TILE), 0, s, a);
     return hipGetLastError();
 }
 

@@ -200,6 +200,7 @@ def lib():
         "frhip_pipeline_device_discovery": (C.c_int, [_vp]),
         "frhip_pipeline_inflated": (_vp, [_vp]),
         "frhip_pipeline_column": (_vp, [_vp, C.c_int]),
+        "frhip_pipeline_soa_column": (_vp, [_vp, C.c_int]),
         "frhip_pipeline_member_offsets": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_pipeline_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int]),
         "frhip_server_create": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
@@ -976,21 +977,22 @@ def select_codec(backend: Backend = Backend.AUTO):
 
 class PipelineConfig(C.Structure):
     _fields_ = [("waves_per_cu", C.c_uint32), ("order", C.c_uint32), ("bam_window", C.c_uint32),
-                ("use_graph", C.c_uint32), ("host_discovery", C.c_uint32), ("inflate_schedule", C.c_int32)]
+                ("use_graph", C.c_uint32), ("host_discovery", C.c_uint32), ("inflate_schedule", C.c_int32),
+                ("soa", C.c_uint32)]
 
 
-STAGES = ["discovery", "inflate", "verify", "speculate", "reconcile", "columns", "total"]
+STAGES = ["discovery", "inflate", "verify", "speculate", "reconcile", "columns", "soa", "total"]
 
 
 class BatchedPipeline:
     """A whole BGZF BAM resident in HBM, decoded to columns in one enqueue."""
 
     def __init__(self, context: HipContext, file_bytes, *, waves_per_cu=0, order=0, bam_window=0,
-                 use_graph=False, host_discovery=False, inflate_schedule=-1):
+                 use_graph=False, host_discovery=False, inflate_schedule=-1, soa=False):
         self.ctx = context
         self._file = _u8(file_bytes)
         cfg = PipelineConfig(waves_per_cu, order, bam_window, 1 if use_graph else 0, 1 if host_discovery else 0,
-                             inflate_schedule)
+                             inflate_schedule, 1 if soa else 0)
         h = _vp()
         e = _Err()
         rc = lib().frhip_pipeline_create(context.handle, _ptr(self._file), self._file.size, C.byref(cfg),
@@ -1049,6 +1051,21 @@ class BatchedPipeline:
 
     def columns_to_host(self) -> dict:
         return {name: self.column_to_host(name) for name, _ in BAM_COLUMNS}
+
+    def soa_column(self, name: str) -> Optional[DeviceBuffer]:
+        """SoA extras of the last run (pipelines created with soa=True)."""
+        h = lib().frhip_pipeline_soa_column(self._h, _SOA_INDEX[name])
+        return None if not h else DeviceBuffer(h, owned=False, keepalive=self)
+
+    def soa_to_host(self) -> dict:
+        out = {}
+        for name, dt in SOA_COLUMNS:
+            buf = self.soa_column(name)
+            dt = np.dtype(dt)
+            a = np.empty(buf.byte_len() // dt.itemsize, dtype=dt)
+            buf.copy_to_host(a)
+            out[name] = a
+        return out
 
     def member_offsets(self):
         n = self.members()

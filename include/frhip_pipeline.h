@@ -32,6 +32,8 @@ typedef struct {
     uint32_t use_graph;        /* 1: capture the whole run once and replay it as one hipGraph */
     uint32_t host_discovery;   /* 1: walk member headers on the host once at create; 0: sieve on the device every run */
     int32_t inflate_schedule;  /* inflate kernel variant (frhip_inflate_schedules); -1 = the context's */
+    uint32_t soa;              /* 1: every run also builds the SoA extras of frhip_bam_soa_build (resolved
+                                  CIGAR ops, packed 4-bit sequence, aux index), sized at create from one run */
 } frhip_pipeline_config;
 
 /* Uploads file[0..len) to HBM once and sizes every buffer for it. Runs member
@@ -63,11 +65,18 @@ int frhip_pipeline_device_discovery(const frhip_pipeline *p);
    next run or free. */
 const frhip_buf *frhip_pipeline_inflated(const frhip_pipeline *p);
 const frhip_buf *frhip_pipeline_column(const frhip_pipeline *p, int column);
+/* With cfg.soa: the SoA column of the last run, in frhip_soa_column order and
+   with the layout frhip_bam_soa_build gives (offset columns records+1 entries);
+   NULL without cfg.soa. Views owned by the pipeline. After a failed run every
+   column view (these and frhip_pipeline_column's) is empty and
+   frhip_pipeline_records is 0. */
+const frhip_buf *frhip_pipeline_soa_column(const frhip_pipeline *p, int column);
 /* Member boundaries (members+1 entries) and compressed offsets (members entries). */
 int frhip_pipeline_member_offsets(const frhip_pipeline *p, uint64_t *offsets, uint64_t *coffsets, frhip_error *err);
 /* Per-stage device times (ms) of the last run without a graph:
    [0] discovery [1] inflate [2] verify [3] boundary speculation [4] reconcile
-   [5] offsets + columns [6] total. Returns the number of stages written. */
+   [5] offsets + columns [6] SoA extras [7] total. Returns the number of stages
+   written. */
 int frhip_pipeline_stage_ms(const frhip_pipeline *p, double *ms, int n);
 
 #ifdef __cplusplus

@@ -97,7 +97,7 @@ def lib():
         _lib.or_bcf_validate.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         _lib.or_bam_soa.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_void_p] * 9 + [C.POINTER(C.c_uint64)]
         _lib.or_mt_bam_columns.restype = C.c_int
-        _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]
+        _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_void_p]
         del u8p
     return _lib
 
@@ -267,13 +267,20 @@ def virtual_offsets(block_offsets, coffsets, positions) -> np.ndarray:
     return out
 
 
-def mt_bam_columns(file_bytes, nthreads: int):
+def mt_bam_columns(file_bytes, nthreads: int, soa: bool = False):
+    """The noodles-structured multithreaded reader over a whole BGZF BAM: record
+    count and column checksums (sums mod 2^64); with soa, also the totals and
+    checksums of the resolved CIGAR ops, packed sequences and aux entries."""
     a = _as_u8(file_bytes)
-    res = np.zeros(7, dtype=np.uint64)
-    rc = lib().or_mt_bam_columns(_ptr(a), a.size, nthreads, _ptr(res))
-    return rc, {"records": int(res[0]), "inflated": int(res[1]), "sum_pos": int(res[2]),
-                "sum_flag": int(res[3]), "sum_lseq": int(res[4]), "sum_off": int(res[5]),
-                "members": int(res[6])}
+    res = np.zeros(13, dtype=np.uint64)
+    rc = lib().or_mt_bam_columns(_ptr(a), a.size, nthreads, 1 if soa else 0, _ptr(res))
+    out = {"records": int(res[0]), "inflated": int(res[1]), "sum_pos": int(res[2]),
+           "sum_flag": int(res[3]), "sum_lseq": int(res[4]), "sum_off": int(res[5]),
+           "members": int(res[6])}
+    if soa:
+        out.update({"ops": int(res[7]), "sum_ops": int(res[8]), "seq_bytes": int(res[9]),
+                    "sum_seq": int(res[10]), "aux": int(res[11]), "sum_aux_value_offset": int(res[12])})
+    return rc, out
 
 
 # ---------------------------------------------------------------------------

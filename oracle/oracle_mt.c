@@ -83,9 +83,28 @@ static int wait_upto(mt_ctx_t *c, uint64_t *ready_members, uint64_t upto)
  * Decodes a whole BGZF BAM held in memory with `nthreads` inflate workers.
  * Fills res[0]=records, res[1]=inflated bytes, res[2]=sum(pos), res[3]=sum(flag),
  * res[4]=sum(l_seq), res[5]=sum(record offsets), res[6]=members.
- * Returns 0, or <0 on a framing / inflate / record error.
+ * With soa != 0 every record's resolved CIGAR ops, packed sequence and aux
+ * entries are also copied into growing columns (or_bam_soa per record, the
+ * accessors of crates/fritillaria-bam/src/columnar/record.rs:220-361), and
+ * res[7]=total ops, res[8]=sum of op words, res[9]=packed sequence bytes,
+ * res[10]=sum of those bytes, res[11]=aux entries, res[12]=sum of their value
+ * offsets. res must hold 13 words.
+ * Returns 0, or <0 on a framing / inflate / record / aux error.
  */
-int or_mt_bam_columns(const uint8_t *file, uint64_t len, int nthreads, uint64_t *res)
+/* Grows *p (elements of `size` bytes) to hold at least `need` elements. */
+static int grow(void **p, uint64_t *cap, uint64_t need, size_t size)
+{
+    if (need <= *cap) return 0;
+    uint64_t c = *cap ? *cap : 1024;
+    while (c < need) c *= 2;
+    void *q = realloc(*p, c * size);
+    if (!q) return -1;
+    *p = q;
+    *cap = c;
+    return 0;
+}
+
+int or_mt_bam_columns(const uint8_t *file, uint64_t len, int nthreads, int soa, uint64_t *res)
 {
     uint64_t cap = len / 28 + 2;
     or_span_t *spans = (or_span_t *)malloc(sizeof(or_span_t) * cap);
@@ -117,6 +136,12 @@ int or_mt_bam_columns(const uint8_t *file, uint64_t len, int nthreads, uint64_t 
     int32_t *c_nref = (int32_t *)malloc(4 * col_cap), *c_npos = (int32_t *)malloc(4 * col_cap);
     int32_t *c_tlen = (int32_t *)malloc(4 * col_cap);
     uint64_t *c_off = (uint64_t *)malloc(8 * col_cap);
+    /* SoA extras: columns grown as records arrive */
+    uint32_t *s_ops = NULL;
+    uint8_t *s_seq = NULL;
+    or_aux_entry_t *s_aux = NULL;
+    uint64_t cap_ops = 0, cap_seq = 0, cap_aux = 0, n_ops_tot = 0, n_seq_tot = 0, n_aux_tot = 0;
+    uint64_t sum_ops = 0, sum_seq = 0, sum_auxoff = 0;
     if (wait_upto(&c, &ready, 12) != 0) { rc = -2; goto join; }
     {
         uint64_t need = 12;
@@ -153,6 +178,28 @@ int or_mt_bam_columns(const uint8_t *file, uint64_t len, int nthreads, uint64_t 
             sum_flag += c_flag[recs];
             sum_lseq += c_lseq[recs];
             sum_off += pos;
+            if (soa) {
+                /* a record of bs bytes holds at most bs/4 ops or aux fields and bs sequence bytes */
+                if (grow((void **)&s_ops, &cap_ops, n_ops_tot + bs / 4 + 1, 4) ||
+                    grow((void **)&s_seq, &cap_seq, n_seq_tot + bs + 1, 1) ||
+                    grow((void **)&s_aux, &cap_aux, n_aux_tot + bs / 4 + 1, sizeof(or_aux_entry_t))) {
+                    rc = -5;
+                    goto join;
+                }
+                uint32_t k_ops, k_seq, k_aux;
+                uint64_t bad = 0, at = pos;
+                if (or_bam_soa(out, &at, 1, &k_ops, &k_seq, &k_aux, &n_ops_tot, s_ops, &n_seq_tot, s_seq,
+                               &n_aux_tot, s_aux, &bad) != 0) {
+                    rc = -6;
+                    goto join;
+                }
+                for (uint32_t j = 0; j < k_ops; ++j) sum_ops += s_ops[n_ops_tot + j];
+                for (uint32_t j = 0; j < k_seq; ++j) sum_seq += s_seq[n_seq_tot + j];
+                for (uint32_t j = 0; j < k_aux; ++j) sum_auxoff += s_aux[n_aux_tot + j].value_offset;
+                n_ops_tot += k_ops;
+                n_seq_tot += k_seq;
+                n_aux_tot += k_aux;
+            }
             ++recs;
             pos = end;
         }
@@ -164,6 +211,9 @@ join:
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
     res[0] = recs; res[1] = total; res[2] = sum_pos; res[3] = sum_flag;
     res[4] = sum_lseq; res[5] = sum_off; res[6] = n;
+    res[7] = n_ops_tot; res[8] = sum_ops; res[9] = n_seq_tot; res[10] = sum_seq;
+    res[11] = n_aux_tot; res[12] = sum_auxoff;
+    free(s_ops); free(s_seq); free(s_aux);
     free(th); free(done); free(out); free(out_off); free(spans);
     free(c_ref); free(c_pos); free(c_flag); free(c_mapq); free(c_lseq);
     free(c_nref); free(c_npos); free(c_tlen); free(c_off);

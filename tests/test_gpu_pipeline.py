@@ -27,7 +27,17 @@ def oracle_whole_file(oracle_mod, data):
     return r, u, start, tail, cols
 
 
-def check_pipeline(pipe, oracle_mod, data, check_bytes=True):
+def check_soa(got, want):
+    for k in want:
+        if k == "aux_entries":
+            assert len(got[k]) == len(want[k])
+            for f in ("tag", "type", "subtype", "value_offset", "count", "size"):
+                assert np.array_equal(got[k][f], want[k][f]), (k, f)
+        else:
+            assert np.array_equal(got[k], want[k]), k
+
+
+def check_pipeline(pipe, oracle_mod, data, check_bytes=True, soa=False):
     r, u, start, tail, cols = oracle_whole_file(oracle_mod, data)
     assert pipe.records() == len(cols["position"]) and pipe.tail() == tail
     assert pipe.records_start() == start
@@ -38,6 +48,12 @@ def check_pipeline(pipe, oracle_mod, data, check_bytes=True):
     got = pipe.columns_to_host()
     for name in REF_COLS + ["virtual_offsets"]:
         assert np.array_equal(got[name], cols[name]), name
+    if soa:
+        rc, want, _ = oracle_mod.bam_soa(u, cols["record_offsets"])
+        assert rc == 0
+        check_soa(pipe.soa_to_host(), want)
+    else:
+        assert pipe.soa_column("cigar_ops") is None
 
 
 @pytest.mark.parametrize("config,n", [(1, 10000), (2, 60000), (3, 40)])
@@ -47,11 +63,11 @@ def test_pipeline_matches_oracle(hip_ctx, oracle_mod, config, n, graph):
     import fritillaria_amd as fa
     from fritillaria_amd import synth
     s = synth.config_bam(config, n_records=n)
-    pipe = fa.BatchedPipeline(hip_ctx, s.compressed, use_graph=graph)
+    pipe = fa.BatchedPipeline(hip_ctx, s.compressed, use_graph=graph, soa=True)
     assert pipe.device_discovery()
     for rep in range(3 if graph else 1):
         pipe.run()
-        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0))
+        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0), soa=True)
         # eager work between replays
         hip_ctx_tmp = fa.HipCodec(hip_ctx).inflate_batch_device(s.compressed[:65536 * 2], fa.discover(
             s.compressed[:200000])[0][:1])
@@ -65,9 +81,9 @@ def test_pipeline_matches_oracle(hip_ctx, oracle_mod, config, n, graph):
 def test_pipeline_fixtures(hip_ctx, oracle_mod, name):
     import fritillaria_amd as fa
     data = golden_bytes(name)
-    pipe = fa.BatchedPipeline(hip_ctx, data)
+    pipe = fa.BatchedPipeline(hip_ctx, data, soa=True)
     pipe.run()
-    check_pipeline(pipe, oracle_mod, data)
+    check_pipeline(pipe, oracle_mod, data, soa=True)
 
 
 @pytest.mark.parametrize("opts", [dict(order=1), dict(waves_per_cu=1), dict(waves_per_cu=8), dict(bam_window=256),
@@ -140,10 +156,34 @@ def test_pipeline_reports_a_bad_crc_like_the_per_call_path(hip_ctx):
     good = bgzf(data, cap=20000, eof=False)
     bad = bgzf_member(b"x" * 100, crc=1)
     blob = good + bad + bgzf(b"", eof=True)
-    pipe = fa.BatchedPipeline(hip_ctx, blob)
+    pipe = fa.BatchedPipeline(hip_ctx, blob, soa=True)
     with pytest.raises(fa.ChecksumMismatch) as ei:
         pipe.run()
     assert ei.value.offset == len(good) and ei.value.expected == 1
+    # a failed run leaves no columns behind
+    assert pipe.records() == 0
+    assert pipe.column("position").byte_len() == 0
+    assert pipe.soa_column("sequence_packed").byte_len() == 0
+
+
+def test_pipeline_soa_malformed_aux_is_reported_and_clears_views(hip_ctx, oracle_mod):
+    """A record whose aux field does not parse fails the SoA stage of the
+    batched run at that record's offset, as frhip_bam_soa_build does; the run
+    before it (good file) and the views after it behave as the contract says."""
+    print("[gpu] pipeline: malformed aux in the SoA stage", flush=True)
+    import fritillaria_amd as fa
+    from helpers import build_record
+    recs = [build_record(b"r%d" % i, b"ACGT" * 6, aux=b"NMC\x01") for i in range(40)]
+    recs[23] = build_record(b"bad", b"ACGT" * 6, aux=b"NMq\x01")  # unknown type byte 'q'
+    hdr = bam_header()
+    blob = bgzf(hdr + b"".join(recs), cap=1500)
+    pipe = fa.BatchedPipeline(hip_ctx, blob, soa=True, use_graph=True)
+    with pytest.raises(fa.Malformed) as ei:
+        pipe.run()
+    u = np.frombuffer(oracle_mod.inflate_batch(blob, oracle_mod.discover(blob).spans).data, np.uint8)
+    rc, rec, _ = oracle_mod.bam_scan_records(u, len(hdr))
+    assert ei.value.position == int(rec[23])
+    assert pipe.records() == 0 and pipe.soa_column("aux_entries").byte_len() == 0
 
 
 @pytest.mark.parametrize("schedule", [16, 18])
@@ -157,9 +197,10 @@ def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order)
     hip_ctx.set_two_phase_min_members(0)  # captured into the graph at creation
     try:
         s = synth.config_bam(2, n_records=60000)
-        pipe = fa.BatchedPipeline(hip_ctx, s.compressed, inflate_schedule=schedule, order=order, use_graph=True)
+        pipe = fa.BatchedPipeline(hip_ctx, s.compressed, inflate_schedule=schedule, order=order, use_graph=True,
+                                  soa=True)
     finally:
         hip_ctx.set_two_phase_min_members(8192)
     for rep in range(3):
         pipe.run()
-        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0))
+        check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0), soa=True)

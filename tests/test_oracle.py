@@ -243,3 +243,28 @@ def test_mt_baseline_matches_single_thread(oracle_mod):
     for t in (1, 3):
         rc, r = oracle_mod.mt_bam_columns(f, t)
         assert rc == 0 and r["records"] == 3000 and r["inflated"] == len(data)
+
+
+def test_mt_baseline_soa_checksums_match_serial_soa(oracle_mod):
+    """The CPU baseline's SoA totals and checksums equal those of the serial
+    or_bam_soa over the same records (itself pinned by the fixtures' known
+    CIGARs and sequences), whatever the worker count."""
+    rng = np.random.default_rng(6)
+    data = bam_header() + b"".join(random_records(rng, 2500))
+    f = bgzf(data, cap=20000)
+    u = np.frombuffer(data, np.uint8)
+    rc, off, _ = oracle_mod.bam_scan_records(u, len(bam_header()))
+    assert rc == 0
+    rc, soa, _ = oracle_mod.bam_soa(u, off)
+    assert rc == 0
+    want = {"ops": int(soa["cigar_offsets"][-1]), "sum_ops": int(soa["cigar_ops"].astype(np.uint64).sum()),
+            "seq_bytes": int(soa["sequence_offsets"][-1]),
+            "sum_seq": int(soa["sequence_packed"].astype(np.uint64).sum()),
+            "aux": int(soa["aux_offsets"][-1]),
+            "sum_aux_value_offset": int(soa["aux_entries"]["value_offset"].astype(np.uint64).sum())}
+    assert want["ops"] > 0 and want["seq_bytes"] > 0 and want["aux"] == 2 * 2500
+    for t in (1, 4):
+        rc, r = oracle_mod.mt_bam_columns(f, t, soa=True)
+        assert rc == 0 and r["records"] == 2500
+        for k, v in want.items():
+            assert r[k] == v, k
