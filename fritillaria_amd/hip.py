@@ -1076,8 +1076,8 @@ class BatchedPipeline:
         return offs, co[:n]
 
     def stage_ms(self) -> dict:
-        out = (C.c_double * 7)()
-        k = lib().frhip_pipeline_stage_ms(self._h, out, 7)
+        out = (C.c_double * len(STAGES))()
+        k = lib().frhip_pipeline_stage_ms(self._h, out, len(STAGES))
         return {STAGES[i]: out[i] for i in range(k)}
 
     def close(self):
