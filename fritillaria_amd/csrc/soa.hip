@@ -12,9 +12,12 @@
 //          the CG:B:I source of a placeholder, and each tile's three sums;
 //   scan   one workgroup: exclusive prefix of the tile sums and the totals;
 //   fill   the same tiles again: a workgroup prefix of its records' counts
-//          plus the tile's base gives every offset, then each thread copies
-//          its record's ops and packed sequence (aligned 4-byte stores,
-//          source words funnel-shifted) and writes its 16-byte aux entries.
+//          plus the tile's base gives every offset; each thread copies its
+//          record's ops and writes its 16-byte aux entries, then each
+//          half-wave copies its wave's packed sequences one record at a time,
+//          consecutive lanes on consecutive aligned words (source words
+//          funnel-shifted), so a store touches one or two cache lines
+//          instead of one per lane.
 //
 // A record's field bounds come from the decoded columns (cigar_start,
 // seq_start, aux_start, record_end, l_seq: coalesced 4-byte loads), not from
@@ -59,8 +62,20 @@ __device__ uint64_t aux_field(const uint8_t *r, uint64_t p, uint64_t end, AuxEnt
     case 's': case 'S': size = 2; break;
     case 'i': case 'I': case 'f': size = 4; break;
     case 'Z': case 'H': {
+        // the first NUL, four bytes per step while a whole word remains:
+        // (w - 0x01010101) & ~w & 0x80808080 has its lowest set bit in the
+        // first zero byte (bits above it may be set by the borrow)
         uint64_t q = v;
-        while (q < end && r[q] != 0) ++q;
+        for (; q + 4 <= end; q += 4) {
+            const uint32_t w = ldu32(r + q);
+            const uint32_t z = (w - 0x01010101u) & ~w & 0x80808080u;
+            if (z) {
+                q += __builtin_ctz(z) >> 3;
+                break;
+            }
+        }
+        if (q + 4 > end)
+            while (q < end && r[q] != 0) ++q;
         if (q >= end) return 0;
         count = q - v;
         size = count + 1;
@@ -141,9 +156,13 @@ __global__ __launch_bounds__(256) void soa_count_kernel(SoaArgs a)
             const uint32_t c0 = a.cigar_start[i], s0 = a.seq_start[i], a0 = a.aux_start[i], end = a.record_end[i];
             const uint32_t ls = a.l_seq[i];
             const uint32_t nc = (s0 - c0) >> 2;
-            uint32_t cg = 0, cg_count = 0, cg_ok = 0;
+            uint32_t cg = 0, cg_count = 0, cg_ok = 0, bnd = 0;
             bool ok = true;
             for (uint64_t p = a0; p < end;) {
+                if (k >= 1 && k <= 4) {
+                    if (p - a0 < 256) bnd |= (uint32_t)(p - a0) << (8 * (k - 1));
+                    else bnd = SOA_WALK;
+                }
                 AuxEntry e;
                 const uint64_t nx = aux_field(r, p, end, e);
                 if (!nx) {
@@ -166,6 +185,7 @@ __global__ __launch_bounds__(256) void soa_count_kernel(SoaArgs a)
             a.n_ops[i] = ops;
             a.n_aux[i] = k;
             a.cg_at[i] = ph && ok ? cg : 0;
+            a.aux_bounds[i] = k <= 5 ? bnd : SOA_WALK;
             if (!ok) atomicMin(a.first_bad, (unsigned long long)i);
         }
         const uint64_t s0 = wave_sum(ops), s1 = wave_sum(sb), s2 = wave_sum(k);
@@ -268,6 +288,97 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
     for (uint32_t j = head + 4 * nw; j < len; ++j) dst[j] = src[j];
 }
 
+// dst[0, len) = src[0, len) by the 32 lanes of a half-wave (gl = lane in the
+// half): aligned 4-byte words of dst from consecutive lanes (one or two cache
+// lines per store), each from two aligned source words funnel-shifted, and
+// the up to 3 head and 3 tail bytes around them by single lanes. No source
+// word read lies past the word of the last source byte.
+__device__ __forceinline__ void group_copy(uint8_t *dst, const uint8_t *src, uint32_t len, uint32_t gl)
+{
+    const uintptr_t d = (uintptr_t)dst, e = d + len;
+    const uintptr_t D0 = (d + 3) & ~(uintptr_t)3; // first whole word
+    const uintptr_t D1 = e & ~(uintptr_t)3;       // end of the whole words
+    const uintptr_t hend = D0 < e ? D0 : e;
+    if (gl < 3 && d + gl < hend) dst[gl] = src[gl];
+    const uintptr_t T0 = D1 > D0 ? D1 : D0;       // tail bytes [T0, e)
+    if (gl >= 3 && gl < 6) {
+        const uintptr_t b = T0 + (gl - 3);
+        if (b < e) dst[b - d] = src[b - d];
+    }
+    if (D1 <= D0) return;
+    const uint32_t nw = (uint32_t)((D1 - D0) >> 2);
+    const uintptr_t s0 = (uintptr_t)src + (D0 - d);
+    const uint32_t sh = (uint32_t)(s0 & 3);
+    const uint32_t *sw = (const uint32_t *)(s0 - sh);
+    uint32_t *dw = (uint32_t *)D0;
+    if (sh == 0) {
+        for (uint32_t k = gl; k < nw; k += 32) dw[k] = sw[k];
+    } else {
+        for (uint32_t k = gl; k < nw; k += 32) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
+    }
+}
+
+// group_copy split into phases so that several records' loads are in flight
+// before any store: plan, then load (the half-wave's first 32 words and the
+// head/tail bytes), then store them, then rest (words from 32 on, long
+// records only).
+struct CopyPlan {
+    uint32_t *dw;
+    const uint32_t *sw;
+    uint8_t *bd;
+    const uint8_t *bs;
+    uint32_t nw, sh, gl, wv, bv;
+    bool has_w, has_b;
+
+    __device__ __forceinline__ void plan(uint8_t *dst, const uint8_t *src, uint32_t len, uint32_t lane_in_half)
+    {
+        gl = lane_in_half;
+        const uintptr_t d = (uintptr_t)dst, e = d + len;
+        const uintptr_t D0 = (d + 3) & ~(uintptr_t)3, D1 = e & ~(uintptr_t)3;
+        const uintptr_t hend = D0 < e ? D0 : e, T0 = D1 > D0 ? D1 : D0;
+        // one byte for lanes 0-2 (head) or 3-5 (tail)
+        uintptr_t b = gl < 3 ? d + gl : T0 + (gl - 3);
+        has_b = len && ((gl < 3 && b < hend) || (gl >= 3 && gl < 6 && b < e));
+        bd = dst + (b - d);
+        bs = src + (b - d);
+        nw = len && D1 > D0 ? (uint32_t)((D1 - D0) >> 2) : 0;
+        const uintptr_t s0 = (uintptr_t)src + (D0 - d);
+        sh = (uint32_t)(s0 & 3);
+        sw = (const uint32_t *)(s0 - sh);
+        dw = (uint32_t *)D0;
+        has_w = gl < nw;
+    }
+    __device__ __forceinline__ void load()
+    {
+        if (has_w) wv = sh ? __builtin_amdgcn_alignbyte(sw[gl + 1], sw[gl], sh) : sw[gl];
+        if (has_b) bv = *bs;
+    }
+    __device__ __forceinline__ void store()
+    {
+        if (has_w) dw[gl] = wv;
+        if (has_b) *bd = (uint8_t)bv;
+    }
+    __device__ __forceinline__ void rest()
+    {
+        if (nw <= 32) return;
+        if (sh == 0) {
+            for (uint32_t k = gl + 32; k < nw; k += 32) dw[k] = sw[k];
+        } else {
+            for (uint32_t k = gl + 32; k < nw; k += 32) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
+        }
+    }
+};
+
+__device__ __forceinline__ void store_entry(AuxEntry *at, const AuxEntry &e)
+{
+    uint4 v;
+    v.x = (uint32_t)e.tag[0] | (uint32_t)e.tag[1] << 8 | (uint32_t)e.type << 16 | (uint32_t)e.subtype << 24;
+    v.y = e.value_offset;
+    v.z = e.count;
+    v.w = e.size;
+    *(uint4 *)at = v;
+}
+
 __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
 {
     __shared__ uint64_t wsum[3][4];
@@ -285,8 +396,9 @@ __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint64_t i = t * SOA_TILE + threadIdx.x;
+        const bool live = i < n;
         uint32_t ops = 0, sb = 0, k = 0;
-        if (i < n) {
+        if (live) {
             ops = a.n_ops[i];
             sb = (a.l_seq[i] + 1) / 2;
             k = a.n_aux[i];
@@ -306,26 +418,83 @@ __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
             o2 += wsum[2][w];
         }
         __syncthreads();
-        if (i >= n) continue;
-        a.op_off[i] = o0;
-        a.seq_off[i] = o1;
-        a.aux_off[i] = o2;
-        if (!fits) continue;
-        const uint8_t *r = a.buf + a.record_offsets[i];
-        // CIGAR: the inline ops, or the CG:B:I array the count pass found
-        const uint32_t cg = a.cg_at[i];
-        const uint32_t src = cg ? cg : a.cigar_start[i];
-        copy_bytes((uint8_t *)(a.ops + o0), r + src, 4 * ops);
-        copy_bytes(a.seq + o1, r + a.seq_start[i], sb);
-        const uint32_t end = a.record_end[i];
-        AuxEntry *aux = a.aux + o2;
-        uint32_t m = 0;
-        for (uint64_t p = a.aux_start[i]; p < end && m < k;) {
-            AuxEntry e;
-            const uint64_t nx = aux_field(r, p, end, e);
-            if (!nx) break;
-            aux[m++] = e;
-            p = nx;
+        uint64_t seq_src = 0;
+        if (live) {
+            a.op_off[i] = o0;
+            a.seq_off[i] = o1;
+            a.aux_off[i] = o2;
+        }
+        if (live && fits) {
+            const uint8_t *r = a.buf + a.record_offsets[i];
+            // CIGAR: the inline ops, or the CG:B:I array the count pass found
+            const uint32_t cg = a.cg_at[i];
+            const uint32_t src = cg ? cg : a.cigar_start[i];
+            copy_bytes((uint8_t *)(a.ops + o0), r + src, 4 * ops);
+            const uint32_t end = a.record_end[i], a0 = a.aux_start[i], bnd = a.aux_bounds[i];
+            AuxEntry *aux = a.aux + o2;
+            if (bnd != SOA_WALK) {
+                // field f spans [a0 + b(f), a0 + b(f + 1)) with b(0) = 0 and
+                // the last field ending at the record end: each head (tag,
+                // type, first value byte) is one word, B counts another
+                uint32_t st[5], hd[5], bc[5];
+#pragma unroll
+                for (uint32_t f = 0; f < 5; ++f) {
+                    st[f] = f == 0 ? a0 : a0 + ((bnd >> (8 * (f - 1))) & 0xFF);
+                    if (f < k) hd[f] = ldu32(r + st[f]);
+                }
+#pragma unroll
+                for (uint32_t f = 0; f < 5; ++f)
+                    if (f < k && (hd[f] >> 16 & 0xFF) == 'B') bc[f] = ldu32(r + st[f] + 4);
+#pragma unroll
+                for (uint32_t f = 0; f < 5; ++f) {
+                    if (f >= k) break;
+                    const uint32_t fe = f + 1 < k ? st[f + 1] : end;
+                    const uint32_t ty = hd[f] >> 16 & 0xFF;
+                    uint4 v;
+                    if (ty == 'B') {
+                        v.x = hd[f];
+                        v.y = st[f] + 8;
+                        v.z = bc[f];
+                    } else {
+                        v.x = hd[f] & 0x00FFFFFFu;
+                        v.y = st[f] + 3;
+                        v.z = ty == 'Z' || ty == 'H' ? fe - st[f] - 4 : 1;
+                    }
+                    v.w = fe - v.y;
+                    *(uint4 *)(aux + f) = v;
+                }
+            } else {
+                uint32_t m = 0;
+                for (uint64_t p = a0; p < end && m < k;) {
+                    AuxEntry e;
+                    const uint64_t nx = aux_field(r, p, end, e);
+                    if (!nx) break;
+                    store_entry(aux + m++, e);
+                    p = nx;
+                }
+            }
+            seq_src = (uint64_t)(uintptr_t)(r + a.seq_start[i]);
+        }
+        // packed sequences: each half-wave copies every other record of its
+        // wave, consecutive lanes on consecutive words, four records per step
+        // with every load issued before the stores
+        const uint32_t len = live && fits ? sb : 0;
+        for (uint32_t it = 0; it < 32; it += 4) {
+            CopyPlan c[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const int src_lane = (int)(2 * (it + u) + (lane >> 5));
+                const uint32_t L = __shfl(len, src_lane, 64);
+                const uint64_t sp = __shfl(seq_src, src_lane, 64);
+                const uint64_t dp = __shfl(o1, src_lane, 64);
+                c[u].plan(a.seq + dp, (const uint8_t *)(uintptr_t)sp, L, lane & 31);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) c[u].load();
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) c[u].store();
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) c[u].rest();
         }
     }
 }

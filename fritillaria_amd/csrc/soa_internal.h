@@ -26,6 +26,8 @@ constexpr uint32_t SOA_TILE = 256;
 // bytes, aux entries), then 1 when a total (or the record count) exceeds the
 // capacity the outputs were sized for; the fill pass then writes nothing.
 enum { SOA_TOT_OPS = 0, SOA_TOT_SEQ = 1, SOA_TOT_AUX = 2, SOA_OVERFLOW = 3, SOA_N_RESULT = 4 };
+// aux_bounds of a record whose fields the fill pass walks again
+constexpr uint32_t SOA_WALK = 0xFFFFFFFFu;
 
 struct SoaArgs {
     const uint8_t *buf;
@@ -34,8 +36,10 @@ struct SoaArgs {
     // decoded record columns: the record's place and its field bounds
     const uint64_t *record_offsets;
     const uint32_t *l_seq, *cigar_start, *seq_start, *aux_start, *record_end;
-    // count pass, per record (cap_rec entries), and per tile
-    uint32_t *n_ops, *n_aux, *cg_at;
+    // count pass, per record (cap_rec entries), and per tile; aux_bounds packs
+    // the starts of aux fields 1..4 as byte offsets from aux_start (records of
+    // at most 5 fields within 256 bytes), else SOA_WALK
+    uint32_t *n_ops, *n_aux, *cg_at, *aux_bounds;
     uint64_t *tile_sums;   // 3 per tile: ops, sequence bytes, aux entries
     uint64_t *tile_base;   // 3 per tile: exclusive prefix of tile_sums
     uint64_t *result;      // SOA_N_RESULT words
@@ -55,7 +59,7 @@ __host__ __device__ inline uint64_t soa_tiles(uint64_t n) { return (n + SOA_TILE
 // alignment: the three per-record counts, the tile sums and bases, and the
 // result words with the first failing record right after them.
 struct SoaScratchLayout {
-    uint64_t bytes, o_ops, o_aux, o_cg, o_sums, o_base, o_result;
+    uint64_t bytes, o_ops, o_aux, o_cg, o_bnd, o_sums, o_base, o_result;
 };
 
 inline SoaScratchLayout soa_scratch_layout(uint64_t cap_rec)
@@ -66,6 +70,7 @@ inline SoaScratchLayout soa_scratch_layout(uint64_t cap_rec)
     L.o_ops = take(cap_rec * 4);
     L.o_aux = take(cap_rec * 4);
     L.o_cg = take(cap_rec * 4);
+    L.o_bnd = take(cap_rec * 4);
     L.o_sums = take(3 * soa_tiles(cap_rec) * 8);
     L.o_base = take(3 * soa_tiles(cap_rec) * 8);
     L.o_result = take((SOA_N_RESULT + 1) * 8);
@@ -78,6 +83,7 @@ inline void soa_bind_scratch(SoaArgs &a, uint8_t *s, const SoaScratchLayout &L)
     a.n_ops = (uint32_t *)(s + L.o_ops);
     a.n_aux = (uint32_t *)(s + L.o_aux);
     a.cg_at = (uint32_t *)(s + L.o_cg);
+    a.aux_bounds = (uint32_t *)(s + L.o_bnd);
     a.tile_sums = (uint64_t *)(s + L.o_sums);
     a.tile_base = (uint64_t *)(s + L.o_base);
     a.result = (uint64_t *)(s + L.o_result);
