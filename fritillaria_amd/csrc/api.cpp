@@ -205,6 +205,7 @@ extern "C" int frhip_ctx_create(int ordinal, void *stream, frhip_ctx **out, frhi
     }
     const char *w = std::getenv("FRHIP_BAM_WINDOW");
     if (w) c->bam_window = (uint32_t)std::strtoul(w, nullptr, 10);
+    noalloc_bind(c);
     *out = c;
     return FRHIP_OK;
 }
@@ -318,6 +319,17 @@ extern "C" int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip
     hipError_t r = launch_copy((uint8_t *)dst->ptr, (const uint8_t *)src->ptr, src->len, ctx->stream);
     if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
     return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "device copy");
+}
+
+extern "C" int frhip_buf_copy_from_host(frhip_buf *b, uint64_t offset, const void *src, uint64_t len,
+                                        frhip_error *err)
+{
+    if (!b || offset > b->len || len > b->len - offset)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, offset, "copy past the end of the buffer");
+    if (len == 0) return FRHIP_OK;
+    (void)hipSetDevice(b->ordinal);
+    hipError_t r = hipMemcpy((uint8_t *)b->ptr + offset, src, len, hipMemcpyHostToDevice);
+    return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "host to device copy");
 }
 
 // ------------------------------------------------------------------------

@@ -65,9 +65,14 @@ struct frhip_ctx {
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
         rec_scratch;
     frhip::PinnedScratch pinned;
+    // device address of this GPU's no-allocation scratch array (noalloc.hip)
+    void *noalloc_scratch = nullptr;
+    hipError_t noalloc_rc = hipSuccess;
 };
 
 namespace frhip {
+// Takes the device address of the no-allocation scratch for ctx's device.
+void noalloc_bind(frhip_ctx *ctx);
 frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err);
 void ctx_retain(frhip_ctx *ctx);
 void ctx_release(frhip_ctx *ctx);

@@ -146,6 +146,8 @@ def lib():
         "frhip_buf_free": (None, [_vp]),
         "frhip_buf_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp), _ep]),
         "frhip_buf_copy_device": (C.c_int, [_vp, _vp, _vp, _ep]),
+        "frhip_buf_copy_from_host": (C.c_int, [_vp, _u64, _vp, _u64, _ep]),
+        "frhip_inflate_batch_device_noalloc": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _ep]),
         "frhip_discover": (C.c_int, [_vp, _u64, _u64, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64), _ep]),
         "frhip_inflate_batch_device": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _ep]),
         "frhip_inflate_batch": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _ep]),
@@ -354,6 +356,11 @@ class DeviceBuffer:
         self.copy_to_host(out)
         return out
 
+    def copy_from_host(self, src, offset: int = 0) -> None:
+        a = _u8(src)
+        e = _Err()
+        _raise(lib().frhip_buf_copy_from_host(self._h, offset, _ptr(a), a.size, C.byref(e)), e)
+
     def free(self):
         if self._owned and self._h:
             lib().frhip_buf_free(self._h)
@@ -467,6 +474,20 @@ class HipCodec:
                                               C.byref(out), _ptr(offs), C.byref(e))
         _raise(rc, e)
         return DeviceInflateBatch(DeviceBuffer(out), offs, sp["compressed_offset"].copy())
+
+    def inflate_batch_device_noalloc(self, d_batch: "DeviceBuffer", batch_len: int, spans,
+                                     d_out: "DeviceBuffer") -> np.ndarray:
+        """frhip_inflate_batch_device_noalloc: inflates into the caller's d_out
+        from the caller's d_batch (batch_len bytes, then 4096 zero bytes), with
+        no device allocation; returns the member boundaries."""
+        sp = _spans_array(spans)
+        offs = np.zeros(sp.size + 1, dtype=np.uint64)
+        e = _Err()
+        rc = lib().frhip_inflate_batch_device_noalloc(self.ctx.handle, d_batch.device_ptr(), batch_len, _ptr(sp),
+                                                      sp.size, d_out.device_ptr(), d_out.byte_len(), _ptr(offs),
+                                                      C.byref(e))
+        _raise(rc, e)
+        return offs
 
     def inflate_batch(self, batch, spans) -> InflateBatch:
         a = _u8(batch)

@@ -118,6 +118,9 @@ void frhip_buf_free(frhip_buf *b);
    vector accesses, which makes it the calibration kernel for HBM byte counters. */
 int frhip_buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_buf **out, frhip_error *err);
 int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip_buf *src, frhip_error *err);
+/* No reference counterpart. Copies src[0..len) into the buffer at `offset`
+   (synchronous); offset + len beyond byte_len is FRHIP_E_INVALID_DEVICE_BATCH. */
+int frhip_buf_copy_from_host(frhip_buf *b, uint64_t offset, const void *src, uint64_t len, frhip_error *err);
 
 /* ---- BGZF member discovery (crates/fritillaria-bgzf/src/discover.rs:51-113) ---- */
 
@@ -144,6 +147,22 @@ int frhip_inflate_batch_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t ba
 int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
                         const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
                         uint64_t *out_offsets, frhip_error *err);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc). The same inflate, verification order and error results as
+   frhip_inflate_batch_device, but nothing is allocated: the caller passes the
+   compressed batch in device memory, d_batch[0..batch_len) followed by 4096
+   zero bytes (checked: FRHIP_E_INVALID_DEVICE_BATCH otherwise), and the output
+   d_out[0..out_cap) with out_cap >= sum(isize); out_offsets[0..n] receives the
+   member boundaries. The member plan and status words live in a fixed device
+   array of the library, one per GPU, so n is at most
+   FRHIP_NOALLOC_MAX_MEMBERS (larger batches: FRHIP_E_INVALID_DEVICE_BATCH).
+   Calls that overlap on one GPU share that array and race. Synchronous. */
+#define FRHIP_NOALLOC_MAX_MEMBERS 4096
+int frhip_inflate_batch_device_noalloc(frhip_ctx *ctx, const void *d_batch, uint64_t batch_len,
+                                       const frhip_block_span *spans, uint64_t n, void *d_out, uint64_t out_cap,
+                                       uint64_t *out_offsets, frhip_error *err);
 
 /* crc32_blocks (crates/fritillaria-cuda/src/backend.rs:737-810): CRC32 of byte
    ranges data[off[i] .. off[i]+len[i]) of a device buffer. */
