@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -92,8 +93,46 @@ frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err)
 void buf_free(frhip_buf *b)
 {
     if (!b) return;
+    event_release(b->ready);
     if (b->ptr) (void)hipFree(b->ptr);
     delete b;
+}
+
+EventRef *event_record(hipStream_t s)
+{
+    EventRef *e = new (std::nothrow) EventRef;
+    if (!e) return nullptr;
+    if (hipEventCreateWithFlags(&e->e, hipEventDisableTiming) != hipSuccess) {
+        delete e;
+        return nullptr;
+    }
+    if (hipEventRecord(e->e, s) != hipSuccess) {
+        (void)hipEventDestroy(e->e);
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+void event_release(EventRef *e)
+{
+    if (e && e->refs.fetch_sub(1) == 1) {
+        (void)hipEventDestroy(e->e);
+        delete e;
+    }
+}
+
+void buf_attach_event(frhip_buf *b, EventRef *e)
+{
+    if (!b || !e) return;
+    e->refs.fetch_add(1);
+    event_release(b->ready);
+    b->ready = e;
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 // Grow-only device scratch owned by the context.
@@ -302,6 +341,25 @@ extern "C" int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t ds
 
 extern "C" void frhip_buf_free(frhip_buf *b) { buf_free(b); }
 
+extern "C" void *frhip_buf_ready_event(const frhip_buf *b)
+{
+    return b && b->len && b->ready ? (void *)b->ready->e : nullptr;
+}
+
+extern "C" int frhip_event_wait(void *event, void *stream, frhip_error *err)
+{
+    if (!event) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no event");
+    hipError_t r = hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0);
+    return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "hipStreamWaitEvent");
+}
+
+extern "C" int frhip_event_synchronize(void *event, frhip_error *err)
+{
+    if (!event) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no event");
+    hipError_t r = hipEventSynchronize((hipEvent_t)event);
+    return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "hipEventSynchronize");
+}
+
 extern "C" int frhip_buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_buf **out, frhip_error *err)
 {
     (void)hipSetDevice(ctx->ordinal);
@@ -435,11 +493,21 @@ namespace frhip {
 
 // Plan + upload + launch + verify. On success *out owns the output.
 int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
-                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t,
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, frhip_inflate_timings *t,
                    uint64_t align_at)
 {
     *out = nullptr;
     (void)hipSetDevice(ctx->ordinal);
+    double t0 = t ? now_ms() : 0;
+    // ends a timed phase: wait for the stream, add the host time since t0
+    auto phase = [&](double &acc) {
+        if (!t) return hipSuccess;
+        hipError_t r = hipStreamSynchronize(ctx->stream);
+        const double t1 = now_ms();
+        acc += t1 - t0;
+        t0 = t1;
+        return r;
+    };
     // BatchPlan::new (backend.rs:95-133)
     std::vector<uint64_t> meta(3 * n + 1);
     uint64_t *in_off = meta.data();
@@ -463,6 +531,12 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
         total += s.isize;
     }
     (void)lens;
+    if (t) {
+        t->blocks += n;
+        for (uint64_t i = 0; i < n; ++i) t->compressed_bytes += spans[i].payload_len;
+        t->inflated_bytes += total;
+        (void)phase(t->plan_ms);
+    }
     frhip_buf *o = buf_alloc(ctx, total, err);
     if (!o) return err ? err->kind : FRHIP_E_DEVICE;
     if (n == 0) {
@@ -487,18 +561,11 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     if (!d_meta) { buf_free(o); return rc; }
     uint32_t *d_st = (uint32_t *)ctx->status_scratch.get(n * 12, err, &rc);
     if (!d_st) { buf_free(o); return rc; }
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-    if (t) {
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        (void)hipEventCreate(&e2);
-        (void)hipEventRecord(e0, ctx->stream);
-    }
     hipError_t r = hipMemcpyAsync(d_in, batch, batch_len, hipMemcpyHostToDevice, ctx->stream);
     if (r == hipSuccess) r = hipMemsetAsync(d_in + batch_len, 0, pad, ctx->stream);
     if (r == hipSuccess) r = hipMemcpyAsync(d_meta, h_meta, meta_bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (r == hipSuccess && t) r = phase(t->upload_ms);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "uploading the batch"); }
-    if (t) (void)hipEventRecord(e1, ctx->stream);
     InflateArgs a{};
     a.in = d_in;
     a.in_off = (const uint64_t *)d_meta;
@@ -522,22 +589,17 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     a.two_phase_min = ctx->two_phase_min;
     r = launch_inflate(a, (uint32_t)n, ctx->stream, ctx->inflate_schedule);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
-    if (t) (void)hipEventRecord(e2, ctx->stream);
+    // the consumer's ordering point: the inflated bytes are final here
+    if (EventRef *ev = event_record(ctx->stream)) {
+        buf_attach_event(o, ev);
+        event_release(ev);
+    }
+    if (t) r = phase(t->kernel_ms);
     uint32_t *h_st = (uint32_t *)ctx->pinned.get(n * 12 > meta_bytes ? n * 12 : meta_bytes, err, &rc);
     if (!h_st) { buf_free(o); return rc; }
-    r = hipMemcpyAsync(h_st, d_st, n * 12, hipMemcpyDeviceToHost, ctx->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(h_st, d_st, n * 12, hipMemcpyDeviceToHost, ctx->stream);
     if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "inflate"); }
-    if (t) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        t->upload_ms += ms;
-        (void)hipEventElapsedTime(&ms, e1, e2);
-        t->kernel_ms += ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipEventDestroy(e2);
-    }
     // verify_and_offsets (backend.rs:221-256): status, then size, then CRC
     const uint32_t *st = h_st, *produced = h_st + n, *crc = h_st + 2 * n;
     offsets[0] = 0;
@@ -564,6 +626,7 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
         }
         offsets[i + 1] = out_off[i] + produced[i];
     }
+    if (t) (void)phase(t->verify_ms);
     *out = o;
     return FRHIP_OK;
 }
@@ -577,21 +640,44 @@ extern "C" int frhip_inflate_batch_device(frhip_ctx *ctx, const uint8_t *batch, 
     return inflate_device(ctx, batch, batch_len, spans, n, out_data, out_offsets, err, nullptr);
 }
 
-extern "C" int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
-                                   const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
-                                   uint64_t *out_offsets, frhip_error *err)
+static int inflate_host(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
+                        uint64_t n, uint8_t *out, uint64_t out_cap, uint64_t *out_offsets, frhip_inflate_timings *t,
+                        frhip_error *err)
 {
     frhip_buf *b = nullptr;
-    int rc = inflate_device(ctx, batch, batch_len, spans, n, &b, out_offsets, err, nullptr);
+    int rc = inflate_device(ctx, batch, batch_len, spans, n, &b, out_offsets, err, t);
     if (rc) return rc;
     if (b->len > out_cap) {
         buf_free(b);
         return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "output buffer holds %llu bytes, batch needs %llu",
                        (unsigned long long)out_cap, (unsigned long long)b->len);
     }
+    const double t0 = t ? now_ms() : 0;
     rc = frhip_buf_copy_to_host(b, out, b->len, err);
+    if (t) t->download_ms += now_ms() - t0;
     buf_free(b);
     return rc;
+}
+
+extern "C" int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                   const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
+                                   uint64_t *out_offsets, frhip_error *err)
+{
+    return inflate_host(ctx, batch, batch_len, spans, n, out, out_cap, out_offsets, nullptr, err);
+}
+
+extern "C" int frhip_inflate_batch_device_timed(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                                const frhip_block_span *spans, uint64_t n, frhip_buf **out_data,
+                                                uint64_t *out_offsets, frhip_inflate_timings *t, frhip_error *err)
+{
+    return inflate_device(ctx, batch, batch_len, spans, n, out_data, out_offsets, err, t);
+}
+
+extern "C" int frhip_inflate_batch_timed(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                         const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
+                                         uint64_t *out_offsets, frhip_inflate_timings *t, frhip_error *err)
+{
+    return inflate_host(ctx, batch, batch_len, spans, n, out, out_cap, out_offsets, t, err);
 }
 
 extern "C" uint32_t frhip_crc32_combine(uint32_t crc1, uint32_t crc2, uint64_t len2)

@@ -53,12 +53,30 @@ int empty_batch(frhip_ctx *ctx, uint64_t start, bool with_voff, frhip_bam_batch 
 
 } // namespace
 
-extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
-                                uint64_t nb, const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out,
-                                frhip_error *err)
+namespace {
+
+// The decode with, when t is given, stream synchronisations between its
+// phases and each phase's host time added to *t (bam.rs:324-574 timings).
+int bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets, uint64_t nb,
+               const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out, frhip_decode_timings *t,
+               frhip_error *err)
 {
     *out = nullptr;
+    double t0 = t ? now_ms() : 0;
+    auto phase = [&](double &acc) {
+        if (!t) return hipSuccess;
+        hipError_t r = hipStreamSynchronize(ctx->stream);
+        const double t1 = now_ms();
+        acc += t1 - t0;
+        t0 = t1;
+        return r;
+    };
+    if (t) ++t->batches;
     if (!data || data->len == 0) return empty_batch(ctx, start, coffsets != nullptr, out, err);
+    if (t) {
+        t->blocks += nb;
+        t->bytes += data->len;
+    }
     if (data->ordinal != ctx->ordinal)
         return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "batch is on device %d but this decoder is on %d",
                        data->ordinal, ctx->ordinal);
@@ -77,6 +95,7 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     hipError_t r = hipMemcpyAsync(s + L.o_bs, block_offsets, (nb + 1) * 8, hipMemcpyHostToDevice, ctx->stream);
     if (r == hipSuccess && coffsets)
         r = hipMemcpyAsync(s + L.o_coff, coffsets, nb * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (r == hipSuccess) r = phase(t ? t->setup_ms : t0);
     if (r != hipSuccess) return hip_err(err, r, "uploading block starts");
 
     BamSpecArgs sa{};
@@ -95,6 +114,7 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     sa.complete = s + L.o_comp;
     sa.rel_offs = (uint16_t *)(s + L.o_rel);
     r = launch_bam_spec(sa, ctx->stream);
+    if (r == hipSuccess) r = phase(t ? t->scan_ms : t0);
     if (r != hipSuccess) return hip_err(err, r, "launching bam_spec");
 
     BamReconcileArgs ra{};
@@ -118,10 +138,12 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     ra.fast_scan = (uint64_t *)(s + L.o_fscan);
     ra.fast_state = (uint32_t *)(s + L.o_fstate);
     r = launch_bam_reconcile(ra, ctx->stream);
+    if (r == hipSuccess) r = phase(t ? t->reconcile_ms : t0);
     if (r != hipSuccess) return hip_err(err, r, "launching bam_reconcile");
     uint64_t totals[4];
     r = hipMemcpyAsync(totals, ra.totals, sizeof totals, hipMemcpyDeviceToHost, ctx->stream);
     if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r == hipSuccess) r = phase(t ? t->totals_ms : t0);
     if (r != hipSuccess) return hip_err(err, r, "bam boundaries");
     if (totals[2] != 0) {
         int k = set_err(err, FRHIP_E_MALFORMED, totals[1], "malformed record reached from a confirmed boundary");
@@ -159,6 +181,7 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     ea.record_offsets = (uint64_t *)b->cols[FRHIP_BAM_RECORD_OFFSETS]->ptr;
     ea.virtual_offsets = coffsets ? (uint64_t *)b->cols[FRHIP_BAM_VIRTUAL_OFFSETS]->ptr : nullptr;
     r = n ? launch_bam_emit(ea, ctx->stream) : hipSuccess;
+    if (r == hipSuccess) r = phase(t ? t->emit_ms : t0);
     if (r != hipSuccess) {
         frhip_bam_batch_free(b);
         return hip_err(err, r, "launching bam_emit");
@@ -183,13 +206,37 @@ extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uin
     da.c.aux_start = (uint32_t *)b->cols[12]->ptr;
     da.c.record_end = (uint32_t *)b->cols[13]->ptr;
     r = launch_bam_decode(da, n, ctx->stream);
+    // the consumer's ordering point for every column of the batch
+    if (r == hipSuccess)
+        if (EventRef *ev = event_record(ctx->stream)) {
+            for (frhip_buf *c : b->cols) buf_attach_event(c, ev);
+            event_release(ev);
+        }
     if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r == hipSuccess) r = phase(t ? t->decode_ms : t0);
     if (r != hipSuccess) {
         frhip_bam_batch_free(b);
         return hip_err(err, r, "bam decode");
     }
+    if (t) t->records += n;
     *out = b;
     return FRHIP_OK;
+}
+
+} // namespace
+
+extern "C" int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
+                                uint64_t nb, const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out,
+                                frhip_error *err)
+{
+    return bam_decode(ctx, data, block_offsets, nb, coffsets, start, out, nullptr, err);
+}
+
+extern "C" int frhip_bam_decode_timed(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
+                                      uint64_t nb, const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out,
+                                      frhip_decode_timings *t, frhip_error *err)
+{
+    return bam_decode(ctx, data, block_offsets, nb, coffsets, start, out, t, err);
 }
 
 extern "C" uint64_t frhip_bam_batch_len(const frhip_bam_batch *b) { return b->n; }
@@ -470,6 +517,11 @@ extern "C" int frhip_bam_soa_build(frhip_ctx *ctx, const frhip_buf *data, const 
     a.cap_seq = tot[SOA_TOT_SEQ];
     a.cap_aux = tot[SOA_TOT_AUX];
     r = launch_soa_fill(a, st);
+    if (r == hipSuccess)
+        if (EventRef *ev = event_record(st)) {
+            for (frhip_buf *c : o->cols) buf_attach_event(c, ev);
+            event_release(ev);
+        }
     if (r == hipSuccess) r = hipStreamSynchronize(st);
     if (r != hipSuccess) return fail(hip_err(err, r, "SoA fill"));
     *out = o;

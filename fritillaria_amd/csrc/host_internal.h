@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <vector>
 
 #include "../../include/frhip.h"
@@ -24,9 +25,15 @@ struct PinnedScratch {
     void release();
 };
 
-struct InflateTiming {
-    double upload_ms = 0, kernel_ms = 0;
+// An event recorded after the kernels that produced a set of buffers, shared
+// by them (CudaAlloc::ready_event, crates/fritillaria-cuda/src/lib.rs:207-224).
+struct EventRef {
+    hipEvent_t e = nullptr;
+    std::atomic<int> refs{1};
 };
+// Records a new event on s; nullptr if that fails (the buffers then have none).
+EventRef *event_record(hipStream_t s);
+void event_release(EventRef *e);
 
 int set_err(frhip_error *e, int kind, uint64_t offset, const char *fmt, ...);
 int hip_err(frhip_error *e, hipError_t r, const char *what);
@@ -36,8 +43,10 @@ int discover(const uint8_t *buf, uint64_t len, uint64_t base, std::vector<frhip_
 // Members [0, align_at) are placed back to back from 0; member align_at and
 // those after it from the next 4 KiB boundary (align_at >= n: no gap).
 // offsets[i + 1] is the end of member i's output.
+// With t, the phases are separated by stream synchronisations and their host
+// wall-clock times added to *t (InflateTimings, backend.rs:136-177).
 int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
-                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, InflateTiming *t,
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, frhip_inflate_timings *t,
                    uint64_t align_at = ~0ull);
 hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n,
                                uint32_t *crc, const uint32_t *x2n, hipStream_t s);
@@ -48,6 +57,7 @@ struct frhip_buf {
     void *ptr;
     uint64_t len;
     int ordinal;
+    frhip::EventRef *ready = nullptr; // owned reference, or none
 };
 
 struct frhip_ctx {
@@ -77,6 +87,10 @@ frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err);
 void ctx_retain(frhip_ctx *ctx);
 void ctx_release(frhip_ctx *ctx);
 void buf_free(frhip_buf *b);
+// Gives b a reference to e (e may be null).
+void buf_attach_event(frhip_buf *b, EventRef *e);
+// Host wall-clock in milliseconds, for the timed entry points.
+double now_ms();
 } // namespace frhip
 
 #endif

@@ -102,6 +102,28 @@ SPAN_DTYPE = np.dtype([("compressed_offset", "<u8"), ("payload_start", "<u8"),
                        ("payload_len", "<u8"), ("isize", "<u4"), ("crc32", "<u4")])
 
 
+class InflateTimings(C.Structure):
+    """InflateTimings (crates/fritillaria-cuda/src/backend.rs:136-177): per-phase
+    host wall-clock in ms, phases separated by stream synchronisation; the
+    timed calls add to it, so one object accumulates over batches."""
+    _fields_ = [("plan_ms", C.c_double), ("upload_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("download_ms", C.c_double), ("verify_ms", C.c_double), ("blocks", C.c_uint64),
+                ("compressed_bytes", C.c_uint64), ("inflated_bytes", C.c_uint64)]
+
+    def total(self) -> float:
+        return self.plan_ms + self.upload_ms + self.kernel_ms + self.download_ms + self.verify_ms
+
+
+class DecodeTimings(C.Structure):
+    """DecodeTimings (crates/fritillaria-cuda/src/bam.rs:55-101), same rules."""
+    _fields_ = [("setup_ms", C.c_double), ("scan_ms", C.c_double), ("reconcile_ms", C.c_double),
+                ("totals_ms", C.c_double), ("emit_ms", C.c_double), ("decode_ms", C.c_double),
+                ("batches", C.c_uint64), ("blocks", C.c_uint64), ("records", C.c_uint64), ("bytes", C.c_uint64)]
+
+    def total(self) -> float:
+        return self.setup_ms + self.scan_ms + self.reconcile_ms + self.totals_ms + self.emit_ms + self.decode_ms
+
+
 def _raise(rc: int, err: _Err):
     if rc != OK:
         raise _ERRORS.get(rc, FrError)(err)
@@ -147,6 +169,12 @@ def lib():
         "frhip_buf_alloc": (C.c_int, [_vp, _u64, C.POINTER(_vp), _ep]),
         "frhip_buf_copy_device": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_buf_copy_from_host": (C.c_int, [_vp, _u64, _vp, _u64, _ep]),
+        "frhip_buf_ready_event": (_vp, [_vp]),
+        "frhip_event_wait": (C.c_int, [_vp, _vp, _ep]),
+        "frhip_event_synchronize": (C.c_int, [_vp, _ep]),
+        "frhip_inflate_batch_device_timed": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _vp, _ep]),
+        "frhip_inflate_batch_timed": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _vp, _ep]),
+        "frhip_bam_decode_timed": (C.c_int, [_vp, _vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _ep]),
         "frhip_inflate_batch_device_noalloc": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _ep]),
         "frhip_discover": (C.c_int, [_vp, _u64, _u64, _vp, _u64, C.POINTER(_u64), C.POINTER(_u64), _ep]),
         "frhip_inflate_batch_device": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _ep]),
@@ -356,6 +384,11 @@ class DeviceBuffer:
         self.copy_to_host(out)
         return out
 
+    def ready_event(self) -> Optional[int]:
+        """The hipEvent_t recorded after the kernels that wrote this buffer
+        (CudaAlloc::ready_event), or None."""
+        return lib().frhip_buf_ready_event(self._h) or None
+
     def copy_from_host(self, src, offset: int = 0) -> None:
         a = _u8(src)
         e = _Err()
@@ -474,6 +507,37 @@ class HipCodec:
                                               C.byref(out), _ptr(offs), C.byref(e))
         _raise(rc, e)
         return DeviceInflateBatch(DeviceBuffer(out), offs, sp["compressed_offset"].copy())
+
+    @staticmethod
+    def ready_event(batch: "DeviceInflateBatch") -> Optional[int]:
+        """CudaCodec::ready_event (crates/fritillaria-cuda/src/lib.rs:207-224):
+        the event marking completion of the inflate that produced `batch`; None
+        for an empty batch."""
+        d = batch.data()
+        return None if d is None else d.ready_event()
+
+    def inflate_batch_device_timed(self, batch, spans, timings: InflateTimings) -> DeviceInflateBatch:
+        a = _u8(batch)
+        sp = _spans_array(spans)
+        offs = np.zeros(sp.size + 1, dtype=np.uint64)
+        out = _vp()
+        e = _Err()
+        rc = lib().frhip_inflate_batch_device_timed(self.ctx.handle, _ptr(a), a.size, _ptr(sp), sp.size,
+                                                    C.byref(out), _ptr(offs), C.byref(timings), C.byref(e))
+        _raise(rc, e)
+        return DeviceInflateBatch(DeviceBuffer(out), offs, sp["compressed_offset"].copy())
+
+    def inflate_batch_timed(self, batch, spans, timings: InflateTimings) -> InflateBatch:
+        a = _u8(batch)
+        sp = _spans_array(spans)
+        total = int(sp["isize"].astype(np.uint64).sum()) if sp.size else 0
+        out = np.empty(max(total, 1), dtype=np.uint8)
+        offs = np.zeros(sp.size + 1, dtype=np.uint64)
+        e = _Err()
+        rc = lib().frhip_inflate_batch_timed(self.ctx.handle, _ptr(a), a.size, _ptr(sp), sp.size, _ptr(out),
+                                             out.size, _ptr(offs), C.byref(timings), C.byref(e))
+        _raise(rc, e)
+        return InflateBatch(out[:total], offs)
 
     def inflate_batch_device_noalloc(self, d_batch: "DeviceBuffer", batch_len: int, spans,
                                      d_out: "DeviceBuffer") -> np.ndarray:
@@ -646,15 +710,24 @@ class BamDecoder:
         return BamSoa(out, records)
 
     def decode(self, batch: DeviceInflateBatch, start: int) -> DeviceRecordBatch:
+        return self.decode_timed(batch, start, None)
+
+    def decode_timed(self, batch: DeviceInflateBatch, start: int,
+                     timings: Optional[DecodeTimings]) -> DeviceRecordBatch:
+        """BamDecoder::decode_timed (crates/fritillaria-cuda/src/bam.rs:161-187):
+        decode, adding each phase's time to `timings` (None: untimed)."""
         data = batch.data()
         offs = np.ascontiguousarray(batch.offsets(), dtype=np.uint64)
         co = batch.compressed_offsets()
         co = None if co is None else np.ascontiguousarray(co, dtype=np.uint64)
         out = _vp()
         e = _Err()
-        rc = lib().frhip_bam_decode(self.ctx.handle, None if data is None else data._h, _ptr(offs),
-                                    len(offs) - 1, None if co is None else _ptr(co), start,
-                                    C.byref(out), C.byref(e))
+        args = (self.ctx.handle, None if data is None else data._h, _ptr(offs), len(offs) - 1,
+                None if co is None else _ptr(co), start, C.byref(out))
+        if timings is None:
+            rc = lib().frhip_bam_decode(*args, C.byref(e))
+        else:
+            rc = lib().frhip_bam_decode_timed(*args, C.byref(timings), C.byref(e))
         _raise(rc, e)
         return DeviceRecordBatch(out, batch)
 
@@ -1122,6 +1195,17 @@ QUERY_RESULT_DTYPE = np.dtype([("first", "<u8"), ("records", "<u8"), ("members",
                                ("inflated_bytes", "<u8"), ("status", "<i4"), ("reserved", "<u4"),
                                ("error_position", "<u8")])
 TO_END = (1 << 64) - 1
+
+
+def event_wait(event: int, stream: int) -> None:
+    """Orders work later enqueued on `stream` after `event` (hipStreamWaitEvent)."""
+    e = _Err()
+    _raise(lib().frhip_event_wait(event, stream, C.byref(e)), e)
+
+
+def event_synchronize(event: int) -> None:
+    e = _Err()
+    _raise(lib().frhip_event_synchronize(event, C.byref(e)), e)
 
 
 def device_memory(ctx: HipContext):

@@ -54,6 +54,30 @@ typedef struct {
     uint32_t crc32;
 } frhip_block_span;
 
+/* InflateTimings (crates/fritillaria-cuda/src/backend.rs:136-177): host
+   wall-clock per phase with the phases separated by stream synchronisations,
+   so the sum bounds the untimed call from above; the timed calls ADD to every
+   field (accumulate over batches). */
+typedef struct {
+    double plan_ms;     /* bounds checks and the ISIZE prefix sum */
+    double upload_ms;   /* compressed batch and member plan to the device */
+    double kernel_ms;   /* inflate with the folded-in CRC32 */
+    double download_ms; /* inflated bytes to the host (host path only) */
+    double verify_ms;   /* status / ISIZE / CRC checks and the offsets */
+    uint64_t blocks, compressed_bytes, inflated_bytes;
+} frhip_inflate_timings;
+
+/* DecodeTimings (crates/fritillaria-cuda/src/bam.rs:55-101), same rules. */
+typedef struct {
+    double setup_ms;     /* block starts uploaded, scratch ready */
+    double scan_ms;      /* boundary speculation and the per-member exact walks */
+    double reconcile_ms; /* the record chain across members */
+    double totals_ms;    /* the one read-back of {records, tail, status} */
+    double emit_ms;      /* record (and virtual) offsets */
+    double decode_ms;    /* the field columns */
+    uint64_t batches, blocks, records, bytes;
+} frhip_decode_timings;
+
 typedef struct frhip_ctx frhip_ctx;
 typedef struct frhip_buf frhip_buf;
 typedef struct frhip_bam_batch frhip_bam_batch;
@@ -112,6 +136,18 @@ void *frhip_buf_device_ptr(const frhip_buf *b);
 /* copy_to_host: rejects dst_len != byte_len with FRHIP_E_INVALID_DEVICE_BATCH and synchronises. */
 int frhip_buf_copy_to_host(const frhip_buf *b, void *dst, uint64_t dst_len, frhip_error *err);
 void frhip_buf_free(frhip_buf *b);
+/* CudaCodec::ready_event (crates/fritillaria-cuda/src/lib.rs:207-224): the
+   hipEvent_t recorded on the producing context's stream after the kernels
+   that wrote this buffer (inflated batches, BAM columns and SoA columns), for
+   a consumer on its own stream to order against (frhip_event_wait) instead of
+   synchronising the host. NULL for an empty buffer or one with no producer
+   event (frhip_buf_alloc, pipeline and server views). Owned by the buffer. */
+void *frhip_buf_ready_event(const frhip_buf *b);
+/* hipStreamWaitEvent(stream, event): work enqueued on `stream` afterwards runs
+   after the event's work. */
+int frhip_event_wait(void *event, void *stream, frhip_error *err);
+/* Blocks the host until the event's work is done. */
+int frhip_event_synchronize(void *event, frhip_error *err);
 /* No reference counterpart. A device buffer of len bytes (contents undefined),
    and a plain device-to-device copy kernel over two buffers of equal length
    (synchronous). The copy reads and writes every byte once, with 16-byte
@@ -147,6 +183,16 @@ int frhip_inflate_batch_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t ba
 int frhip_inflate_batch(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
                         const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
                         uint64_t *out_offsets, frhip_error *err);
+
+/* CudaContext::inflate_batch_device_timed / inflate_batch_timed
+   (crates/fritillaria-cuda/src/backend.rs:463-582): the calls above, adding
+   each phase's time to *t. */
+int frhip_inflate_batch_device_timed(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                                     const frhip_block_span *spans, uint64_t n, frhip_buf **out_data,
+                                     uint64_t *out_offsets, frhip_inflate_timings *t, frhip_error *err);
+int frhip_inflate_batch_timed(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len,
+                              const frhip_block_span *spans, uint64_t n, uint8_t *out, uint64_t out_cap,
+                              uint64_t *out_offsets, frhip_inflate_timings *t, frhip_error *err);
 
 /* No counterpart in the reference: for callers that cannot allocate device
    memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -204,6 +250,11 @@ typedef enum {
 int frhip_bam_decode(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
                      uint64_t n_blocks, const uint64_t *coffsets, uint64_t start,
                      frhip_bam_batch **out, frhip_error *err);
+/* BamDecoder::decode_timed (crates/fritillaria-cuda/src/bam.rs:161-187): the
+   same, adding each phase's time to *t. */
+int frhip_bam_decode_timed(frhip_ctx *ctx, const frhip_buf *data, const uint64_t *block_offsets,
+                           uint64_t n_blocks, const uint64_t *coffsets, uint64_t start, frhip_bam_batch **out,
+                           frhip_decode_timings *t, frhip_error *err);
 uint64_t frhip_bam_batch_len(const frhip_bam_batch *b);
 uint64_t frhip_bam_batch_tail(const frhip_bam_batch *b);
 /* Members the true chain did not enter at a speculated candidate (a diagnostic,
