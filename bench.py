@@ -162,54 +162,215 @@ def per_call_pass(fa, ctx, compressed, blocks_per_batch=256, max_batches=None):
     return n_rec, u_bytes, batches
 
 
-def bcf_fastq_bench(args):
-    """The mixed configuration: a BCF of 200,000 biallelic sites x 2,504 diploid
-    samples (GT only, INFO AC/AN/AF) and a bgzipped FASTQ of 5,000,000 x 150 bp
-    records, decoded together per call (frhip_bcf_fastq_decode): the two
-    streams' members inflated and verified in one launch of the persistent
-    inflate, then the BCF boundary scan with site-core columns and the FASTQ
-    boundary scan with its columns. Host-fed: each call uploads the compressed
-    bytes over PCIe, as the per-call drop-in does. Prints one JSON line."""
-    import numpy as np
+def reader_sums(fa, ctx, compressed, np, blocks_per_batch=256):
+    """The per-call drop-in loop (DeviceBgzfReader + BamDecoder) over a whole
+    file: (records, inflated bytes, batches, sum pos, sum flag, sum l_seq)."""
+    codec = fa.HipCodec(ctx)
+    decoder = fa.BamDecoder(ctx)
+    reader = fa.DeviceBgzfReader(compressed, codec, blocks_per_batch=blocks_per_batch)
+    first, n, u, b, sp, sf, sl = True, 0, 0, 0, 0, 0, 0
+    while True:
+        batch = reader.next_batch()
+        if batch is None:
+            break
+        start = batch.start
+        if first:
+            start, _ = fa.bam_parse_header(batch.data.data().to_vec()[: 1 << 20])
+            first = False
+        records = decoder.decode(batch.data, start)
+        if len(records):
+            sp += int(records.column_to_host("position").astype(np.int64).sum())
+            sf += int(records.column_to_host("flags").astype(np.uint64).sum())
+            sl += int(records.column_to_host("sequence_len").astype(np.uint64).sum())
+        n += len(records)
+        u += batch.data.byte_len() - start
+        reader.carry_from(records.tail())
+        del records
+        b += 1
+    return n, u, b, sp & ((1 << 64) - 1), sf, sl
 
-    import fritillaria_amd as fa
+
+def cpu_cores():
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(cores, 64))
+
+
+def plain_small_leg(fa, ctx, np, steps):
+    """Plain path, small: the short-read BAM of 10,000 x 150 bp records on a 1 Mbp
+    reference (level 6, noodles framing), decoded per call by the drop-in
+    reader + decoder (host-fed, one batch of 256 members), columns refID, pos,
+    flag, MAPQ, l_seq; record count and column sums against the CPU reader."""
+    import oracle
     from fritillaria_amd import synth
+    s = synth.config_bam(1)
+    reader_sums(fa, ctx, s.compressed, np)  # warm
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = reader_sums(fa, ctx, s.compressed, np)
+    dt = (time.perf_counter() - t0) / steps
+    n, u = got[0], got[1]
+    t0 = time.perf_counter()
+    rc, r = oracle.mt_bam_columns(s.compressed, cpu_cores())
+    cdt = time.perf_counter() - t0
+    require(rc == 0 and (n, got[3], got[4], got[5]) == (r["records"], r["sum_pos"], r["sum_flag"], r["sum_lseq"]),
+            f"plain small BAM: per-call {got} vs CPU reader {r}")
+    log(f"plain small BAM per call: {dt * 1e3:.2f} ms, {s.inflated_len / dt / 1e9:.3f} GB/s, {n} records")
+    return {"workload": "plain path, small: 10,000 x 150 bp records, 1 Mbp reference, level 6, noodles framing; "
+                        "DeviceBgzfReader + BamDecoder per call (host-fed), columns refID/pos/flag/MAPQ/l_seq",
+            "records": n, "inflated_bytes": int(s.inflated_len), "compressed_bytes": int(s.compressed.size),
+            "ms_per_call": round(dt * 1e3, 3), "gbps": round(s.inflated_len / dt / 1e9, 3),
+            "records_per_s": round(n / dt, 1), "calls": steps,
+            "parity": {"equal_to_cpu_reader": True, "sums": ["records", "pos", "flag", "l_seq"]},
+            "cpu_baseline": {"value": round(s.inflated_len / cdt / 1e9, 3), "unit": "GB/s", "cores": cpu_cores(),
+                             "kind": "port", "sample": f"whole file, {cdt * 1e3:.1f} ms"}}
 
+
+def long_read_leg(fa, ctx, np, steps, schedule):
+    """Unaligned long reads: 50,000 records of log-normal length (median 20 kb,
+    sigma 1, capped at 1 Mb) with MM:Z, ML:B:C and fi:B:S kinetics aux fields.
+    Batched: the whole file resident in HBM through the graph-replayed pipeline
+    with the SoA extras; per call: the host-fed reader loop (records span many
+    members, so batches carry partial records). Both against the CPU reader."""
+    import oracle
+    from fritillaria_amd import synth
+    t0 = time.time()
+    s = synth.config_bam(3)
+    C, U = int(s.compressed.size), int(s.inflated_len)
+    log(f"generated the long-read BAM: {s.n_records} records, {C} compressed / {U} inflated bytes "
+        f"in {time.time() - t0:.1f}s")
+    pipe = fa.BatchedPipeline(ctx, s.compressed, use_graph=True, inflate_schedule=schedule, soa=True)
+    pipe.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.run()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    sig = column_checksums(pipe, np)
+    fallbacks = pipe.fallbacks()
+    pipe.close()
+    timed = fa.BatchedPipeline(ctx, s.compressed, use_graph=False, inflate_schedule=schedule, soa=True)
+    timed.run()
+    stages = {k: round(v, 3) for k, v in timed.stage_ms().items()}
+    timed.close()
+    t0 = time.perf_counter()
+    pc = reader_sums(fa, ctx, s.compressed, np)
+    pdt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc, r = oracle.mt_bam_columns(s.compressed, cpu_cores(), soa=True)
+    cdt = time.perf_counter() - t0
+    mism = [k for k in sig if sig[k] != r[k]]
+    require(rc == 0 and not mism, f"long-read BAM: batched columns differ from the CPU reader's in {mism}")
+    require((pc[0], pc[3], pc[4], pc[5]) == (r["records"], r["sum_pos"], r["sum_flag"], r["sum_lseq"]),
+            f"long-read BAM: per-call {pc} vs CPU reader {r}")
+    log(f"long-read BAM batched: {dt * 1e3:.2f} ms/step, {U / dt / 1e9:.2f} GB/s, fallbacks {fallbacks}, "
+        f"stages {stages}; per call {U / pdt / 1e9:.2f} GB/s")
+    return {"workload": "unaligned long-read BAM: 50,000 records, log-normal length (median 20 kb, sigma 1.0, "
+                        "cap 1 Mb), MM:Z + ML:B:C + fi:B:S kinetics, level 6, noodles framing",
+            "records": sig["records"], "members": None, "inflated_bytes": U, "compressed_bytes": C,
+            "batched": {"ms_per_step": round(dt * 1e3, 3), "gbps": round(U / dt / 1e9, 3),
+                        "records_per_s": round(sig["records"] / dt, 1), "steps": steps, "graph": True,
+                        "soa": True, "fallbacks": fallbacks, "stage_ms": stages},
+            "per_call": {"seconds": round(pdt, 3), "gbps": round(U / pdt / 1e9, 3),
+                         "records_per_s": round(pc[0] / pdt, 1), "batches": pc[2], "blocks_per_batch": 256,
+                         "host_fed": True},
+            "parity": {"column_checksums": sig, "equal_to_cpu_reader": True},
+            "cpu_baseline": {"value": round(r["inflated"] / cdt / 1e9, 3), "unit": "GB/s", "cores": cpu_cores(),
+                             "kind": "port", "sample": f"whole file with the SoA extras, {cdt:.2f} s"}}
+
+
+def bcf_fastq_leg(fa, ctx, np, steps, schedule):
+    """The mixed configuration at full size, decoded together per call
+    (frhip_bcf_fastq_decode, host-fed); record counts and column sums against
+    the oracle's BCF and FASTQ scans of the same streams."""
+    import oracle
+    from fritillaria_amd import synth
     t0 = time.time()
     b = synth.config_bcf()
     f = synth.config_fastq()
     log(f"generated BCF {b.n_records} sites ({b.compressed.size} -> {b.inflated.size} bytes), FASTQ {f.n_records} "
         f"records ({f.compressed.size} -> {f.inflated.size} bytes) in {time.time() - t0:.1f}s")
-    ctx = fa.HipContext(0)
-    schedule = args.schedule if args.schedule >= 0 else 16
     ctx.set_inflate_schedule(schedule)
-    both, spans, n_bcf = fa.join_bcf_fastq(b.compressed, fa.discover(b.compressed)[0], f.compressed,
-                                           fa.discover(f.compressed)[0])
+    sb, sf = fa.discover(b.compressed)[0], fa.discover(f.compressed)[0]
+    both, spans, n_bcf = fa.join_bcf_fastq(b.compressed, sb, f.compressed, sf)
     U = int(b.inflated.size) + int(f.inflated.size)
-    for _ in range(args.warmup):
-        r = fa.decode_bcf_fastq_joined(ctx, both, spans, n_bcf)
-        del r
-    ctx.synchronize()
+    r = fa.decode_bcf_fastq_joined(ctx, both, spans, n_bcf)
+    del r
     times = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         t0 = time.perf_counter()
         r = fa.decode_bcf_fastq_joined(ctx, both, spans, n_bcf)
         times.append(time.perf_counter() - t0)
-        nb, nf = len(r.bcf), len(r.fastq)
-        del r
-    dt = sum(times)
-    ms = dt * 1e3 / args.steps
-    assert nb == b.n_records and nf == f.n_records, (nb, nf)
-    line = {"metric": "BCF+FASTQ GB/s inflated→HBM columns, per call (bit-exact)", "value": round(U / (dt / args.steps) / 1e9, 3),
-            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        if _ < steps - 1:
+            del r
+    gb = {k: r.bcf.column_to_host(k) for k in ("record_offsets", "position", "allele_count", "record_end")}
+    gf = {k: r.fastq.column_to_host(k) for k in ("record_offsets", "sequence_len", "quality_start")}
+    del r
+    dt = sum(times) / steps
+    # the CPU path over the same streams: zlib inflate of both on the host cores
+    # (member-parallel, as noodles' multithreaded reader), then the oracle's BCF
+    # and FASTQ record walks; it is the baseline and, with its columns, the checker
+    t0 = time.perf_counter()
+    rc0, ub = oracle.mt_inflate(b.compressed, cpu_cores())
+    rc4, uf = oracle.mt_inflate(f.compressed, cpu_cores())
+    t_inf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc1, off_b, tail_b, _ = oracle.bcf_scan_records(ub, b.records_start)
+    rc2, cols_b, _ = oracle.bcf_decode_columns(ub, off_b, tail_b)
+    rc3, off_f, bounds_f, _, _ = oracle.fastq_scan_records(uf, 0)
+    t_scan = time.perf_counter() - t0
+    require(rc0 == 0 and rc4 == 0, "CPU inflate of the BCF / FASTQ streams failed")
+    del ub, uf
+    require(rc1 == 0 and rc2 == 0 and rc3 == 0, "oracle BCF/FASTQ scans failed")
+    u64 = lambda a: int(np.asarray(a).astype(np.uint64).sum())
+    want = {"bcf_records": len(off_b), "bcf_sum_off": u64(off_b), "bcf_sum_pos": u64(cols_b["position"]),
+            "bcf_sum_alleles": u64(cols_b["allele_count"]), "bcf_sum_end": u64(cols_b["record_end"]),
+            "fastq_records": len(off_f), "fastq_sum_off": u64(off_f), "fastq_sum_len": u64(bounds_f[:, 3]),
+            "fastq_sum_qual_start": u64(bounds_f[:, 2])}
+    got = {"bcf_records": len(gb["record_offsets"]), "bcf_sum_off": u64(gb["record_offsets"]),
+           "bcf_sum_pos": u64(gb["position"]), "bcf_sum_alleles": u64(gb["allele_count"]),
+           "bcf_sum_end": u64(gb["record_end"]), "fastq_records": len(gf["record_offsets"]),
+           "fastq_sum_off": u64(gf["record_offsets"]), "fastq_sum_len": u64(gf["sequence_len"]),
+           "fastq_sum_qual_start": u64(gf["quality_start"])}
+    mism = [k for k in want if want[k] != got[k]]
+    require(not mism and got["bcf_records"] == b.n_records and got["fastq_records"] == f.n_records,
+            f"BCF + FASTQ columns differ from the oracle's in {mism}: {got} vs {want}")
+    n_rec = got["bcf_records"] + got["fastq_records"]
+    log(f"BCF + FASTQ per call: {dt * 1e3:.2f} ms, {U / dt / 1e9:.2f} GB/s, {n_rec / dt / 1e6:.1f} M records/s")
+    return {"workload": "mixed BCF (200,000 sites x 2,504 diploid samples, GT only, INFO AC/AN/AF) + bgzipped "
+                        "FASTQ (5,000,000 x 150 bp), one inflate launch for both, then both boundary scans and "
+                        "column decodes; host-fed (compressed bytes uploaded each call)",
+            "members": int(spans.size), "bcf_members": n_bcf, "compressed_bytes": int(both.size),
+            "inflated_bytes": U, "inflate_schedule": schedule, "calls": steps,
+            "ms_per_call": round(dt * 1e3, 3), "gbps": round(U / dt / 1e9, 3),
+            "records_per_s": round(n_rec / dt, 1), "p50_ms": round(sorted(times)[len(times) // 2] * 1e3, 3),
+            "parity": {"checksums": got, "equal_to_oracle_scans": True},
+            "cpu_baseline": {"value": round(U / (t_inf + t_scan) / 1e9, 3), "unit": "GB/s", "cores": cpu_cores(),
+                             "kind": "port",
+                             "sample": f"both whole streams: zlib 1.2.11 inflate on {cpu_cores()} threads "
+                                       f"({t_inf:.2f} s), then the single-threaded BCF and FASTQ record walks "
+                                       f"({t_scan:.2f} s)"}}
+
+
+def bcf_fastq_bench(args):
+    """The mixed configuration alone (--workload bcf-fastq): the BCF of 200,000
+    sites x 2,504 samples and the bgzipped FASTQ of 5,000,000 x 150 bp reads,
+    decoded together per call, host-fed, with the same enforced column parity
+    and CPU baseline as the leg of the default run. Prints one JSON line."""
+    import numpy as np
+
+    import fritillaria_amd as fa
+
+    ctx = fa.HipContext(0)
+    schedule = args.schedule if args.schedule >= 0 else 16
+    leg = bcf_fastq_leg(fa, ctx, np, steps=args.steps, schedule=schedule)
+    line = {"metric": "BCF+FASTQ GB/s inflated→HBM columns, per call (bit-exact)", "value": leg["gbps"],
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": leg["ms_per_call"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-            "config": {"workload": "mixed BCF (200,000 sites x 2,504 diploid samples, GT only) + bgzipped FASTQ "
-                                   "(5,000,000 x 150 bp), one inflate launch for both, then both boundary scans "
-                                   "and column decodes; host-fed (compressed bytes uploaded each call)",
-                       "members": int(spans.size), "bcf_members": n_bcf, "compressed_bytes": int(both.size),
-                       "inflated_bytes": U, "inflate_schedule": schedule},
-            "records_per_s": round((nb + nf) / (dt / args.steps), 1), "bcf_records": nb, "fastq_records": nf,
-            "p50_ms": round(sorted(times)[len(times) // 2] * 1e3, 3), "max_ms": round(max(times) * 1e3, 3)}
+            "config": {k: leg[k] for k in ("workload", "members", "bcf_members", "compressed_bytes",
+                                           "inflated_bytes", "inflate_schedule")},
+            "records_per_s": leg["records_per_s"], "p50_ms": leg["p50_ms"], "parity": leg["parity"],
+            "cpu_baseline": leg["cpu_baseline"]}
     print(json.dumps(line), flush=True)
 
 
@@ -232,6 +393,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--workload", choices=["bam", "bcf-fastq"], default="bam",
                     help="bcf-fastq: the mixed BCF + bgzipped FASTQ configuration, decoded together per call")
+    ap.add_argument("--configs", default="1,3,4",
+                    help="other configurations measured after the headline one: 1 plain small BAM per call, "
+                         "3 long-read BAM (batched and per call), 4 mixed BCF + FASTQ; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -357,6 +521,16 @@ def main():
                "records_match": r["records"] == n_rec}
         log(f"cpu baseline: {cpu['value']:.2f} GB/s on {cores} threads ({cdt:.2f} s)")
 
+    configs = {}
+    want = [] if args.configs == "none" else [c.strip() for c in args.configs.split(",") if c.strip()]
+    del pipe
+    if "1" in want:
+        configs["plain_small_bam"] = plain_small_leg(fa, ctx, np, steps=5)
+    if "3" in want:
+        configs["long_read_bam"] = long_read_leg(fa, ctx, np, steps=3, schedule=args.schedule)
+    if "4" in want:
+        configs["bcf_fastq"] = bcf_fastq_leg(fa, ctx, np, steps=3, schedule=args.schedule)
+
     variant = fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None
     two_phase = variant is not None and variant[3] == 3
     inflate_kernel = ("fr_inflate_lanes_kernel + fr_inflate_resolve_kernel (two-phase inflate: lane-per-member "
@@ -414,6 +588,7 @@ def main():
         "serving": serving,
         "parity": {"column_checksums": batched_sig, "equal_to_cpu_reader": True,
                    "inflated_crc_recheck_mismatches": bad_members},
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

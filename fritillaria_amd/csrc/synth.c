@@ -106,7 +106,7 @@ static void make_shape(const synth_bam_params *P, uint64_t i, shape_t *s)
         L = P->len_min + rbelow(&r, P->len_max - P->len_min + 1);
     } else {
         double v = exp(log(P->ln_median) + P->ln_sigma * rnorm(&r));
-        if (v > P->len_cap) v = P->len_cap;
+        if (v > P->len_cap || (P->cap_every && i % P->cap_every == P->cap_every / 2)) v = P->len_cap;
         if (v < 1) v = 1;
         L = (uint32_t)v;
     }

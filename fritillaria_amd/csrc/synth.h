@@ -19,6 +19,7 @@ typedef struct {
     int32_t unaligned;   /* refID -1, pos -1, flag 4, no CIGAR */
     uint32_t ref_len;
     int32_t threads;
+    uint32_t cap_every;  /* log-normal lengths: records k*cap_every + cap_every/2 take len_cap (0: none) */
 } synth_bam_params;
 
 typedef struct {

@@ -21,7 +21,7 @@ class _BamParams(C.Structure):
                 ("len_fixed", C.c_uint32), ("len_min", C.c_uint32), ("len_max", C.c_uint32),
                 ("ln_median", C.c_double), ("ln_sigma", C.c_double), ("len_cap", C.c_uint32),
                 ("indel_p", C.c_double), ("aux_mode", C.c_int32), ("unaligned", C.c_int32),
-                ("ref_len", C.c_uint32), ("threads", C.c_int32)]
+                ("ref_len", C.c_uint32), ("threads", C.c_int32), ("cap_every", C.c_uint32)]
 
 
 class _BgzfParams(C.Structure):
@@ -62,8 +62,10 @@ def default_threads() -> int:
 
 def bam_stream(n_records: int, seed: int, *, read_len=150, read_range=None, lognormal=None,
                indel_p=0.0, long_aux=False, unaligned=False, ref_len=1_000_000,
-               threads=None):
-    """Uncompressed BAM stream (header + records). Returns (bytes array, records_start)."""
+               threads=None, cap_every=0):
+    """Uncompressed BAM stream (header + records). Returns (bytes array, records_start).
+    With log-normal lengths, cap_every > 0 gives every cap_every-th record the
+    length cap (tests of records that span many members)."""
     p = _BamParams()
     p.n_records = n_records
     p.seed = seed
@@ -81,6 +83,7 @@ def bam_stream(n_records: int, seed: int, *, read_len=150, read_range=None, logn
     p.unaligned = 1 if unaligned else 0
     p.ref_len = ref_len
     p.threads = threads or default_threads()
+    p.cap_every = cap_every
     ptr = C.c_void_p()
     n = C.c_uint64()
     rs = C.c_uint64()
@@ -123,9 +126,11 @@ class SynthBam:
 CONFIG_SEED = 0x5EED0000
 
 
-def config_bam(config: int, n_records=None, threads=None) -> SynthBam:
+def config_bam(config: int, n_records=None, threads=None, cap_every=0) -> SynthBam:
     """BAM inputs of the benchmark configurations (1: short reads, 2: batched large,
-    3: unaligned long reads); `n_records` scales a configuration down for tests."""
+    3: unaligned long reads); `n_records` scales a configuration down for tests,
+    and for configuration 3 `cap_every` forces every cap_every-th read to the
+    1 Mb cap. Configurations 1 and 2 share the generator: one 1 Mbp reference."""
     seed = CONFIG_SEED + config
     if config == 1:
         n = n_records or 10_000
@@ -133,13 +138,13 @@ def config_bam(config: int, n_records=None, threads=None) -> SynthBam:
         c = bgzf(u, level=6, seed=seed, threads=threads)
     elif config == 2:
         n = n_records or 20_000_000
-        u, rs = bam_stream(n, seed, read_range=(100, 250), indel_p=0.01, ref_len=250_000_000,
+        u, rs = bam_stream(n, seed, read_range=(100, 250), indel_p=0.01, ref_len=1_000_000,
                            threads=threads)
         c = bgzf(u, mixed_levels=True, stored_frac=0.01, seed=seed, threads=threads)
     elif config == 3:
         n = n_records or 50_000
         u, rs = bam_stream(n, seed, lognormal=(20_000.0, 1.0, 1_000_000), long_aux=True,
-                           unaligned=True, ref_len=1_000_000, threads=threads)
+                           unaligned=True, ref_len=1_000_000, threads=threads, cap_every=cap_every)
         c = bgzf(u, level=6, seed=seed, threads=threads)
     else:
         raise ValueError(f"no BAM configuration {config}")
@@ -167,14 +172,18 @@ def fastq_stream(n_records: int, seed: int, read_len: int = 150) -> np.ndarray:
         out[:, 13 - d] = ord("0") + (idx // 10 ** d) % 10
     out[:, 14] = ord("\n")
     s0 = name_len
-    out[:, s0:s0 + read_len] = np.frombuffer(b"ACGT", dtype=np.uint8)[rng.integers(0, 4, (n_records, read_len))]
+    out[:, s0:s0 + read_len] = np.frombuffer(b"ACGT", dtype=np.uint8)[
+        rng.integers(0, 4, (n_records, read_len), dtype=np.uint8)]
     out[:, s0 + read_len] = ord("\n")
     p0 = s0 + read_len + 1
     out[:, p0] = ord("+")
     out[:, p0 + 1] = ord("\n")
     q0 = p0 + 2
-    q = np.clip(np.rint(rng.normal(35.0, 5.0, (n_records, read_len))), 2, 41).astype(np.uint8) + 33
-    out[:, q0:q0 + read_len] = q
+    # round(N(35, 5)) clipped to [2, 41] by inverse CDF: a uniform 16-bit draw
+    # indexes the distribution's quantiles at (k + 0.5) / 65536
+    from scipy.stats import norm
+    table = (np.clip(np.rint(norm.ppf((np.arange(65536) + 0.5) / 65536, 35.0, 5.0)), 2, 41) + 33).astype(np.uint8)
+    out[:, q0:q0 + read_len] = table[rng.integers(0, 65536, (n_records, read_len), dtype=np.uint16)]
     out[:, q0 + read_len] = ord("\n")
     return out.reshape(-1)
 

@@ -98,6 +98,8 @@ def lib():
         _lib.or_bam_soa.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_void_p] * 9 + [C.POINTER(C.c_uint64)]
         _lib.or_mt_bam_columns.restype = C.c_int
         _lib.or_mt_bam_columns.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_void_p]
+        _lib.or_mt_inflate.restype = C.c_int
+        _lib.or_mt_inflate.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]
         del u8p
     return _lib
 
@@ -281,6 +283,19 @@ def mt_bam_columns(file_bytes, nthreads: int, soa: bool = False):
         out.update({"ops": int(res[7]), "sum_ops": int(res[8]), "seq_bytes": int(res[9]),
                     "sum_seq": int(res[10]), "aux": int(res[11]), "sum_aux_value_offset": int(res[12])})
     return rc, out
+
+
+def mt_inflate(file_bytes, nthreads: int):
+    """Whole-stream BGZF inflate with zlib on `nthreads` workers (or_mt_inflate):
+    (rc, inflated bytes)."""
+    a = _as_u8(file_bytes)
+    n = C.c_uint64(0)
+    rc = lib().or_mt_inflate(_ptr(a), a.size, nthreads, None, C.byref(n))
+    if rc != 0:
+        return rc, None
+    out = np.empty(max(n.value, 1), np.uint8)
+    rc = lib().or_mt_inflate(_ptr(a), a.size, nthreads, _ptr(out), C.byref(n))
+    return rc, out[: n.value]
 
 
 # ---------------------------------------------------------------------------

@@ -219,3 +219,38 @@ join:
     free(c_nref); free(c_npos); free(c_tlen); free(c_off);
     return rc;
 }
+
+/*
+ * Inflates a whole BGZF stream held in memory with `nthreads` workers (zlib,
+ * CRC32 and ISIZE checked per member), the same member-parallel inflate the
+ * BAM reader above uses, for formats whose record walk runs afterwards on the
+ * whole buffer (the BCF and FASTQ baselines). *out_len receives the inflated
+ * size; out must hold it (call with out == NULL to learn it). Returns 0, or <0
+ * on a framing / inflate error.
+ */
+int or_mt_inflate(const uint8_t *file, uint64_t len, int nthreads, uint8_t *out, uint64_t *out_len)
+{
+    uint64_t cap = len / 28 + 2;
+    or_span_t *spans = (or_span_t *)malloc(sizeof(or_span_t) * cap);
+    uint64_t position = 0, eoff = 0;
+    int err = 0;
+    uint64_t n = or_discover(file, len, 0, spans, cap, &position, &err, &eoff);
+    if (err || position != len) { free(spans); return -1; }
+    uint64_t *out_off = (uint64_t *)malloc(sizeof(uint64_t) * (n + 1));
+    out_off[0] = 0;
+    for (uint64_t i = 0; i < n; ++i) out_off[i + 1] = out_off[i] + spans[i].isize;
+    *out_len = out_off[n];
+    if (!out) { free(out_off); free(spans); return 0; }
+    atomic_int *done = (atomic_int *)calloc(n ? n : 1, sizeof(atomic_int));
+    mt_ctx_t c = {file, spans, n, out_off, out, 0, done};
+    atomic_init(&c.next, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, worker, &c);
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    int rc = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        if (atomic_load(&done[i]) != 1) rc = -2;
+    free(th); free(done); free(out_off); free(spans);
+    return rc;
+}

@@ -66,6 +66,11 @@ def test_synthetic_bcf_and_fastq_together(hip_ctx, codec, oracle_mod, two_phase)
     from fritillaria_amd import synth
     b = synth.config_bcf(2000)
     f = synth.config_fastq(60_000)
+    run_and_compare(hip_ctx, codec, oracle_mod, b, f, two_phase)
+
+
+def run_and_compare(hip_ctx, codec, oracle_mod, b, f, two_phase):
+    import fritillaria_amd as fa
     if two_phase:
         hip_ctx.set_inflate_schedule(16)
         hip_ctx.set_two_phase_min_members(0)
@@ -74,7 +79,7 @@ def test_synthetic_bcf_and_fastq_together(hip_ctx, codec, oracle_mod, two_phase)
     finally:
         hip_ctx.set_inflate_schedule(5)
         hip_ctx.set_two_phase_min_members(8192)
-    assert len(gb["record_offsets"]) == 2000 and len(gf["record_offsets"]) == 60_000
+    assert len(gb["record_offsets"]) == b.n_records and len(gf["record_offsets"]) == f.n_records
     bcf_alone = fa.BcfScanner(hip_ctx).decode(
         codec.inflate_batch_device(b.compressed, fa.discover(b.compressed)[0]), b.records_start, b.samples,
         b.contigs).to_host()
@@ -84,6 +89,17 @@ def test_synthetic_bcf_and_fastq_together(hip_ctx, codec, oracle_mod, two_phase)
         assert np.array_equal(gb[name], bcf_alone[name]), name
     for name in fq_alone:
         assert np.array_equal(gf[name], fq_alone[name]), name
+
+
+def test_mixed_at_twenty_thousand_sites(hip_ctx, codec, oracle_mod):
+    """The mixed configuration at a tenth of its full size: 20,000 sites x 2,504
+    samples (about 100 MB inflated, 2,000 members) and 500,000 reads, on the
+    two-phase inflate (enough members to run it), against the oracle scans."""
+    print("[gpu] bcf+fastq together at 20,000 sites / 500,000 reads", flush=True)
+    from fritillaria_amd import synth
+    b = synth.config_bcf(20_000)
+    f = synth.config_fastq(500_000)
+    run_and_compare(hip_ctx, codec, oracle_mod, b, f, True)
 
 
 @pytest.mark.parametrize("bcf_name,fq_name", [("kg_phase3.bcf", "illumina.fastq.gz"),

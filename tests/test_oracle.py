@@ -268,3 +268,22 @@ def test_mt_baseline_soa_checksums_match_serial_soa(oracle_mod):
         assert rc == 0 and r["records"] == 2500
         for k, v in want.items():
             assert r[k] == v, k
+
+
+def test_mt_inflate_equals_zlib_per_member(oracle_mod):
+    """The multithreaded whole-stream inflate (the BCF / FASTQ CPU baseline's)
+    gives Python zlib's bytes, and fails on a bad trailer CRC."""
+    import zlib
+    rng = np.random.default_rng(8)
+    data = bytes(rng.integers(0, 4, 300_000, dtype=np.uint8) + 65)
+    f = bgzf(data, cap=7000)
+    for t in (1, 5):
+        rc, u = oracle_mod.mt_inflate(f, t)
+        assert rc == 0 and u.tobytes() == data
+    bad = bytearray(f)
+    d = oracle_mod.discover(f)
+    at = int(d.spans["payload_start"][3] + d.spans["payload_len"][3])
+    bad[at] ^= 1
+    rc, _ = oracle_mod.mt_inflate(bytes(bad), 3)
+    assert rc != 0
+    assert zlib.decompress(f[18:18 + int(d.spans["payload_len"][0])], -15) == data[:7000]
