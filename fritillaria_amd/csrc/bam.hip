@@ -26,6 +26,10 @@ namespace frhip {
 namespace {
 
 enum : uint64_t { BAM_OK = 0, BAM_MALFORMED = 1 };
+// Members the parallel reconcile looks back over for the closest one holding a
+// record start (runs of members inside one long record); longer runs take the
+// serial reconcile.
+constexpr uint32_t FAST_MAX_HOPS = 4096;
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uni64(uint64_t v)
@@ -277,8 +281,11 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
         }
         return;
     }
-    const uint64_t T = (sb1 - sb) > a.window ? sb + a.window : sb1;
-    // candidate walks to the window end
+    // candidate walks to the window end; a window without survivors grows by
+    // another window, up to the member end, so that a member left with none
+    // holds no record start at all (the true start passes the filter): the
+    // chain then passes it by, which the parallel reconcile relies on
+    uint64_t T = (sb1 - sb) > a.window ? sb + a.window : sb1;
     uint32_t n_surv = 0;
     int agree = 1;
     uint64_t L = 0;
@@ -315,6 +322,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
             }
             n_surv += (uint32_t)__popcll(m);
         }
+        if (base + 64 >= T && n_surv == 0 && T < sb1) T = (sb1 - T) > a.window ? T + a.window : sb1;
     }
     agree = (int)uni((uint32_t)(__ballot(!agree) == 0));
     if (n_surv == 0 || !agree) {
@@ -555,15 +563,35 @@ __global__ __launch_bounds__(256) void bam_fast_check_kernel(BamReconcileArgs a)
         a.mode[j] = 0;
         return;
     }
-    // entered at the speculative exit of the closest non-empty member before it
+    // entered at the speculative exit of the closest earlier member holding a
+    // record start; the members between hold none (their speculation scanned
+    // them whole and found no candidate), so the chain passes them by, which
+    // each of them checks for itself below
     uint64_t e = a.start_dev ? *a.start_dev : a.start;
-    bool ok = a.spec_ok[j] != 0;
-    uint32_t i = j;
-    while (i > 0 && a.block_starts[i] == a.block_starts[i - 1]) --i;
-    if (i > 0) {
-        ok = ok && a.spec_ok[i - 1] != 0;
-        e = a.exitp[i - 1];
+    bool ok = true;
+    for (uint32_t i = j, hops = 0; i > 0; --i) {
+        const uint32_t k = i - 1;
+        if (a.block_starts[k + 1] == a.block_starts[k]) continue;
+        if (a.n_surv[k] == 0) {
+            if (++hops > FAST_MAX_HOPS) {
+                ok = false;
+                break;
+            }
+            continue;
+        }
+        ok = a.spec_ok[k] != 0;
+        e = a.exitp[k];
+        break;
     }
+    if (a.n_surv[j] == 0) {
+        // no record starts here: the chain must pass the member by
+        a.entry[j] = BAM_NONE;
+        a.count[j] = 0;
+        a.mode[j] = 0;
+        if (!ok || e < s1) atomicMin(&a.fast_state[0], j);
+        return;
+    }
+    ok = ok && a.spec_ok[j] != 0;
     uint32_t found = 0, cnt = 0;
     if (ok && e < s1) {
         const uint32_t ns = a.n_surv[j];
@@ -601,11 +629,12 @@ __global__ __launch_bounds__(256) void bam_fast_finish_kernel(BamReconcileArgs a
         }
     }
     if (j == 0) {
-        // tail: the exit of the last non-empty member the chain reached
+        // tail: the exit of the last member the chain entered (members it
+        // passed by, and empty ones, have no exit of their own)
         uint32_t t = last;
-        while (t > 0 && a.block_starts[t + 1] == a.block_starts[t]) --t;
+        while (t > 0 && a.entry[t] == BAM_NONE) --t;
         a.totals[0] = a.fast_scan[last + 1];
-        a.totals[1] = a.exitp[t];
+        a.totals[1] = a.entry[t] == BAM_NONE ? (a.start_dev ? *a.start_dev : a.start) : a.exitp[t];
         a.totals[2] = BAM_OK;
         a.totals[3] = 0;
         a.fast_state[2] = 1;
