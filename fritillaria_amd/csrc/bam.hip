@@ -187,6 +187,78 @@ __device__ __forceinline__ Step cand_step(const uint8_t *buf, uint64_t len, uint
     return {end, 0};
 }
 
+// Whether a candidate's landing past the window can start a record: the
+// fixed fields of a BAM record there (block_size at least 32, IDs and
+// positions at least -1, a name of at least one character and its NUL). A
+// landing within 36 bytes of the buffer end, or whose name runs past it,
+// passes (the tail is the reconcile's business). Checked only where a walk
+// leaves its window: a true record's successor always passes, while a chance
+// match in random bytes (quality strings, kinetics arrays) almost never does,
+// so members a long record covers keep no false survivors.
+__device__ __forceinline__ bool landing_plausible(const uint8_t *buf, uint64_t len, uint64_t at)
+{
+    if (at + 36 > len) return true;
+    uint32_t w[9];
+    fixed_words(buf, len, at, w);
+    const uint32_t name_len = w[3] & 0xFF;
+    if (w[0] < 32 || name_len < 2) return false;
+    if ((int32_t)w[1] < -1 || (int32_t)w[2] < -1 || (int32_t)w[6] < -1 || (int32_t)w[7] < -1) return false;
+    return at + 36 + name_len > len || buf[at + 35 + name_len] == 0;
+}
+
+// The first position in [from, to) whose fixed fields could begin a record
+// (block_size at least 32 and inside the buffer, IDs and positions at least -1,
+// a name of at least two bytes ending in its NUL, the fields fitting the
+// block): every record
+// that passes cand_step passes this, so no candidate before it can survive.
+// A wave tests 1,024 positions per step, each lane 16 consecutive ones from
+// three aligned 16-byte loads; within 48 bytes of the buffer end every
+// position counts (the candidate walk decides there). Returns `to` when none.
+__device__ uint64_t skim_bam(const uint8_t *buf, uint64_t len, uint64_t from, uint64_t to, uint32_t lane)
+{
+    for (uint64_t c = from - (((uintptr_t)(buf + from)) & 15); c < to; c += 1024) {
+        const uint64_t p0 = c + 16ull * lane;
+        uint32_t mask = 0;
+        if (p0 + 48 <= len) {
+            const uint4 *q = reinterpret_cast<const uint4 *>(buf + p0);
+            const uint4 v0 = q[0], v1 = q[1], v2 = q[2];
+            const uint32_t d[12] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, v2.z, v2.w};
+#pragma unroll
+            for (uint32_t o = 0; o < 16; ++o) {
+                uint32_t w[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    w[k] = (o & 3) ? __builtin_amdgcn_alignbit(d[(o >> 2) + k + 1], d[(o >> 2) + k], (o & 3) * 8)
+                                   : d[(o >> 2) + k];
+                const uint64_t p = p0 + o;
+                const uint32_t name_len = w[3] & 0xFF;
+                const uint64_t need = 36ull + name_len + 4ull * (w[4] & 0xFFFF) + ((uint64_t)w[5] + 1) / 2 + w[5];
+                const bool ok = p >= from && p < to && w[0] >= 32 && p + 4 + (uint64_t)w[0] <= len &&
+                                (int32_t)w[1] >= -1 && (int32_t)w[2] >= -1 && (int32_t)w[6] >= -1 &&
+                                (int32_t)w[7] >= -1 && name_len >= 2 && need <= 4ull + w[0];
+                mask |= (uint32_t)ok << o;
+            }
+            // and the name's NUL (one byte load for the ~2% of random
+            // positions that get this far)
+            for (uint32_t m = mask; m; m &= m - 1) {
+                const uint32_t o = (uint32_t)__builtin_ctz(m);
+                const uint64_t p = p0 + o;
+                const uint64_t at = p + 35 + buf[p + 12];
+                if (at < len && buf[at] != 0) mask &= ~(1u << o);
+            }
+        } else {
+            for (uint32_t o = 0; o < 16; ++o) mask |= (uint32_t)(p0 + o >= from && p0 + o < to) << o;
+        }
+        const uint64_t bal = __ballot(mask != 0);
+        if (bal) {
+            const uint32_t l = (uint32_t)__ffsll((unsigned long long)bal) - 1;
+            const uint32_t mk = (uint32_t)__shfl((int)mask, (int)l, 64);
+            return c + 16ull * l + (uint32_t)__builtin_ctz(mk);
+        }
+    }
+    return to;
+}
+
 // BCF: one step of scan_records (crates/fritillaria-bcf/src/columnar/record.rs:60-95):
 // fewer than 8 bytes or a record past the buffer is the tail, l_shared < 24 is
 // malformed.
@@ -306,6 +378,7 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
                 ++cnt;
                 cur = s.next;
             }
+            if (F == CHAIN_BAM && alive && !landing_plausible(a.buf, len, cur)) alive = 0;
         }
         uint64_t m = __ballot(alive);
         if (m) {
@@ -322,7 +395,18 @@ __global__ __launch_bounds__(64) void bam_spec_kernel(BamSpecArgs a)
             }
             n_surv += (uint32_t)__popcll(m);
         }
-        if (base + 64 >= T && n_surv == 0 && T < sb1) T = (sb1 - T) > a.window ? T + a.window : sb1;
+        if (base + 64 >= T && n_surv == 0 && T < sb1) {
+            // no survivor yet: the next window starts at the first position
+            // that could begin a record (BAM), or right after this one (BCF)
+            const uint64_t q = F == CHAIN_BAM ? skim_bam(a.buf, len, T, sb1, lane) : T;
+            if (q >= sb1) {
+                T = sb1;
+                base = sb1;
+            } else {
+                base = q - 64; // the loop step moves it to q
+                T = (sb1 - q) > a.window ? q + a.window : sb1;
+            }
+        }
     }
     agree = (int)uni((uint32_t)(__ballot(!agree) == 0));
     if (n_surv == 0 || !agree) {
