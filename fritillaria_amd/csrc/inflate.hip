@@ -2272,33 +2272,74 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
-// CRC32 of independent byte ranges, one wave per range: lane l takes the l-th
-// of 64 equal segments; segments fold with x^(8 * bytes after the segment).
-__global__ __launch_bounds__(64) void fr_crc32_kernel(const uint8_t *data, const uint64_t *off,
-                                                      const uint32_t *len, uint32_t n, uint32_t *crc,
-                                                      const uint32_t *x2n)
+// CRC32 of independent byte ranges data[off[m], off[m] + len[m]), one wave per
+// range, four waves per workgroup. A range is a head of up to 15 bytes, a body
+// of 16-byte aligned chunks and a tail of up to 15 bytes. Lane l takes body
+// chunks l, l + 64, ...: each step the wave loads 1 KiB with one coalesced
+// 16-byte load per lane; a chunk's raw CRC is sixteen slice-by-16 table
+// lookups, and the lane folds its chunks by Horner's rule, multiplying its
+// accumulator by x^(8*1024) (the 1 KiB between a lane's chunks) with four
+// 256-entry tables. The lanes' accumulators, weighted by the bytes after their
+// last chunk, and the head and tail CRCs fold into the range's CRC.
+constexpr uint32_t CRC_WAVES = 4;
+
+__global__ __launch_bounds__(64 * CRC_WAVES) void fr_crc32_kernel(const uint8_t *data, const uint64_t *off,
+                                                                  const uint32_t *len, uint32_t n, uint32_t *crc,
+                                                                  const uint32_t *x2n)
 {
-    __shared__ uint32_t tab[256];
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < 256; i += 64) {
-        uint32_t c = i;
+    __shared__ uint32_t s16[16][256]; // raw CRC of byte v followed by k zero bytes
+    __shared__ uint32_t zt[4][256];   // (v << 8j) * x^(8*1024) mod P
+    const uint32_t t = threadIdx.x;
+    {
+        uint32_t c = t;
         for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (FR_CRC_POLY & (0u - (c & 1u)));
-        tab[i] = c;
+        s16[0][t] = c;
     }
     __syncthreads();
-    for (uint32_t m = blockIdx.x; m < n; m += gridDim.x) {
-        const uint8_t *p = data + off[m];
-        const uint32_t L = len[m];
-        const uint32_t S = (L + 63) / 64;
-        uint32_t lo = lane * S, hi = lo + S;
-        if (lo > L) lo = L;
-        if (hi > L) hi = L;
-        uint32_t r = 0;
-        for (uint32_t k = lo; k < hi; ++k) r = tab[(r ^ p[k]) & 0xFF] ^ (r >> 8);
-        r = fr_mulmod(r, fr_xpow8(x2n, L - hi));
+    for (int k = 1; k < 16; ++k) {
+        const uint32_t v = s16[k - 1][t];
+        s16[k][t] = s16[0][v & 0xFF] ^ (v >> 8);
+        __syncthreads();
+    }
+    const uint32_t X = fr_xpow8(x2n, 1024);
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) r ^= __shfl_xor(r, o, 64);
-        if (lane == 0) crc[m] = r ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(x2n, L)) ^ 0xFFFFFFFFu;
+    for (int j = 0; j < 4; ++j) zt[j][t] = fr_mulmod(t << (8 * j), X);
+    __syncthreads();
+    const uint32_t lane = t & 63, w = t >> 6;
+    for (uint32_t m = blockIdx.x * CRC_WAVES + w; m < n; m += gridDim.x * CRC_WAVES) {
+        const uint64_t a = off[m];
+        const uint32_t L = len[m];
+        const uint64_t e = a + L;
+        uint64_t A0 = (a + 15) & ~15ull, E0 = e & ~15ull;
+        if (A0 > e) A0 = e;
+        if (E0 < A0) E0 = A0;
+        uint32_t part = 0;
+        if (lane == 0) {
+            uint32_t r = 0;
+            for (uint64_t p = a; p < A0; ++p) r = s16[0][(r ^ data[p]) & 0xFF] ^ (r >> 8);
+            part = fr_mulmod(r, fr_xpow8(x2n, e - A0));
+        } else if (lane == 1) {
+            uint32_t r = 0;
+            for (uint64_t p = E0; p < e; ++p) r = s16[0][(r ^ data[p]) & 0xFF] ^ (r >> 8);
+            part = r;
+        }
+        const uint64_t nc = (E0 - A0) / 16;
+        const uint4 *body = reinterpret_cast<const uint4 *>(data + A0);
+        uint32_t acc = 0;
+        uint64_t last = ~0ull;
+        for (uint64_t c = lane; c < nc; c += 64) {
+            const uint4 v = body[c];
+            const uint32_t q[4] = {v.x, v.y, v.z, v.w};
+            uint32_t r = 0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) r ^= s16[15 - i][(q[i >> 2] >> (8 * (i & 3))) & 0xFF];
+            acc = zt[0][acc & 0xFF] ^ zt[1][(acc >> 8) & 0xFF] ^ zt[2][(acc >> 16) & 0xFF] ^ zt[3][acc >> 24] ^ r;
+            last = c;
+        }
+        if (last != ~0ull) part ^= fr_mulmod(acc, fr_xpow8(x2n, 16 * (nc - 1 - last) + (e - E0)));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) part ^= __shfl_xor(part, o, 64);
+        if (lane == 0) crc[m] = part ^ fr_mulmod(0xFFFFFFFFu, fr_xpow8(x2n, L)) ^ 0xFFFFFFFFu;
     }
 }
 
@@ -2308,8 +2349,9 @@ hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const u
                                uint32_t *crc, const uint32_t *x2n, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    uint32_t grid = n < 65536 ? n : 65536;
-    hipLaunchKernelGGL(fr_crc32_kernel, dim3(grid), dim3(64), 0, s, data, off, len, n, crc, x2n);
+    uint32_t grid = (n + CRC_WAVES - 1) / CRC_WAVES;
+    if (grid > 8192) grid = 8192;
+    hipLaunchKernelGGL(fr_crc32_kernel, dim3(grid), dim3(64 * CRC_WAVES), 0, s, data, off, len, n, crc, x2n);
     return hipGetLastError();
 }
 

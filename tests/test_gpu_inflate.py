@@ -174,6 +174,31 @@ def test_crc32_blocks(codec):
     assert got.tolist() == [zlib.crc32(data[o:o + n]) for o, n in zip(offs, lens)]
 
 
+def test_crc32_blocks_random_ranges(codec):
+    """Every alignment of head and tail and lengths from 0 to a whole member,
+    over a 200,000-byte device buffer, against zlib."""
+    import fritillaria_amd as fa
+    from helpers import bgzf
+    rng = np.random.default_rng(61)
+    data = rng.integers(0, 256, 200_000, dtype=np.uint8).tobytes()
+    raw = bgzf(data, cap=65000, level=1)
+    b = codec.inflate_batch_device(raw, fa.discover(raw)[0])
+    assert b.data().to_vec().tobytes() == data
+    offs, lens = [], []
+    for o in range(0, 40):
+        for n in (0, 1, 15, 16, 17, 31, 1023, 1024, 1025, 4096 + 7):
+            offs.append(o * 997 % 50_000)
+            lens.append(n)
+    for _ in range(300):
+        o = int(rng.integers(0, 199_000))
+        offs.append(o)
+        lens.append(int(rng.integers(0, min(70_000, 200_000 - o) + 1)))
+    offs.append(0)
+    lens.append(200_000)
+    got = codec.crc32_blocks(b.data(), offs, lens)
+    assert got.tolist() == [zlib.crc32(data[o:o + n]) for o, n in zip(offs, lens)]
+
+
 def test_device_copy_kernel(hip_ctx):
     import fritillaria_amd as fa
     import numpy as np
