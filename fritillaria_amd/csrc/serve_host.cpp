@@ -43,6 +43,29 @@ struct Plan {
 
 } // namespace
 
+// One batch in flight: its own stream, scratch, columns and host staging, so
+// that several batches can be on the device at once.
+struct ServeSlot {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    Scratch meta, work, arena, recs, cols[N_COLS];
+    PinnedScratch pin;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr; // the batch's first and last device work
+    // the batch on the device (launched, not yet finished)
+    bool busy = false, timed = false;
+    std::vector<Plan> plans;
+    std::vector<uint64_t> seg_of, base_of;
+    uint64_t nq = 0, nb = 0, ns = 0, total = 0;
+    uint8_t *hb = nullptr; // read-back staging: member status, segment totals and bases
+    uint8_t *du = nullptr;
+    uint8_t *dc[N_COLS] = {};
+    // results of the last finished batch
+    frhip_buf col_view[N_COLS] = {};
+    frhip_buf arena_view = {};
+    uint64_t last_records = 0;
+    double t0_ms = 0, t1_ms = 0;
+};
+
 struct frhip_server {
     frhip_ctx *ctx = nullptr;
     uint8_t *d_file = nullptr;
@@ -50,11 +73,8 @@ struct frhip_server {
     uint64_t n = 0;
     std::vector<uint64_t> coff, in_off, uprefix;
     std::vector<uint32_t> in_len, isize, crc;
-    Scratch meta, work, arena, recs, cols[N_COLS];
-    PinnedScratch pin;
-    frhip_buf col_view[N_COLS] = {};
-    frhip_buf arena_view = {};
-    uint64_t last_records = 0;
+    std::vector<ServeSlot> slots;
+    hipEvent_t ev_ref = nullptr; // the origin of the batches' device intervals
 
     // Member holding compressed offset c as its start, or NONE.
     uint64_t member_at(uint64_t c) const
@@ -152,7 +172,15 @@ uint64_t records_cap(uint64_t bytes) { return bytes / 36 + 1; }
 extern "C" int frhip_server_create(frhip_ctx *ctx, const uint8_t *file, uint64_t len, frhip_server **out,
                                    frhip_error *err)
 {
+    return frhip_server_create_slots(ctx, file, len, FRHIP_SERVER_DEFAULT_SLOTS, out, err);
+}
+
+extern "C" int frhip_server_create_slots(frhip_ctx *ctx, const uint8_t *file, uint64_t len, int slots,
+                                         frhip_server **out, frhip_error *err)
+{
     *out = nullptr;
+    if (slots < 1 || slots > FRHIP_SERVER_MAX_SLOTS)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "a server has 1 to %d slots", FRHIP_SERVER_MAX_SLOTS);
     std::vector<frhip_block_span> spans;
     uint64_t position = 0;
     int rc = discover(file, len, 0, spans, &position, err);
@@ -182,6 +210,25 @@ extern "C" int frhip_server_create(frhip_ctx *ctx, const uint8_t *file, uint64_t
     hipError_t r = hipMalloc(&s->d_file, len + 4096);
     if (r == hipSuccess) r = hipMemcpy(s->d_file, file, len, hipMemcpyHostToDevice);
     if (r == hipSuccess) r = hipMemset(s->d_file + len, 0, 4096);
+    // slot 0 runs on the context's stream, the others on streams of their own
+    // (default priority)
+    s->slots.resize(slots);
+    for (int k = 0; r == hipSuccess && k < slots; ++k) {
+        ServeSlot &sl = s->slots[k];
+        if (k == 0) {
+            sl.stream = ctx->stream;
+        } else {
+            r = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+            sl.own_stream = r == hipSuccess;
+        }
+        if (r == hipSuccess) r = hipEventCreate(&sl.ev0);
+        if (r == hipSuccess) r = hipEventCreate(&sl.ev1);
+        for (int c = 0; c < N_COLS; ++c) sl.col_view[c] = frhip_buf{nullptr, 0, ctx->ordinal};
+        sl.arena_view = frhip_buf{nullptr, 0, ctx->ordinal};
+    }
+    if (r == hipSuccess) r = hipEventCreate(&s->ev_ref);
+    if (r == hipSuccess) r = hipEventRecord(s->ev_ref, ctx->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
     if (r != hipSuccess) {
         frhip_server_free(s);
         return hip_err(err, r, "uploading the file");
@@ -197,13 +244,20 @@ extern "C" void frhip_server_free(frhip_server *s)
         (void)hipSetDevice(s->ctx->ordinal);
         (void)hipStreamSynchronize(s->ctx->stream);
     }
+    for (ServeSlot &sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        sl.meta.release();
+        sl.work.release();
+        sl.arena.release();
+        sl.recs.release();
+        for (auto &c : sl.cols) c.release();
+        sl.pin.release();
+        if (sl.ev0) (void)hipEventDestroy(sl.ev0);
+        if (sl.ev1) (void)hipEventDestroy(sl.ev1);
+        if (sl.own_stream) (void)hipStreamDestroy(sl.stream);
+    }
+    if (s->ev_ref) (void)hipEventDestroy(s->ev_ref);
     if (s->d_file) (void)hipFree(s->d_file);
-    s->meta.release();
-    s->work.release();
-    s->arena.release();
-    s->recs.release();
-    for (auto &c : s->cols) c.release();
-    s->pin.release();
     if (s->ctx) ctx_release(s->ctx);
     delete s;
 }
@@ -234,31 +288,42 @@ extern "C" uint64_t frhip_server_batch_footprint(const frhip_server *s, const fr
            inflate_scratch_bytes(s->ctx->inflate_schedule, bytes, members);
 }
 
-extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t nq, frhip_query_result *res,
-                                frhip_error *err)
+extern "C" int frhip_server_slots(const frhip_server *s) { return (int)s->slots.size(); }
+
+extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query *q, uint64_t nq, frhip_error *err)
 {
+    if (slot < 0 || slot >= (int)s->slots.size())
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no slot %d", slot);
+    ServeSlot &sl = s->slots[slot];
+    if (sl.busy) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "slot %d has a batch in flight", slot);
     frhip_ctx *ctx = s->ctx;
     (void)hipSetDevice(ctx->ordinal);
-    hipStream_t st = ctx->stream;
+    hipStream_t st = sl.stream;
     // plan every query; queries with members become segments
-    std::vector<Plan> plans(nq);
-    std::vector<uint64_t> seg_of(nq, NONE), base_of(nq, 0);
+    std::vector<Plan> &plans = sl.plans;
+    std::vector<uint64_t> &seg_of = sl.seg_of, &base_of = sl.base_of;
+    plans.assign(nq, Plan{});
+    seg_of.assign(nq, NONE);
+    base_of.assign(nq, 0);
     uint64_t nb = 0, ns = 0, total = 0;
     for (uint64_t i = 0; i < nq; ++i) {
         plans[i] = s->plan(q[i]);
-        std::memset(&res[i], 0, sizeof res[i]);
-        res[i].status = plans[i].status;
         if (plans[i].status != FRHIP_OK || plans[i].m1 == plans[i].m0) continue;
         seg_of[i] = ns++;
         base_of[i] = total;
         nb += plans[i].m1 - plans[i].m0;
         total += plans[i].bytes;
-        res[i].members = plans[i].m1 - plans[i].m0;
-        res[i].inflated_bytes = plans[i].bytes;
     }
-    s->last_records = 0;
-    for (int c = 0; c < N_COLS; ++c) s->col_view[c] = frhip_buf{nullptr, 0, ctx->ordinal};
-    s->arena_view = frhip_buf{nullptr, 0, ctx->ordinal};
+    sl.nq = nq;
+    sl.nb = nb;
+    sl.ns = ns;
+    sl.total = total;
+    sl.last_records = 0;
+    for (int c = 0; c < N_COLS; ++c) sl.col_view[c] = frhip_buf{nullptr, 0, ctx->ordinal};
+    sl.arena_view = frhip_buf{nullptr, 0, ctx->ordinal};
+    sl.timed = false;
+    sl.busy = true;
+    sl.t0_ms = sl.t1_ms = 0;
     if (ns == 0) return FRHIP_OK;
 
     // host side of the member and segment arrays
@@ -266,8 +331,13 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
     const WorkLayout W = work_layout(nb, ns);
     const uint64_t back = align_up(nb * 12, 256) + ns * 32 + (ns + 1) * 8;
     int rc;
-    uint8_t *h = (uint8_t *)s->pin.get(std::max(M.bytes, back), err, &rc);
-    if (!h) return rc;
+    // a failure before the batch is on the stream leaves the slot free
+    auto fail = [&](int k) {
+        sl.busy = false;
+        return k;
+    };
+    uint8_t *h = (uint8_t *)sl.pin.get(std::max(M.bytes, back), err, &rc);
+    if (!h) return fail(rc);
     {
         uint64_t *in_off = (uint64_t *)(h + M.o_in_off), *coff = (uint64_t *)(h + M.o_coff);
         uint64_t *out_off = (uint64_t *)(h + M.o_out_off), *until = (uint64_t *)(h + M.o_until);
@@ -298,19 +368,21 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
         mb[ns] = (uint32_t)j;
         out_off[nb] = total;
     }
-    uint8_t *dm = (uint8_t *)s->meta.get(M.bytes, err, &rc);
-    if (!dm) return rc;
-    uint8_t *dw = (uint8_t *)s->work.get(W.bytes, err, &rc);
-    if (!dw) return rc;
-    uint8_t *du = (uint8_t *)s->arena.get(total + 16, err, &rc);
-    if (!du) return rc;
+    uint8_t *dm = (uint8_t *)sl.meta.get(M.bytes, err, &rc);
+    if (!dm) return fail(rc);
+    uint8_t *dw = (uint8_t *)sl.work.get(W.bytes, err, &rc);
+    if (!dw) return fail(rc);
+    uint8_t *du = (uint8_t *)sl.arena.get(total + 16, err, &rc);
+    if (!du) return fail(rc);
     const uint64_t cap = records_cap(total);
-    uint8_t *dc[N_COLS];
+    uint8_t **dc = sl.dc;
     for (int c = 0; c < N_COLS; ++c) {
-        dc[c] = (uint8_t *)s->cols[c].get(cap * WIDTHS[c], err, &rc);
-        if (!dc[c]) return rc;
+        dc[c] = (uint8_t *)sl.cols[c].get(cap * WIDTHS[c], err, &rc);
+        if (!dc[c]) return fail(rc);
     }
-    hipError_t r = hipMemcpyAsync(dm, h, M.bytes, hipMemcpyHostToDevice, st);
+    sl.du = du;
+    hipError_t r = hipEventRecord(sl.ev0, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(dm, h, M.bytes, hipMemcpyHostToDevice, st);
     uint32_t *d_st = (uint32_t *)(dw + W.o_st);
     if (r == hipSuccess) r = launch_fill_u64((uint64_t *)(dw + W.o_queue), 0, 1, st);
 
@@ -331,8 +403,8 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
     a.x2n = ctx->x2n;
     void *d_rec = nullptr;
     if (const uint64_t sb = inflate_scratch_bytes(ctx->inflate_schedule, total, nb)) {
-        d_rec = s->recs.get(sb, err, &rc);
-        if (!d_rec) return rc;
+        d_rec = sl.recs.get(sb, err, &rc);
+        if (!d_rec) return fail(rc);
     }
     inflate_scratch_bind(a, d_rec, nb);
     a.two_phase_min = ctx->two_phase_min;
@@ -431,8 +503,54 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
     if (r == hipSuccess) r = hipMemcpyAsync(hb, d_st, nb * 12, hipMemcpyDeviceToHost, st);
     if (r == hipSuccess) r = hipMemcpyAsync(hb + o_tot, ra.seg_totals, ns * 32, hipMemcpyDeviceToHost, st);
     if (r == hipSuccess) r = hipMemcpyAsync(hb + o_base, seg_base, (ns + 1) * 8, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r == hipSuccess) r = hipEventRecord(sl.ev1, st);
+    if (r != hipSuccess) {
+        // whatever was enqueued finishes before the slot is used again
+        (void)hipStreamSynchronize(st);
+        return fail(hip_err(err, r, "serving batch"));
+    }
+    sl.hb = hb;
+    sl.timed = true;
+    return FRHIP_OK;
+}
+
+extern "C" int frhip_server_query(const frhip_server *s, int slot)
+{
+    if (slot < 0 || slot >= (int)s->slots.size()) return 1;
+    const ServeSlot &sl = s->slots[slot];
+    if (!sl.busy || !sl.timed) return 1;
+    return hipEventQuery(sl.ev1) == hipSuccess ? 1 : 0;
+}
+
+extern "C" int frhip_server_finish(frhip_server *s, int slot, frhip_query_result *res, frhip_error *err)
+{
+    if (slot < 0 || slot >= (int)s->slots.size())
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no slot %d", slot);
+    ServeSlot &sl = s->slots[slot];
+    if (!sl.busy) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "slot %d has no batch", slot);
+    frhip_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->ordinal);
+    const uint64_t nq = sl.nq, nb = sl.nb, ns = sl.ns, total = sl.total;
+    const std::vector<Plan> &plans = sl.plans;
+    const std::vector<uint64_t> &seg_of = sl.seg_of, &base_of = sl.base_of;
+    for (uint64_t i = 0; i < nq; ++i) {
+        std::memset(&res[i], 0, sizeof res[i]);
+        res[i].status = plans[i].status;
+        if (seg_of[i] == NONE) continue;
+        res[i].members = plans[i].m1 - plans[i].m0;
+        res[i].inflated_bytes = plans[i].bytes;
+    }
+    sl.busy = false;
+    if (ns == 0) return FRHIP_OK;
+    hipError_t r = hipEventSynchronize(sl.ev1);
     if (r != hipSuccess) return hip_err(err, r, "serving batch");
+    float a0 = 0, a1 = 0;
+    (void)hipEventElapsedTime(&a0, s->ev_ref, sl.ev0);
+    (void)hipEventElapsedTime(&a1, s->ev_ref, sl.ev1);
+    sl.t0_ms = a0;
+    sl.t1_ms = a1;
+    const uint8_t *hb = sl.hb;
+    const uint64_t o_tot = align_up(nb * 12, 256), o_base = o_tot + ns * 32;
     const uint32_t *status = (const uint32_t *)hb, *produced = status + nb, *crcs = status + 2 * nb;
     const uint64_t *tot = (const uint64_t *)(hb + o_tot), *base = (const uint64_t *)(hb + o_base);
 
@@ -461,19 +579,47 @@ extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t 
             res[i].error_position = tail;
         }
     }
-    s->last_records = base[ns];
-    for (int c = 0; c < N_COLS; ++c) s->col_view[c] = frhip_buf{dc[c], s->last_records * WIDTHS[c], ctx->ordinal};
-    s->arena_view = frhip_buf{du, total, ctx->ordinal};
+    sl.last_records = base[ns];
+    for (int c = 0; c < N_COLS; ++c) sl.col_view[c] = frhip_buf{sl.dc[c], sl.last_records * WIDTHS[c], ctx->ordinal};
+    sl.arena_view = frhip_buf{sl.du, total, ctx->ordinal};
     return FRHIP_OK;
+}
+
+extern "C" int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t nq, frhip_query_result *res,
+                                frhip_error *err)
+{
+    const int rc = frhip_server_launch(s, 0, q, nq, err);
+    if (rc != FRHIP_OK) return rc;
+    return frhip_server_finish(s, 0, res, err);
+}
+
+extern "C" int frhip_server_batch_interval(const frhip_server *s, int slot, double *start_ms, double *end_ms)
+{
+    if (slot < 0 || slot >= (int)s->slots.size()) return FRHIP_E_INVALID_DEVICE_BATCH;
+    const ServeSlot &sl = s->slots[slot];
+    if (start_ms) *start_ms = sl.t0_ms;
+    if (end_ms) *end_ms = sl.t1_ms;
+    return FRHIP_OK;
+}
+
+extern "C" const frhip_buf *frhip_server_slot_column(const frhip_server *s, int slot, int column)
+{
+    if (!s || slot < 0 || slot >= (int)s->slots.size() || column < 0 || column >= N_COLS) return nullptr;
+    return &s->slots[slot].col_view[column];
+}
+
+extern "C" const frhip_buf *frhip_server_slot_inflated(const frhip_server *s, int slot)
+{
+    if (!s || slot < 0 || slot >= (int)s->slots.size()) return nullptr;
+    return &s->slots[slot].arena_view;
 }
 
 extern "C" const frhip_buf *frhip_server_column(const frhip_server *s, int column)
 {
-    if (!s || column < 0 || column >= N_COLS) return nullptr;
-    return &s->col_view[column];
+    return frhip_server_slot_column(s, 0, column);
 }
 
-extern "C" const frhip_buf *frhip_server_inflated(const frhip_server *s) { return s ? &s->arena_view : nullptr; }
+extern "C" const frhip_buf *frhip_server_inflated(const frhip_server *s) { return frhip_server_slot_inflated(s, 0); }
 
 extern "C" int frhip_device_memory(frhip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, frhip_error *err)
 {

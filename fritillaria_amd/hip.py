@@ -234,6 +234,14 @@ def lib():
         "frhip_pipeline_member_offsets": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_pipeline_stage_ms": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int]),
         "frhip_server_create": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_server_create_slots": (C.c_int, [_vp, _vp, _u64, C.c_int, C.POINTER(_vp), _ep]),
+        "frhip_server_slots": (C.c_int, [_vp]),
+        "frhip_server_launch": (C.c_int, [_vp, C.c_int, _vp, _u64, _ep]),
+        "frhip_server_query": (C.c_int, [_vp, C.c_int]),
+        "frhip_server_finish": (C.c_int, [_vp, C.c_int, _vp, _ep]),
+        "frhip_server_batch_interval": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "frhip_server_slot_column": (_vp, [_vp, C.c_int, C.c_int]),
+        "frhip_server_slot_inflated": (_vp, [_vp, C.c_int]),
         "frhip_server_free": (None, [_vp]),
         "frhip_server_members": (_u64, [_vp]),
         "frhip_server_member_table": (C.c_int, [_vp, _vp, _vp, _ep]),
@@ -1221,15 +1229,51 @@ class QueryServer:
     queries per launch (noodles csi Query semantics,
     crates/fritillaria-csi/src/io/query.rs:66-82)."""
 
-    def __init__(self, context: HipContext, file_bytes):
+    def __init__(self, context: HipContext, file_bytes, slots: int = 2):
         self.ctx = context
         self._file = _u8(file_bytes)
         h = _vp()
         e = _Err()
-        _raise(lib().frhip_server_create(context.handle, _ptr(self._file), self._file.size, C.byref(h),
-                                         C.byref(e)), e)
+        _raise(lib().frhip_server_create_slots(context.handle, _ptr(self._file), self._file.size, slots,
+                                               C.byref(h), C.byref(e)), e)
         self._h = h
         self._file = None
+        self._launched = {}
+
+    def slots(self) -> int:
+        return int(lib().frhip_server_slots(self._h))
+
+    def launch(self, slot: int, queries) -> None:
+        """Enqueues the queries as one batch in `slot` and returns at once."""
+        q = self._queries(queries)
+        e = _Err()
+        _raise(lib().frhip_server_launch(self._h, slot, _ptr(q), len(q), C.byref(e)), e)
+        self._launched[slot] = len(q)
+
+    def query(self, slot: int) -> bool:
+        """True when the slot's batch is done on the device (or it has none)."""
+        return bool(lib().frhip_server_query(self._h, slot))
+
+    def finish(self, slot: int) -> np.ndarray:
+        """Waits for the slot's batch; its QUERY_RESULT_DTYPE results."""
+        res = np.zeros(self._launched.pop(slot, 0), QUERY_RESULT_DTYPE)
+        e = _Err()
+        _raise(lib().frhip_server_finish(self._h, slot, _ptr(res), C.byref(e)), e)
+        return res
+
+    def batch_interval(self, slot: int):
+        """(start, end) in ms from the server's creation of the slot's last batch on the device."""
+        a, b = C.c_double(), C.c_double()
+        lib().frhip_server_batch_interval(self._h, slot, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def slot_column_to_host(self, slot: int, name: str) -> np.ndarray:
+        buf = DeviceBuffer(lib().frhip_server_slot_column(self._h, slot, _COL_INDEX[name]), owned=False,
+                           keepalive=self)
+        dt = dict(BAM_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
 
     def members(self) -> int:
         return int(lib().frhip_server_members(self._h))

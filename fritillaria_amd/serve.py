@@ -4,10 +4,12 @@ queries, continuous batching with admission by free HBM, latency percentiles.
 Requests follow the config-5 workload: each is a [vstart, vend) range whose
 compressed size is log-uniform over 64 KiB - 256 MiB, snapped to record-start
 virtual offsets (noodles csi Query semantics,
-crates/fritillaria-csi/src/io/query.rs:66-82). The loop runs in wall-clock time:
-every request that has arrived and fits the HBM budget joins the next batch,
-one batch runs at a time, and a request's latency is its batch's completion
-time minus its arrival time.
+crates/fritillaria-csi/src/io/query.rs:66-82). The loop runs in wall-clock time
+and batches continuously: whenever one of the server's slots is free and
+requests have arrived, those requests (in arrival order, while they fit the
+HBM budget left by the batches still on the device) go to the GPU as one
+batch, which runs alongside the batches already there; a request's latency is
+its batch's completion time minus its arrival time.
 """
 import time
 from dataclasses import dataclass, field
@@ -35,6 +37,21 @@ class ServeStats:
     batches: int
     batch_sizes: List[int] = field(default_factory=list)
     failures: int = 0
+    # device interval (start, end) of every batch, ms, and how many batches
+    # were launched while another was still on the device
+    intervals: List[tuple] = field(default_factory=list)
+    overlapped_launches: int = 0
+
+    def overlap_ms(self) -> float:
+        """Device time during which two or more batches were running."""
+        ev = sorted([(a, 1) for a, _ in self.intervals] + [(b, -1) for _, b in self.intervals])
+        depth, last, total = 0, 0.0, 0.0
+        for t, d in ev:
+            if depth >= 2:
+                total += t - last
+            depth += d
+            last = t
+        return total
 
     def summary(self) -> dict:
         lat = self.latencies_ms
@@ -47,6 +64,8 @@ class ServeStats:
             "batches": self.batches,
             "mean_batch": round(float(np.mean(self.batch_sizes)), 2) if self.batch_sizes else 0.0,
             "failures": self.failures,
+            "overlapped_launches": self.overlapped_launches,
+            "device_overlap_ms": round(self.overlap_ms(), 3),
         }
 
 
@@ -75,49 +94,80 @@ def make_requests(voffs: np.ndarray, coffsets: np.ndarray, n: int, rate_hz: floa
 
 
 def serve(server: "hip.QueryServer", requests: List[Request], budget_bytes: int,
-          max_batch: Optional[int] = None) -> ServeStats:
-    """Continuous batching: whenever the device is free, every request that has
-    arrived joins the next batch in arrival order while the batch's device
-    footprint stays within budget_bytes (a request larger than the budget runs
-    alone). Blocks until every request is served."""
+          max_batch: Optional[int] = None, free_bytes=None) -> ServeStats:
+    """Continuous batching over the server's slots. Whenever a slot is free and
+    requests have arrived, they join a new batch in arrival order while its
+    device footprint fits what is left of the budget: budget_bytes less the
+    footprints of the batches still on the device, and no more than the free
+    HBM read (free_bytes(), hipMemGetInfo by default) at that moment. A request
+    larger than the budget runs alone once nothing else is in flight. The batch
+    is launched at once and overlaps the ones already running; finished slots
+    are collected as soon as the device reports them. Blocks until every
+    request is served."""
     n = len(requests)
+    k_slots = server.slots()
+    if free_bytes is None:
+        free_bytes = lambda: hip.device_memory(server.ctx)[0]  # noqa: E731
     # device footprint of each request alone; a batch is charged their sum
     fp = [server.footprint([(r.vstart, r.vend)]) for r in requests]
     lat = np.zeros(n)
     pending: List[int] = []
+    in_flight = {}  # slot -> (request indices, charged bytes)
     nxt = 0
     done = 0
-    inflated = records = batches = failures = 0
-    sizes = []
+    inflated = records = batches = failures = overlapped = 0
+    sizes, intervals = [], []
     t0 = time.perf_counter()
     while done < n:
         now = time.perf_counter() - t0
         while nxt < n and requests[nxt].arrival <= now:
             pending.append(nxt)
             nxt += 1
-        if not pending:
-            time.sleep(max(0.0, requests[nxt].arrival - now))
-            continue
-        batch = [pending[0]]
-        charged = fp[pending[0]]
-        for k in pending[1:]:
-            if (max_batch and len(batch) >= max_batch) or charged + fp[k] > budget_bytes:
-                break
-            batch.append(k)
-            charged += fp[k]
-        pending = pending[len(batch):]
-        res = server.run([(requests[i].vstart, requests[i].vend) for i in batch])
-        end = time.perf_counter() - t0
-        for i, r in zip(batch, res):
-            lat[i] = (end - requests[i].arrival) * 1e3
-            if r["status"] != 0:
-                failures += 1
-            inflated += int(r["inflated_bytes"])
-            records += int(r["records"])
-        done += len(batch)
-        batches += 1
-        sizes.append(len(batch))
-    return ServeStats(lat, time.perf_counter() - t0, inflated, records, batches, sizes, failures)
+        progressed = False
+        for slot in list(in_flight):
+            if not server.query(slot):
+                continue
+            batch, _ = in_flight.pop(slot)
+            res = server.finish(slot)
+            end = time.perf_counter() - t0
+            intervals.append(server.batch_interval(slot))
+            for i, r in zip(batch, res):
+                lat[i] = (end - requests[i].arrival) * 1e3
+                if r["status"] != 0:
+                    failures += 1
+                inflated += int(r["inflated_bytes"])
+                records += int(r["records"])
+            done += len(batch)
+            progressed = True
+        free_slots = [k for k in range(k_slots) if k not in in_flight]
+        if pending and free_slots:
+            charged_now = sum(c for _, c in in_flight.values())
+            room = min(budget_bytes - charged_now, int(free_bytes()))
+            batch, charged = [], 0
+            for k in pending:
+                if (max_batch and len(batch) >= max_batch) or charged + fp[k] > room:
+                    break
+                batch.append(k)
+                charged += fp[k]
+            if not batch and not in_flight:
+                batch, charged = [pending[0]], fp[pending[0]]  # too large for the budget: alone
+            if batch:
+                pending = pending[len(batch):]
+                slot = free_slots[0]
+                if in_flight:
+                    overlapped += 1
+                server.launch(slot, [(requests[i].vstart, requests[i].vend) for i in batch])
+                in_flight[slot] = (batch, charged)
+                batches += 1
+                sizes.append(len(batch))
+                progressed = True
+        if not progressed:
+            if not in_flight and not pending and nxt < n:
+                time.sleep(max(0.0, requests[nxt].arrival - (time.perf_counter() - t0)))
+            else:
+                time.sleep(0.0001)
+    return ServeStats(lat, time.perf_counter() - t0, inflated, records, batches, sizes, failures, intervals,
+                      overlapped)
 
 
 def serve_plain(ctx: "hip.HipContext", compressed: np.ndarray, requests: List[Request], coffsets: np.ndarray,

@@ -40,8 +40,18 @@ typedef struct {
 } frhip_query_result;
 
 /* Uploads file[0..len) to HBM once and walks its member headers on the host
-   (the same strict walk as frhip_discover). */
+   (the same strict walk as frhip_discover). The server has `slots` batch slots
+   (1..FRHIP_SERVER_MAX_SLOTS): each holds one batch on the device at a time,
+   with its own stream (slot 0: the context's; the others: streams the server
+   creates, default priority), scratch and columns, so batches in different
+   slots run concurrently. frhip_server_create makes
+   FRHIP_SERVER_DEFAULT_SLOTS. */
+#define FRHIP_SERVER_DEFAULT_SLOTS 2
+#define FRHIP_SERVER_MAX_SLOTS 4
 int frhip_server_create(frhip_ctx *ctx, const uint8_t *file, uint64_t len, frhip_server **out, frhip_error *err);
+int frhip_server_create_slots(frhip_ctx *ctx, const uint8_t *file, uint64_t len, int slots, frhip_server **out,
+                              frhip_error *err);
+int frhip_server_slots(const frhip_server *s);
 void frhip_server_free(frhip_server *s);
 uint64_t frhip_server_members(const frhip_server *s);
 /* Compressed offset and inflated-prefix of every member (members and members+1 entries). */
@@ -50,13 +60,30 @@ int frhip_server_member_table(const frhip_server *s, uint64_t *coffsets, uint64_
    bytes, columns at their worst-case record count, scan scratch): what an
    admission policy charges against free HBM. */
 uint64_t frhip_server_batch_footprint(const frhip_server *s, const frhip_query *q, uint64_t n);
-/* Runs the queries as one batch, synchronously. Per-query failures (a vstart
-   or vend that names no member, a malformed record, a member that fails
-   verification) are reported in res[i].status; the call itself fails only
-   when the device does. */
+/* Runs the queries as one batch in slot 0, synchronously (launch, then
+   finish). Per-query failures (a vstart or vend that names no member, a
+   malformed record, a member that fails verification) are reported in
+   res[i].status; the call itself fails only when the device does. */
 int frhip_server_run(frhip_server *s, const frhip_query *q, uint64_t n, frhip_query_result *res, frhip_error *err);
-/* Columns of the last batch, in frhip_bam_column order (virtual offsets included),
-   and the inflated bytes they index. Owned by the server; valid until the next run. */
+/* Continuous batching. launch plans the n queries, uploads their member
+   tables and enqueues the whole batch on the slot's stream, returning without
+   waiting for the device (a slot with a batch not yet finished is
+   FRHIP_E_INVALID_DEVICE_BATCH). query is 1 when the slot's batch is done on
+   the device (or it has none), 0 while it runs. finish waits for it, fills
+   res[0..n) as frhip_server_run does and frees the slot; the slot's columns
+   then hold the batch's records. A query's results do not depend on which
+   slot or batch it ran in. */
+int frhip_server_launch(frhip_server *s, int slot, const frhip_query *q, uint64_t n, frhip_error *err);
+int frhip_server_query(const frhip_server *s, int slot);
+int frhip_server_finish(frhip_server *s, int slot, frhip_query_result *res, frhip_error *err);
+/* Device interval of the slot's last finished batch, in ms from the server's
+   creation (HIP events on the slot's stream around the batch's work). */
+int frhip_server_batch_interval(const frhip_server *s, int slot, double *start_ms, double *end_ms);
+/* Columns of the slot's last finished batch, in frhip_bam_column order (virtual
+   offsets included), and the inflated bytes they index. Owned by the server;
+   valid until the slot's next launch. frhip_server_column / _inflated are slot 0's. */
+const frhip_buf *frhip_server_slot_column(const frhip_server *s, int slot, int column);
+const frhip_buf *frhip_server_slot_inflated(const frhip_server *s, int slot);
 const frhip_buf *frhip_server_column(const frhip_server *s, int column);
 const frhip_buf *frhip_server_inflated(const frhip_server *s);
 /* hipMemGetInfo of the context's device. */

@@ -87,3 +87,72 @@ def test_invalid_start_is_reported_per_query(served):
     res = server.run([(bad, fa.TO_END), (int(v[3]), int(v[9]))])
     assert res[0]["status"] == 1  # FRHIP_E_INVALID_BLOCK
     assert res[1]["status"] == 0 and res[1]["records"] == 6
+
+
+def check_slot(server, slot, res, queries, cols):
+    """Each query's records in the slot's columns equal the oracle's records
+    with virtual offsets in [vstart, vend)."""
+    import fritillaria_amd as fa
+    got = {c: server.slot_column_to_host(slot, c) for c in COLS}
+    v = cols["virtual_offsets"]
+    for k, (vs, ve) in enumerate(queries):
+        i = int(np.searchsorted(v, vs))
+        j = max(i, len(v) if ve == fa.TO_END else int(np.searchsorted(v, ve)))
+        assert res[k]["status"] == 0 and int(res[k]["records"]) == j - i, (slot, k, res[k])
+        f = int(res[k]["first"])
+        for c in COLS:
+            assert np.array_equal(got[c][f:f + j - i], cols[c][i:j]), (slot, k, c)
+
+
+def test_two_slots_run_concurrently(served):
+    """A batch launched in slot 1 while slot 0's is on the device: both give the
+    oracle's records, the same as each query alone, and their device intervals
+    overlap."""
+    print("[gpu] serve: two slots in flight", flush=True)
+    server, cols, offs, _ = served
+    assert server.slots() == 2
+    rng = np.random.default_rng(21)
+    qa = sample_queries(cols["virtual_offsets"], rng, 30)
+    qb = sample_queries(cols["virtual_offsets"], rng, 30)
+    server.launch(0, qa)
+    server.launch(1, qb)
+    with pytest.raises(Exception):
+        server.launch(1, qb[:1])  # busy
+    rb = server.finish(1)
+    ra = server.finish(0)
+    check_slot(server, 0, ra, qa, cols)
+    check_slot(server, 1, rb, qb, cols)
+    (a0, a1), (b0, b1) = server.batch_interval(0), server.batch_interval(1)
+    assert 0 <= a0 < a1 and 0 <= b0 < b1
+    assert b0 < a1 and a0 < b1, ((a0, a1), (b0, b1))
+    for k in range(0, 30, 7):
+        alone = server.run([qb[k]])
+        assert alone[0]["records"] == rb[k]["records"]
+        n = int(alone[0]["records"])
+        f = int(rb[k]["first"])
+        for c in ("position", "virtual_offsets", "record_end"):
+            assert np.array_equal(server.column_to_host(c)[:n], server.slot_column_to_host(1, c)[f:f + n])
+
+
+def test_serve_loop_overlaps_batches(served):
+    """The continuous-batching loop with staggered arrivals: every request is
+    served without failure, the records served are the oracle's, and batches
+    launched while others were on the device overlapped them."""
+    print("[gpu] serve loop with staggered arrivals", flush=True)
+    from fritillaria_amd import serve as sv
+    server, cols, offs, _ = served
+    v = cols["virtual_offsets"]
+    co, _ = server.member_table()
+    reqs = sv.make_requests(v, co, 60, rate_hz=4000.0, seed=7, lo_bytes=64 << 10, hi_bytes=4 << 20)
+    st = sv.serve(server, reqs, budget_bytes=8 << 30)
+    summ = st.summary()
+    assert summ["failures"] == 0 and summ["requests"] == 60
+    want = 0
+    for r in reqs:
+        i = int(np.searchsorted(v, r.vstart))
+        j = len(v) if r.vend == (1 << 64) - 1 else int(np.searchsorted(v, r.vend))
+        want += max(0, j - i)
+    assert summ["records"] == want
+    assert st.batches == len(st.intervals)
+    print(f"[gpu] serve loop: {summ}", flush=True)
+    assert summ["overlapped_launches"] >= 1 and summ["device_overlap_ms"] > 0
