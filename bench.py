@@ -88,7 +88,11 @@ def serve_pass(fa, ctx, compressed, voffs, n_requests, log):
         f"({bytes_per_req / 1e6:.1f} MB inflated per request, mean batch {cap['mean_batch']})")
     out = {"workload": "range queries over the batched large BAM: log-uniform 64 KiB-256 MiB compressed extents, "
                        "record-start virtual offsets, Poisson arrivals",
-           "hbm_budget_bytes": budget, "capacity": cap, "loads": {}}
+           "hbm_budget_bytes": budget, "slots": server.slots(),
+           "policy": f"continuous batching over {server.slots()} slots; a batch joins others on the device while "
+                     f"they hold under {sv.OVERLAP_BELOW >> 20} MiB compressed; admission by free HBM re-read at "
+                     "each launch",
+           "capacity": cap, "loads": {}}
     for load in (0.3, 0.6, 0.9):
         reqs = sv.make_requests(voffs, co, n_requests, rate_hz=load * req_per_s, seed=0x5EED0005 + int(load * 10))
         r = sv.serve(server, reqs, budget).summary()
