@@ -94,8 +94,19 @@ void buf_free(frhip_buf *b)
 {
     if (!b) return;
     event_release(b->ready);
-    if (b->ptr) (void)hipFree(b->ptr);
+    if (b->owner)
+        owner_release(b->owner);
+    else if (b->ptr)
+        (void)hipFree(b->ptr);
     delete b;
+}
+
+void owner_release(BufOwner *o)
+{
+    if (o && o->refs.fetch_sub(1) == 1) {
+        if (o->ptr) (void)hipFree(o->ptr);
+        delete o;
+    }
 }
 
 EventRef *event_record(hipStream_t s)

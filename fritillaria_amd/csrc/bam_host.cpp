@@ -308,7 +308,106 @@ struct frhip_reader {
     std::vector<frhip_block_span> spans;
     std::vector<uint64_t> offsets, coffsets;
     bool inner_exhausted = false;
+    // Read-ahead: the members [cache_coff[0], ...) inflated by one launch
+    // (cache_uoff: their output offsets, n + 1), which the next batches are
+    // cut from as views instead of inflating each batch on its own.
+    BufOwner *cache = nullptr;
+    EventRef *cache_ready = nullptr;
+    std::vector<uint64_t> cache_coff, cache_uoff;
 };
+
+namespace {
+
+// Members one read-ahead launch inflates: enough for the two-phase form (one
+// member per lane) to fill the device, and many batches' worth.
+uint64_t read_ahead_members(const frhip_reader *r)
+{
+    return std::max<uint64_t>(32768, 16 * r->blocks_per_batch);
+}
+constexpr uint64_t READ_AHEAD_MAX_BYTES = 1ull << 30; // compressed bytes per read-ahead launch
+
+void drop_cache(frhip_reader *r)
+{
+    owner_release(r->cache);
+    event_release(r->cache_ready);
+    r->cache = nullptr;
+    r->cache_ready = nullptr;
+    r->cache_coff.clear();
+    r->cache_uoff.clear();
+}
+
+// The current batch (r->spans) as a view into the read-ahead, or nullptr when
+// the read-ahead does not hold all of its members.
+frhip_buf *cache_view(frhip_reader *r, frhip_error *err)
+{
+    const uint64_t k = r->spans.size();
+    if (!r->cache || k == 0) return nullptr;
+    auto it = std::lower_bound(r->cache_coff.begin(), r->cache_coff.end(), r->spans[0].compressed_offset);
+    if (it == r->cache_coff.end() || *it != r->spans[0].compressed_offset) return nullptr;
+    const uint64_t i0 = (uint64_t)(it - r->cache_coff.begin());
+    if (i0 + k > r->cache_coff.size() || r->cache_coff[i0 + k - 1] != r->spans[k - 1].compressed_offset) return nullptr;
+    frhip_buf *b = new (std::nothrow) frhip_buf;
+    if (!b) {
+        set_err(err, FRHIP_E_DEVICE, 0, "host allocation failed");
+        return nullptr;
+    }
+    b->ptr = (uint8_t *)r->cache->ptr + r->cache_uoff[i0];
+    b->len = r->cache_uoff[i0 + k] - r->cache_uoff[i0];
+    b->ordinal = r->ctx->ordinal;
+    r->cache->refs.fetch_add(1);
+    b->owner = r->cache;
+    buf_attach_event(b, r->cache_ready);
+    for (uint64_t j = 0; j <= k; ++j) r->offsets[j] = r->cache_uoff[i0 + j] - r->cache_uoff[i0];
+    return b;
+}
+
+// Inflates the read-ahead from the batch's first member on. A member that
+// fails verification ends the read-ahead before it, so that the batch holding
+// it is inflated on its own and reports the failure as it always has.
+void refill_cache(frhip_reader *r)
+{
+    drop_cache(r);
+    if (r->spans.empty()) return;
+    const uint64_t c0 = r->spans[0].compressed_offset;
+    if (c0 >= r->src_len) return;
+    const uint8_t *base = r->src + c0;
+    uint64_t avail = std::min<uint64_t>(r->src_len - c0, READ_AHEAD_MAX_BYTES);
+    std::vector<frhip_block_span> ra;
+    uint64_t consumed = 0;
+    frhip_error e{};
+    (void)discover(base, avail, c0, ra, &consumed, &e); // stops at a partial or bad member
+    if (ra.size() > read_ahead_members(r)) ra.resize(read_ahead_members(r));
+    while (ra.size() > r->spans.size()) {
+        const uint64_t bytes = ra.back().payload_start + ra.back().payload_len + 8;
+        std::vector<uint64_t> offs(ra.size() + 1);
+        frhip_buf *data = nullptr;
+        frhip_error fe{};
+        const int rc = inflate_device(r->ctx, base, bytes, ra.data(), ra.size(), &data, offs.data(), &fe, nullptr);
+        if (rc == FRHIP_OK) {
+            r->cache = new (std::nothrow) BufOwner;
+            if (!r->cache) {
+                buf_free(data);
+                return;
+            }
+            r->cache->ptr = data->ptr;
+            r->cache_ready = data->ready;
+            data->ptr = nullptr;
+            data->ready = nullptr;
+            buf_free(data);
+            r->cache_uoff.assign(offs.begin(), offs.end());
+            r->cache_coff.resize(ra.size());
+            for (size_t i = 0; i < ra.size(); ++i) r->cache_coff[i] = ra[i].compressed_offset;
+            return;
+        }
+        if (rc != FRHIP_E_INFLATE && rc != FRHIP_E_SIZE_MISMATCH && rc != FRHIP_E_CHECKSUM_MISMATCH) return;
+        // keep the members before the failing one
+        size_t keep = 0;
+        while (keep < ra.size() && ra[keep].compressed_offset < fe.offset) ++keep;
+        ra.resize(keep);
+    }
+}
+
+} // namespace
 
 extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t len, uint64_t blocks_per_batch,
                                  frhip_reader **out, frhip_error *err)
@@ -327,6 +426,7 @@ extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t l
 extern "C" void frhip_reader_free(frhip_reader *r)
 {
     if (!r) return;
+    drop_cache(r);
     ctx_release(r->ctx);
     delete r;
 }
@@ -391,10 +491,16 @@ extern "C" int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_bu
             continue;
         }
         r->offsets.assign(r->spans.size() + 1, 0);
-        frhip_buf *data = nullptr;
-        rc = inflate_device(r->ctx, r->compressed.data(), r->compressed.size(), r->spans.data(), r->spans.size(),
-                            &data, r->offsets.data(), err, nullptr);
-        if (rc) return rc;
+        frhip_buf *data = cache_view(r, err);
+        if (!data) {
+            refill_cache(r);
+            data = cache_view(r, err);
+        }
+        if (!data) {
+            rc = inflate_device(r->ctx, r->compressed.data(), r->compressed.size(), r->spans.data(),
+                                r->spans.size(), &data, r->offsets.data(), err, nullptr);
+            if (rc) return rc;
+        }
         if (data->len == 0) {
             buf_free(data);
             if (r->inner_exhausted) return FRHIP_OK;

@@ -35,7 +35,7 @@ struct InflateSchedule {
 };
 extern const InflateSchedule INFLATE_SCHEDULES[];
 extern const int N_INFLATE_SCHEDULES;
-constexpr int DEFAULT_INFLATE_SCHEDULE = 5;
+constexpr int DEFAULT_INFLATE_SCHEDULE = 16;
 
 // Launches `waves` waves (one member each at a time) of schedule `schedule`.
 hipError_t launch_inflate(const InflateArgs &a, uint32_t waves, hipStream_t stream, int schedule);

@@ -35,6 +35,15 @@ struct EventRef {
 EventRef *event_record(hipStream_t s);
 void event_release(EventRef *e);
 
+// A device allocation shared by several buffers that view parts of it (a
+// reader's inflated read-ahead and the batches cut from it); freed with the
+// last reference.
+struct BufOwner {
+    void *ptr = nullptr;
+    std::atomic<int> refs{1};
+};
+void owner_release(BufOwner *o);
+
 int set_err(frhip_error *e, int kind, uint64_t offset, const char *fmt, ...);
 int hip_err(frhip_error *e, hipError_t r, const char *what);
 const char *inflate_status_message(uint32_t status);
@@ -58,6 +67,7 @@ struct frhip_buf {
     uint64_t len;
     int ordinal;
     frhip::EventRef *ready = nullptr; // owned reference, or none
+    frhip::BufOwner *owner = nullptr; // set: ptr lies inside owner's allocation, which buf_free releases
 };
 
 struct frhip_ctx {
@@ -70,7 +80,7 @@ struct frhip_ctx {
     bool own_stream = false;
     uint32_t *x2n = nullptr;
     uint32_t bam_window = 2048;
-    int inflate_schedule = 5; // index into INFLATE_SCHEDULES
+    int inflate_schedule = 16; // index into INFLATE_SCHEDULES (DEFAULT_INFLATE_SCHEDULE)
     uint32_t two_phase_min = 8192; // smaller batches of a two-phase schedule run the one-wave form
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
         rec_scratch;

@@ -109,8 +109,10 @@ int frhip_ctx_synchronize(frhip_ctx *ctx, frhip_error *err);
    No reference counterpart; the reference speculates on member starts only. */
 int frhip_ctx_set_bam_window(frhip_ctx *ctx, uint32_t window);
 /* Inflate kernel variant (output ring size per member x waves per workgroup),
-   an index into the table frhip_inflate_schedules describes. Results do not
-   depend on it. No reference counterpart. */
+   an index into the table frhip_inflate_schedules describes; the default is
+   16 (two-phase, 4 KiB ring, whose small batches run the one-wave
+   lane-speculative form). Results do not depend on it. No reference
+   counterpart. */
 int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
 /* With a two-phase schedule (token form 3: one member per lane, then one wave
    per member), batches of fewer than min_members members run the one-wave

@@ -269,3 +269,57 @@ def test_speculation_holds_past_2_gib(codec, decoder, oracle_mod):
     assert rc == 0 and len(records) == len(want) == 8_000_000 and records.tail() == tail
     assert np.array_equal(records.column_to_host("record_offsets"), want)
     assert records.fallbacks() <= 2 + len(spans) // 50, records.fallbacks()
+
+
+@pytest.mark.parametrize("bpb", [1, 3])
+def test_reader_read_ahead_reports_a_bad_member_where_it_lies(codec, decoder, bpb):
+    """The reader inflates ahead of the batches it hands out; a member with a
+    bad trailer CRC inside that window still fails exactly the batch that holds
+    it (ChecksumMismatch at its offset), after every earlier batch was served
+    with the records it holds, as when each batch is inflated on its own."""
+    print(f"[gpu] reader read-ahead with a bad member, blocks_per_batch={bpb}", flush=True)
+    import fritillaria_amd as fa
+    from helpers import bgzf_member
+    rng = np.random.default_rng(77)
+    data = bam_header() + b"".join(random_records(rng, 9000))
+    cap = 60000
+    members = [bgzf_member(data[i:i + cap]) for i in range(0, len(data), cap)]
+    bad = 17
+    payload = data[bad * cap:(bad + 1) * cap]
+    import zlib
+    members[bad] = bgzf_member(payload, crc=zlib.crc32(payload) ^ 1)
+    blob = b"".join(members) + bgzf(b"", eof=True)
+    bad_off = sum(len(m) for m in members[:bad])
+    reader = fa.DeviceBgzfReader(blob, codec, blocks_per_batch=bpb)
+    first, served = True, 0
+    with pytest.raises(fa.ChecksumMismatch) as ei:
+        while True:
+            b = reader.next_batch()
+            if b is None:
+                break
+            start = b.start
+            if first:
+                start, _ = fa.bam_parse_header(b.data.data().to_vec())
+                first = False
+            rec = decoder.decode(b.data, start)
+            served += len(rec)
+            reader.carry_from(rec.tail())
+    assert ei.value.offset == bad_off
+    # the batches before the failing one hold members before the bad one only;
+    # with one 64 KiB budget per batch (a member or two) they held every record
+    # that ends before the member carried into the failing batch
+    ends = len(bam_header()) + np.cumsum([4 + int.from_bytes(r[:4], "little") for r in _records_of(data)])
+    hi = int(np.searchsorted(ends, bad * cap, side="right"))
+    assert served <= hi, (served, hi)
+    if bpb == 1:
+        assert served >= int(np.searchsorted(ends, (bad - 2) * cap, side="right")) > 0
+
+
+def _records_of(data):
+    hdr = len(bam_header())
+    out, p = [], hdr
+    while p + 4 <= len(data):
+        n = 4 + int.from_bytes(data[p:p + 4], "little")
+        out.append(data[p:p + n])
+        p += n
+    return out

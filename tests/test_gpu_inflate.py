@@ -250,5 +250,5 @@ def test_every_schedule_matches_oracle(hip_ctx, oracle_mod, schedule):
                 codec.inflate_batch_device(member, fa.discover(member)[0])
             assert ei.value.code == want, name
     finally:
-        hip_ctx.set_inflate_schedule(5)
+        hip_ctx.set_inflate_schedule(16)
         hip_ctx.set_two_phase_min_members(8192)

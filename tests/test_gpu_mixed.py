@@ -77,7 +77,7 @@ def run_and_compare(hip_ctx, codec, oracle_mod, b, f, two_phase):
     try:
         r, gb, gf = run_pair(hip_ctx, oracle_mod, b.compressed.tobytes(), f.compressed.tobytes())
     finally:
-        hip_ctx.set_inflate_schedule(5)
+        hip_ctx.set_inflate_schedule(16)
         hip_ctx.set_two_phase_min_members(8192)
     assert len(gb["record_offsets"]) == b.n_records and len(gf["record_offsets"]) == f.n_records
     bcf_alone = fa.BcfScanner(hip_ctx).decode(
