@@ -13,5 +13,5 @@ from .hip import (  # noqa: F401
     crc32_combine, device_is_available, discover, select_codec, QueryServer, QUERY_RESULT_DTYPE, TO_END,
     device_memory, BamSoa, SOA_COLUMNS, AUX_ENTRY_DTYPE, BCF_COLUMNS, BcfRecordBatch, BcfScanner, bcf_parse_header,
     BcfFastqBatch, decode_bcf_fastq, decode_bcf_fastq_joined, join_bcf_fastq,
-    InflateTimings, DecodeTimings, event_wait, event_synchronize,
+    InflateTimings, DecodeTimings, event_wait, event_synchronize, PinnedBuffer,
 )

@@ -390,6 +390,22 @@ extern "C" int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip
     return r == hipSuccess ? FRHIP_OK : hip_err(err, r, "device copy");
 }
 
+extern "C" int frhip_host_alloc(uint64_t len, void **out, frhip_error *err)
+{
+    *out = nullptr;
+    hipError_t r = hipHostMalloc(out, len ? len : 1, hipHostMallocDefault);
+    if (r != hipSuccess) {
+        *out = nullptr;
+        return hip_err(err, r, "hipHostMalloc");
+    }
+    return FRHIP_OK;
+}
+
+extern "C" void frhip_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
 extern "C" int frhip_buf_copy_from_host(frhip_buf *b, uint64_t offset, const void *src, uint64_t len,
                                         frhip_error *err)
 {

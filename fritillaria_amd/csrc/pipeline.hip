@@ -359,6 +359,24 @@ __global__ void fill_u64_kernel(uint64_t *p, uint64_t v, uint64_t n)
         p[i] = v;
 }
 
+// When the discovered members' inflated total is not the size the output was
+// built for (a re-uploaded file whose member table differs), every member's
+// output extent becomes empty: the inflate then writes nothing and the run
+// fails verification instead of writing past the output.
+__global__ void guard_total_kernel(uint64_t *out_off, uint32_t *isize, uint64_t n, uint64_t expect,
+                                   uint64_t *bad)
+{
+    if (out_off[n] == expect) return;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        out_off[i] = 0;
+        isize[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out_off[n] = 0;
+        *bad = 1;
+    }
+}
+
 } // namespace
 
 uint64_t sieve_tiles(uint64_t len) { return (len + SIEVE_TILE - 1) / SIEVE_TILE; }
@@ -425,6 +443,16 @@ hipError_t launch_copy(uint8_t *dst, const uint8_t *src, uint64_t n, hipStream_t
     if (grid > 8192) grid = 8192;
     if (grid == 0) grid = 1;
     hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)grid), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_guard_total(uint64_t *out_off, uint32_t *isize, uint64_t n, uint64_t expect, uint64_t *bad,
+                              hipStream_t s)
+{
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 1024) grid = 1024;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(guard_total_kernel, dim3((uint32_t)grid), dim3(256), 0, s, out_off, isize, n, expect, bad);
     return hipGetLastError();
 }
 

@@ -49,7 +49,13 @@ struct frhip_pipeline {
     frhip_pipeline_config cfg{};
     uint64_t file_len = 0, n = 0, U = 0, cap_rec = 0, n_tiles = 0, cand_cap = 0;
     bool device_discovery = false;
+    bool sized = false;          // U and the buffers sized from it are final
     uint8_t *d_file = nullptr;
+    // re-uploads of the file (frhip_pipeline_upload): a copy stream, the
+    // copy's event the next launch waits for, the last launch's end event
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_uploaded = nullptr, ev_done = nullptr;
+    bool upload_pending = false, launched = false;
     // discovery
     uint32_t *d_tile_counts = nullptr;
     uint64_t *d_tile_base = nullptr;
@@ -119,6 +125,8 @@ hipError_t enqueue_discovery(frhip_pipeline *p, hipStream_t s)
                          p->cand_cap, p->m, (uint64_t *)&p->d_res->chain_bad, s);
     if (r == hipSuccess)
         r = launch_scan_u32_u64(p->m.isize, p->m.out_off, p->d_tile_base + p->n_tiles, 0, s);
+    if (r == hipSuccess && p->sized)
+        r = launch_guard_total(p->m.out_off, p->m.isize, p->n, p->U, (uint64_t *)&p->d_res->chain_bad, s);
     return r;
 }
 
@@ -479,6 +487,7 @@ extern "C" int frhip_pipeline_create(frhip_ctx *ctx, const uint8_t *file, uint64
         if (r != hipSuccess) return fail(hip_err(err, r, "uploading members"));
     }
     r = hipMemcpy(&p->U, p->m.out_off + p->n, 8, hipMemcpyDeviceToHost);
+    p->sized = true;
     if (r == hipSuccess) r = hipMemcpy(p->d_n, &p->n, 8, hipMemcpyHostToDevice);
     if (r == hipSuccess) r = hipMemcpy(p->d_ulen, &p->U, 8, hipMemcpyHostToDevice);
     if (r != hipSuccess) return fail(hip_err(err, r, "member table"));
@@ -585,8 +594,12 @@ extern "C" void frhip_pipeline_free(frhip_pipeline *p)
     if (!p) return;
     (void)hipSetDevice(p->ctx->ordinal);
     (void)hipStreamSynchronize(p->ctx->stream);
+    if (p->copy_stream) (void)hipStreamSynchronize(p->copy_stream);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->ev_uploaded) (void)hipEventDestroy(p->ev_uploaded);
+    if (p->ev_done) (void)hipEventDestroy(p->ev_done);
+    if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
     for (auto e : p->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *a : p->allocs) (void)hipFree(a);
@@ -600,7 +613,12 @@ extern "C" int frhip_pipeline_launch(frhip_pipeline *p, frhip_error *err)
     (void)hipSetDevice(p->ctx->ordinal);
     hipStream_t s = p->ctx->stream;
     if (p->n == 0) return FRHIP_OK;
-    hipError_t r;
+    hipError_t r = hipSuccess;
+    if (p->upload_pending) {
+        r = hipStreamWaitEvent(s, p->ev_uploaded, 0);
+        if (r != hipSuccess) return hip_err(err, r, "waiting for the upload");
+        p->upload_pending = false;
+    }
     if (p->exec) {
         r = hipGraphLaunch(p->exec, s);
         p->timed = false;
@@ -609,7 +627,38 @@ extern "C" int frhip_pipeline_launch(frhip_pipeline *p, frhip_error *err)
         p->timed = true;
     }
     if (r == hipSuccess) r = hipMemcpyAsync(p->h_res, p->d_res, sizeof(ResultBlock), hipMemcpyDeviceToHost, s);
+    if (r == hipSuccess && p->ev_done) {
+        r = hipEventRecord(p->ev_done, s);
+        p->launched = true;
+    }
     if (r != hipSuccess) return hip_err(err, r, "launching the pipeline");
+    return FRHIP_OK;
+}
+
+extern "C" int frhip_pipeline_upload(frhip_pipeline *p, const uint8_t *file, uint64_t len, frhip_error *err)
+{
+    if (len != p->file_len)
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "file of %llu bytes, the pipeline was built for %llu",
+                       (unsigned long long)len, (unsigned long long)p->file_len);
+    (void)hipSetDevice(p->ctx->ordinal);
+    hipError_t r = hipSuccess;
+    if (!p->copy_stream) {
+        // runs launched before there was an end event to wait for
+        r = hipStreamSynchronize(p->ctx->stream);
+        if (r == hipSuccess) r = hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking);
+        if (r == hipSuccess) r = hipEventCreateWithFlags(&p->ev_uploaded, hipEventDisableTiming);
+        if (r == hipSuccess) r = hipEventCreateWithFlags(&p->ev_done, hipEventDisableTiming);
+        if (r != hipSuccess) return hip_err(err, r, "creating the upload stream");
+    }
+    // the last run has read the old bytes before the copy overwrites them
+    if (p->launched) r = hipStreamWaitEvent(p->copy_stream, p->ev_done, 0);
+    const uint64_t chunk = 64ull << 20;
+    for (uint64_t o = 0; r == hipSuccess && o < len; o += chunk)
+        r = hipMemcpyAsync(p->d_file + o, file + o, std::min(chunk, len - o), hipMemcpyHostToDevice,
+                           p->copy_stream);
+    if (r == hipSuccess) r = hipEventRecord(p->ev_uploaded, p->copy_stream);
+    if (r != hipSuccess) return hip_err(err, r, "uploading the file");
+    p->upload_pending = true;
     return FRHIP_OK;
 }
 

@@ -30,6 +30,9 @@ hipError_t launch_verify(const uint32_t *status, const uint32_t *produced, const
 hipError_t launch_bam_header(const uint8_t *u, const uint64_t *u_len, uint64_t *records_start, uint64_t *status,
                              hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s);
+// out_off[n] != expect: every member's extent emptied, *bad set (see pipeline.hip)
+hipError_t launch_guard_total(uint64_t *out_off, uint32_t *isize, uint64_t n, uint64_t expect, uint64_t *bad,
+                              hipStream_t s);
 // dst and src 16-byte aligned (device allocations are)
 hipError_t launch_copy(uint8_t *dst, const uint8_t *src, uint64_t n, hipStream_t s);
 

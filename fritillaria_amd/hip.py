@@ -170,6 +170,9 @@ def lib():
         "frhip_buf_copy_device": (C.c_int, [_vp, _vp, _vp, _ep]),
         "frhip_buf_copy_from_host": (C.c_int, [_vp, _u64, _vp, _u64, _ep]),
         "frhip_buf_ready_event": (_vp, [_vp]),
+        "frhip_host_alloc": (C.c_int, [_u64, C.POINTER(_vp), _ep]),
+        "frhip_host_free": (None, [_vp]),
+        "frhip_pipeline_upload": (C.c_int, [_vp, _vp, _u64, _ep]),
         "frhip_event_wait": (C.c_int, [_vp, _vp, _ep]),
         "frhip_event_synchronize": (C.c_int, [_vp, _ep]),
         "frhip_inflate_batch_device_timed": (C.c_int, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_vp), _vp, _vp, _ep]),
@@ -1086,6 +1089,29 @@ class PipelineConfig(C.Structure):
 STAGES = ["discovery", "inflate", "verify", "speculate", "reconcile", "columns", "soa", "total"]
 
 
+class PinnedBuffer:
+    """Page-locked host memory (frhip_host_alloc) as a numpy uint8 array."""
+
+    def __init__(self, nbytes: int):
+        p = _vp()
+        e = _Err()
+        _raise(lib().frhip_host_alloc(nbytes, C.byref(p), C.byref(e)), e)
+        self._p = p
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(nbytes, 1)).from_address(p.value))[:nbytes]
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self.array = None
+            lib().frhip_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BatchedPipeline:
     """A whole BGZF BAM resident in HBM, decoded to columns in one enqueue."""
 
@@ -1109,6 +1135,15 @@ class BatchedPipeline:
     def launch(self) -> None:
         e = _Err()
         _raise(lib().frhip_pipeline_launch(self._h, C.byref(e)), e)
+
+    def upload(self, file_bytes) -> None:
+        """Host-fed form: refills the resident file from host memory on the
+        pipeline's copy stream (asynchronous; the next launch waits for it).
+        `file_bytes` must stay alive until that launch."""
+        a = _u8(file_bytes)
+        self._upload_src = a
+        e = _Err()
+        _raise(lib().frhip_pipeline_upload(self._h, _ptr(a), a.size, C.byref(e)), e)
 
     def finish(self) -> None:
         e = _Err()

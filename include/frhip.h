@@ -160,6 +160,11 @@ int frhip_buf_copy_device(frhip_ctx *ctx, frhip_buf *dst, const frhip_buf *src, 
    (synchronous); offset + len beyond byte_len is FRHIP_E_INVALID_DEVICE_BATCH. */
 int frhip_buf_copy_from_host(frhip_buf *b, uint64_t offset, const void *src, uint64_t len, frhip_error *err);
 
+/* No reference counterpart. Pinned (page-locked) host memory, which host to
+   device copies read at full PCIe bandwidth. */
+int frhip_host_alloc(uint64_t len, void **out, frhip_error *err);
+void frhip_host_free(void *p);
+
 /* ---- BGZF member discovery (crates/fritillaria-bgzf/src/discover.rs:51-113) ---- */
 
 /* Host walk of member headers in buf[0..len), base = absolute offset of buf[0].

@@ -51,6 +51,18 @@ int frhip_pipeline_run(frhip_pipeline *p, frhip_error *err);
 int frhip_pipeline_launch(frhip_pipeline *p, frhip_error *err);
 int frhip_pipeline_finish(frhip_pipeline *p, frhip_error *err);
 
+/* The host-fed form: copies file[0..len) into the pipeline's resident file
+   buffer on a copy stream of the pipeline's own and returns at once; the copy
+   waits for the last launch to be done with the old bytes, and the next
+   launch waits for the copy, so a caller alternating two pipelines overlaps
+   one file's upload with the other's decode. len must be the length the
+   pipeline was created with and the file should have the same member table
+   (the use: the next batch of a stream of equal batches, or the same file
+   again from host memory); with device discovery, a member table whose
+   inflated total differs makes the run fail verification. For full PCIe
+   bandwidth the file should be in pinned memory (frhip_host_alloc). */
+int frhip_pipeline_upload(frhip_pipeline *p, const uint8_t *file, uint64_t len, frhip_error *err);
+
 uint64_t frhip_pipeline_members(const frhip_pipeline *p);
 uint64_t frhip_pipeline_inflated_len(const frhip_pipeline *p);
 uint64_t frhip_pipeline_compressed_len(const frhip_pipeline *p);

@@ -204,3 +204,43 @@ def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order)
     for rep in range(3):
         pipe.run()
         check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0), soa=True)
+
+
+def test_host_fed_pipelines_alternating(hip_ctx, oracle_mod):
+    """Two pipelines fed from pinned host memory in turn, each upload
+    overlapping the other's run: every run gives the oracle's columns. A
+    re-upload whose member table sums to another inflated size fails
+    verification (and a good upload afterwards runs again)."""
+    print("[gpu] pipeline: host-fed uploads alternating", flush=True)
+    import struct
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(2, n_records=40000)
+    pinned = fa.PinnedBuffer(s.compressed.size)
+    pinned.array[:] = s.compressed
+    pipes = [fa.BatchedPipeline(hip_ctx, pinned.array, use_graph=True, soa=True) for _ in range(2)]
+    pipes[0].upload(pinned.array)
+    for k in range(4):
+        cur, nxt = pipes[k % 2], pipes[(k + 1) % 2]
+        cur.launch()
+        nxt.upload(pinned.array)
+        cur.finish()
+        check_pipeline(cur, oracle_mod, s.compressed, check_bytes=(k == 0), soa=True)
+    # a member's ISIZE changed: the discovered total no longer matches
+    bad = s.compressed.copy()
+    d = oracle_mod.discover(s.compressed)
+    k = len(d.spans) // 2
+    end = int(d.spans["payload_start"][k] + d.spans["payload_len"][k])
+    isz = int(d.spans["isize"][k])
+    bad[end + 4:end + 8] = np.frombuffer(struct.pack("<I", isz - 100), np.uint8)
+    p = pipes[0]
+    p.upload(bad)
+    with pytest.raises(fa.FrError):
+        p.run()
+    assert p.records() == 0
+    p.upload(pinned.array)
+    p.run()
+    check_pipeline(p, oracle_mod, s.compressed, check_bytes=True, soa=True)
+    for p in pipes:
+        p.close()
+    pinned.close()
