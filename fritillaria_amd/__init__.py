@@ -14,4 +14,5 @@ from .hip import (  # noqa: F401
     device_memory, BamSoa, SOA_COLUMNS, AUX_ENTRY_DTYPE, BCF_COLUMNS, BcfRecordBatch, BcfScanner, bcf_parse_header,
     BcfFastqBatch, decode_bcf_fastq, decode_bcf_fastq_joined, join_bcf_fastq,
     InflateTimings, DecodeTimings, event_wait, event_synchronize, PinnedBuffer,
+    BinningIndex, IoError, CHUNK_DTYPE, REGION_DTYPE, REGION_RESULT_DTYPE,
 )

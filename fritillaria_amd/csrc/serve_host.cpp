@@ -16,11 +16,13 @@
 #include <vector>
 
 #include "../../include/frhip.h"
+#include "../../include/frhip_index.h"
 #include "../../include/frhip_serve.h"
 #include "bam_layout.h"
 #include "frhip_internal.h"
 #include "host_internal.h"
 #include "pipeline_internal.h"
+#include "region_internal.h"
 
 using namespace frhip;
 
@@ -59,9 +61,21 @@ struct ServeSlot {
     uint8_t *hb = nullptr; // read-back staging: member status, segment totals and bases
     uint8_t *du = nullptr;
     uint8_t *dc[N_COLS] = {};
+    // region batches: the regions, the chunk queries [q0[r], q0[r + 1]) of
+    // region r, an index failure per region, the filter's scratch and rows
+    bool regions = false;
+    std::vector<RegionDev> reg;
+    std::vector<uint64_t> reg_q0;
+    std::vector<int32_t> reg_status;
+    std::vector<uint32_t> q_region;
+    std::vector<frhip_query> reg_queries;
+    std::vector<frhip_query_result> reg_qres;
+    Scratch rscratch, sel;
+    uint64_t *d_sel = nullptr;
     // results of the last finished batch
     frhip_buf col_view[N_COLS] = {};
     frhip_buf arena_view = {};
+    frhip_buf sel_view = {};
     uint64_t last_records = 0;
     double t0_ms = 0, t1_ms = 0;
 };
@@ -120,10 +134,10 @@ namespace {
 // The per-member and per-segment arrays of one batch, packed into one upload.
 struct MetaLayout {
     uint64_t o_in_off, o_in_len, o_isize, o_crc, o_coff, o_out_off, o_until, o_len, o_seg, o_mb, o_start, o_slen,
-        bytes;
+        o_reg, o_sreg, bytes;
 };
 
-MetaLayout meta_layout(uint64_t nb, uint64_t ns)
+MetaLayout meta_layout(uint64_t nb, uint64_t ns, uint64_t nr = 0)
 {
     MetaLayout L{};
     uint64_t p = 0;
@@ -140,8 +154,28 @@ MetaLayout meta_layout(uint64_t nb, uint64_t ns)
     L.o_mb = take((ns + 1) * 4);
     L.o_start = take(ns * 8);
     L.o_slen = take(ns * 8);
+    L.o_reg = take(nr * sizeof(RegionDev));
+    L.o_sreg = take(nr ? ns * 4 : 0);
     L.bytes = p;
     return L;
+}
+
+// What one batch reads back: member status words, segment totals and bases,
+// and for a region batch each segment's first selected row and first row
+// failing the region check.
+struct BackLayout {
+    uint64_t o_tot, o_base, o_first, o_bad, bytes;
+};
+
+BackLayout back_layout(uint64_t nb, uint64_t ns, bool regions)
+{
+    BackLayout B{};
+    B.o_tot = align_up(nb * 12, 256);
+    B.o_base = B.o_tot + ns * 32;
+    B.o_first = B.o_base + (ns + 1) * 8;
+    B.o_bad = B.o_first + (regions ? (ns + 1) * 8 : 0);
+    B.bytes = B.o_bad + (regions ? ns * 8 : 0);
+    return B;
 }
 
 // Device scratch: inflate status, the work queue, the boundary scan, segment totals.
@@ -225,6 +259,7 @@ extern "C" int frhip_server_create_slots(frhip_ctx *ctx, const uint8_t *file, ui
         if (r == hipSuccess) r = hipEventCreate(&sl.ev1);
         for (int c = 0; c < N_COLS; ++c) sl.col_view[c] = frhip_buf{nullptr, 0, ctx->ordinal};
         sl.arena_view = frhip_buf{nullptr, 0, ctx->ordinal};
+        sl.sel_view = frhip_buf{nullptr, 0, ctx->ordinal};
     }
     if (r == hipSuccess) r = hipEventCreate(&s->ev_ref);
     if (r == hipSuccess) r = hipEventRecord(s->ev_ref, ctx->stream);
@@ -251,6 +286,8 @@ extern "C" void frhip_server_free(frhip_server *s)
         sl.arena.release();
         sl.recs.release();
         for (auto &c : sl.cols) c.release();
+        sl.rscratch.release();
+        sl.sel.release();
         sl.pin.release();
         if (sl.ev0) (void)hipEventDestroy(sl.ev0);
         if (sl.ev1) (void)hipEventDestroy(sl.ev1);
@@ -290,7 +327,12 @@ extern "C" uint64_t frhip_server_batch_footprint(const frhip_server *s, const fr
 
 extern "C" int frhip_server_slots(const frhip_server *s) { return (int)s->slots.size(); }
 
-extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query *q, uint64_t nq, frhip_error *err)
+namespace {
+
+// Plans and enqueues one batch of queries in a slot. A region batch (the
+// slot's reg / q_region filled by the caller) also runs the region filter
+// after the decode and reads back its per-segment results.
+int launch_batch(frhip_server *s, int slot, const frhip_query *q, uint64_t nq, bool regions, frhip_error *err)
 {
     if (slot < 0 || slot >= (int)s->slots.size())
         return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no slot %d", slot);
@@ -321,15 +363,19 @@ extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query 
     sl.last_records = 0;
     for (int c = 0; c < N_COLS; ++c) sl.col_view[c] = frhip_buf{nullptr, 0, ctx->ordinal};
     sl.arena_view = frhip_buf{nullptr, 0, ctx->ordinal};
+    sl.sel_view = frhip_buf{nullptr, 0, ctx->ordinal};
+    sl.regions = regions;
     sl.timed = false;
     sl.busy = true;
     sl.t0_ms = sl.t1_ms = 0;
     if (ns == 0) return FRHIP_OK;
 
     // host side of the member and segment arrays
-    const MetaLayout M = meta_layout(nb, ns);
+    const uint64_t nr = regions ? sl.reg.size() : 0;
+    const MetaLayout M = meta_layout(nb, ns, nr);
     const WorkLayout W = work_layout(nb, ns);
-    const uint64_t back = align_up(nb * 12, 256) + ns * 32 + (ns + 1) * 8;
+    const BackLayout B = back_layout(nb, ns, regions);
+    const uint64_t back = B.bytes;
     int rc;
     // a failure before the batch is on the stream leaves the slot free
     auto fail = [&](int k) {
@@ -345,12 +391,15 @@ extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query 
         uint64_t *slen = (uint64_t *)(h + M.o_slen);
         uint32_t *in_len = (uint32_t *)(h + M.o_in_len), *isz = (uint32_t *)(h + M.o_isize);
         uint32_t *crc = (uint32_t *)(h + M.o_crc), *seg = (uint32_t *)(h + M.o_seg), *mb = (uint32_t *)(h + M.o_mb);
+        uint32_t *sreg = (uint32_t *)(h + M.o_sreg);
+        if (nr) std::memcpy(h + M.o_reg, sl.reg.data(), nr * sizeof(RegionDev));
         uint64_t j = 0;
         for (uint64_t i = 0; i < nq; ++i) {
             if (seg_of[i] == NONE) continue;
             const Plan &p = plans[i];
             const uint64_t sq = seg_of[i], base = base_of[i];
             mb[sq] = (uint32_t)j;
+            if (nr) sreg[sq] = sl.q_region[i];
             sstart[sq] = base + p.start;
             slen[sq] = base + p.bytes;
             for (uint64_t m = p.m0; m < p.m1; ++m, ++j) {
@@ -381,6 +430,17 @@ extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query 
         if (!dc[c]) return fail(rc);
     }
     sl.du = du;
+    RegionFilterArgs fa{};
+    RegionScratchLayout RL{};
+    if (regions) {
+        RL = region_scratch_layout(cap, ns);
+        uint8_t *rs = (uint8_t *)sl.rscratch.get(RL.bytes, err, &rc);
+        if (!rs) return fail(rc);
+        sl.d_sel = (uint64_t *)sl.sel.get(cap * 8, err, &rc);
+        if (!sl.d_sel) return fail(rc);
+        region_bind_scratch(fa, rs, RL);
+        fa.sel = sl.d_sel;
+    }
     hipError_t r = hipEventRecord(sl.ev0, st);
     if (r == hipSuccess) r = hipMemcpyAsync(dm, h, M.bytes, hipMemcpyHostToDevice, st);
     uint32_t *d_st = (uint32_t *)(dw + W.o_st);
@@ -496,13 +556,31 @@ extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query 
     da.c.aux_start = (uint32_t *)dc[12];
     da.c.record_end = (uint32_t *)dc[13];
     if (r == hipSuccess) r = launch_bam_decode(da, cap, st);
+    if (regions) {
+        fa.buf = du;
+        fa.record_offsets = da.c.record_offsets;
+        fa.ref_id = da.c.ref_id;
+        fa.pos = da.c.pos;
+        fa.cigar_start = da.c.cigar_start;
+        fa.seq_start = da.c.seq_start;
+        fa.seg_base = seg_base;
+        fa.ns = (uint32_t)ns;
+        fa.seg_region = (const uint32_t *)(dm + M.o_sreg);
+        fa.regions = (const RegionDev *)(dm + M.o_reg);
+        if (r == hipSuccess) r = launch_fill_u64((uint64_t *)fa.seg_bad, ~0ull, ns, st);
+        if (r == hipSuccess) r = launch_region_filter(fa, cap, st);
+    }
 
-    // one read-back: member status, segment totals and bases
+    // one read-back: member status, segment totals and bases (and the
+    // region filter's per-segment results)
     uint8_t *hb = h; // the upload has been consumed once the stream reaches this copy
-    const uint64_t o_tot = align_up(nb * 12, 256), o_base = o_tot + ns * 32;
     if (r == hipSuccess) r = hipMemcpyAsync(hb, d_st, nb * 12, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(hb + o_tot, ra.seg_totals, ns * 32, hipMemcpyDeviceToHost, st);
-    if (r == hipSuccess) r = hipMemcpyAsync(hb + o_base, seg_base, (ns + 1) * 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(hb + B.o_tot, ra.seg_totals, ns * 32, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(hb + B.o_base, seg_base, (ns + 1) * 8, hipMemcpyDeviceToHost, st);
+    if (regions) {
+        if (r == hipSuccess) r = hipMemcpyAsync(hb + B.o_first, fa.sel_first, (ns + 1) * 8, hipMemcpyDeviceToHost, st);
+        if (r == hipSuccess) r = hipMemcpyAsync(hb + B.o_bad, fa.seg_bad, ns * 8, hipMemcpyDeviceToHost, st);
+    }
     if (r == hipSuccess) r = hipEventRecord(sl.ev1, st);
     if (r != hipSuccess) {
         // whatever was enqueued finishes before the slot is used again
@@ -512,6 +590,13 @@ extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query 
     sl.hb = hb;
     sl.timed = true;
     return FRHIP_OK;
+}
+
+} // namespace
+
+extern "C" int frhip_server_launch(frhip_server *s, int slot, const frhip_query *q, uint64_t nq, frhip_error *err)
+{
+    return launch_batch(s, slot, q, nq, false, err);
 }
 
 extern "C" int frhip_server_query(const frhip_server *s, int slot)
@@ -550,9 +635,9 @@ extern "C" int frhip_server_finish(frhip_server *s, int slot, frhip_query_result
     sl.t0_ms = a0;
     sl.t1_ms = a1;
     const uint8_t *hb = sl.hb;
-    const uint64_t o_tot = align_up(nb * 12, 256), o_base = o_tot + ns * 32;
+    const BackLayout B = back_layout(nb, ns, sl.regions);
     const uint32_t *status = (const uint32_t *)hb, *produced = status + nb, *crcs = status + 2 * nb;
-    const uint64_t *tot = (const uint64_t *)(hb + o_tot), *base = (const uint64_t *)(hb + o_base);
+    const uint64_t *tot = (const uint64_t *)(hb + B.o_tot), *base = (const uint64_t *)(hb + B.o_base);
 
     uint64_t j = 0;
     for (uint64_t i = 0; i < nq; ++i) {
@@ -620,6 +705,106 @@ extern "C" const frhip_buf *frhip_server_column(const frhip_server *s, int colum
 }
 
 extern "C" const frhip_buf *frhip_server_inflated(const frhip_server *s) { return frhip_server_slot_inflated(s, 0); }
+
+extern "C" int frhip_server_launch_regions(frhip_server *s, int slot, const frhip_index *ix, const frhip_region *r,
+                                           uint64_t n, frhip_error *err)
+{
+    if (slot < 0 || slot >= (int)s->slots.size())
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no slot %d", slot);
+    ServeSlot &sl = s->slots[slot];
+    if (sl.busy) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "slot %d has a batch in flight", slot);
+    if (!ix) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no index");
+    sl.reg.assign(n, RegionDev{});
+    sl.reg_q0.assign(n + 1, 0);
+    sl.reg_status.assign(n, FRHIP_OK);
+    sl.q_region.clear();
+    sl.reg_queries.clear();
+    std::vector<frhip_chunk> chunks;
+    for (uint64_t k = 0; k < n; ++k) {
+        sl.reg_q0[k] = sl.reg_queries.size();
+        RegionDev &d = sl.reg[k];
+        d.ref_id = r[k].ref_id;
+        d.bounded = (r[k].start || r[k].end) ? 1u : 0u;
+        d.start = r[k].start ? r[k].start : 1;
+        d.end = r[k].end ? r[k].end : (uint64_t)INT64_MAX;
+        if (r[k].ref_id < 0) {
+            sl.reg_status[k] = FRHIP_E_IO;
+            continue;
+        }
+        uint64_t nc = 0;
+        frhip_error e2{};
+        int rc = frhip_index_query(ix, (uint32_t)r[k].ref_id, r[k].start, r[k].end, nullptr, 0, &nc, &e2);
+        if (rc == FRHIP_OK && nc) {
+            chunks.resize(nc);
+            rc = frhip_index_query(ix, (uint32_t)r[k].ref_id, r[k].start, r[k].end, chunks.data(), nc, &nc, &e2);
+        }
+        if (rc != FRHIP_OK) {
+            sl.reg_status[k] = rc;
+            continue;
+        }
+        for (uint64_t c = 0; c < nc; ++c) {
+            sl.reg_queries.push_back(frhip_query{chunks[c].start, chunks[c].end});
+            sl.q_region.push_back((uint32_t)k);
+        }
+    }
+    sl.reg_q0[n] = sl.reg_queries.size();
+    return launch_batch(s, slot, sl.reg_queries.data(), sl.reg_queries.size(), true, err);
+}
+
+extern "C" int frhip_server_finish_regions(frhip_server *s, int slot, frhip_region_result *res, frhip_error *err)
+{
+    if (slot < 0 || slot >= (int)s->slots.size())
+        return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "no slot %d", slot);
+    ServeSlot &sl = s->slots[slot];
+    if (!sl.busy) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "slot %d has no batch", slot);
+    if (!sl.regions) return set_err(err, FRHIP_E_INVALID_DEVICE_BATCH, 0, "slot %d holds no region batch", slot);
+    const uint64_t nq = sl.reg_queries.size(), nr = sl.reg.size();
+    sl.reg_qres.assign(nq, frhip_query_result{});
+    const int rc = frhip_server_finish(s, slot, sl.reg_qres.data(), err);
+    if (rc != FRHIP_OK) return rc;
+    const uint64_t ns = sl.ns;
+    const BackLayout B = back_layout(sl.nb, ns, true);
+    const uint64_t *first = (const uint64_t *)(sl.hb + B.o_first), *bad = (const uint64_t *)(sl.hb + B.o_bad);
+    const uint64_t *base = (const uint64_t *)(sl.hb + B.o_base);
+    for (uint64_t k = 0; k < nr; ++k) {
+        frhip_region_result &o = res[k];
+        std::memset(&o, 0, sizeof o);
+        o.status = sl.reg_status[k];
+        o.chunks = sl.reg_q0[k + 1] - sl.reg_q0[k];
+        if (o.status != FRHIP_OK) continue;
+        uint64_t s_lo = NONE, s_hi = 0;
+        for (uint64_t qi = sl.reg_q0[k]; qi < sl.reg_q0[k + 1]; ++qi) {
+            const frhip_query_result &qr = sl.reg_qres[qi];
+            const uint64_t before = o.scanned;
+            o.members += qr.members;
+            o.inflated_bytes += qr.inflated_bytes;
+            o.scanned += qr.records;
+            if (o.status == FRHIP_OK && qr.status != FRHIP_OK) {
+                o.status = qr.status;
+                o.error_position = qr.error_position;
+            }
+            const uint64_t sq = sl.seg_of[qi];
+            if (sq == NONE) continue;
+            if (s_lo == NONE) s_lo = sq;
+            s_hi = sq + 1;
+            if (o.status == FRHIP_OK && bad[sq] != ~0ull) {
+                o.status = FRHIP_E_MALFORMED;
+                o.error_position = before + (bad[sq] - base[sq]);
+            }
+        }
+        if (o.status != FRHIP_OK || s_lo == NONE) continue;
+        o.first = first[s_lo];
+        o.records = first[s_hi] - first[s_lo];
+    }
+    sl.sel_view = frhip_buf{sl.d_sel, ns ? first[ns] * 8 : 0, s->ctx->ordinal};
+    return FRHIP_OK;
+}
+
+extern "C" const frhip_buf *frhip_server_slot_selection(const frhip_server *s, int slot)
+{
+    if (!s || slot < 0 || slot >= (int)s->slots.size()) return nullptr;
+    return &s->slots[slot].sel_view;
+}
 
 extern "C" int frhip_device_memory(frhip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, frhip_error *err)
 {

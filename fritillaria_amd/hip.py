@@ -83,8 +83,13 @@ class HipError(FrError):
     kind = E_DEVICE
 
 
+class IoError(FrError):
+    """Io: a malformed index (InvalidData) or a region it cannot answer (InvalidInput)."""
+    kind = E_IO
+
+
 _ERRORS = {c.kind: c for c in (InvalidBlock, ChecksumMismatch, SizeMismatch, InflateError, Malformed,
-                               InvalidDeviceBatch, HipUnavailable, HipError)}
+                               InvalidDeviceBatch, HipUnavailable, HipError, IoError)}
 
 
 class _Err(C.Structure):
@@ -253,6 +258,17 @@ def lib():
         "frhip_server_column": (_vp, [_vp, C.c_int]),
         "frhip_server_inflated": (_vp, [_vp]),
         "frhip_device_memory": (C.c_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), _ep]),
+        "frhip_server_launch_regions": (C.c_int, [_vp, C.c_int, _vp, _vp, _u64, _ep]),
+        "frhip_server_finish_regions": (C.c_int, [_vp, C.c_int, _vp, _ep]),
+        "frhip_server_slot_selection": (_vp, [_vp, C.c_int]),
+        "frhip_index_read": (C.c_int, [_vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_index_free": (None, [_vp]),
+        "frhip_index_format": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "frhip_index_references": (C.c_uint32, [_vp]),
+        "frhip_index_unplaced_unmapped": (C.c_int, [_vp, C.POINTER(_u64)]),
+        "frhip_index_metadata": (C.c_int, [_vp, C.c_uint32, _vp]),
+        "frhip_index_last_first_record_start": (C.c_int, [_vp, C.POINTER(_u64)]),
+        "frhip_index_query": (C.c_int, [_vp, C.c_uint32, _u64, _u64, _vp, _u64, C.POINTER(_u64), _ep]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -263,15 +279,14 @@ def lib():
 
 
 def exported_symbols() -> list:
-    """Every function declared in include/frhip.h, for the load/export check."""
+    """Every function declared in include/*.h, for the load/export check."""
+    import glob
     import re
     here = os.path.dirname(os.path.abspath(__file__))
     names = []
-    for h in ("frhip.h", "frhip_pipeline.h", "frhip_serve.h"):
-        p = os.path.join(os.path.dirname(here), "include", h)
-        if os.path.exists(p):
-            txt = open(p).read()
-            names += re.findall(r"\b(frhip_[a-z0-9_]+)\s*\(", txt)
+    for p in sorted(glob.glob(os.path.join(os.path.dirname(here), "include", "*.h"))):
+        txt = open(p).read()
+        names += re.findall(r"\b(frhip_[a-z0-9_]+)\s*\(", txt)
     return sorted(set(names))
 
 
@@ -1259,10 +1274,93 @@ def device_memory(ctx: HipContext):
     return f.value, t.value
 
 
+CHUNK_DTYPE = np.dtype([("start", "<u8"), ("end", "<u8")])
+REGION_DTYPE = np.dtype([("ref_id", "<i4"), ("reserved", "<u4"), ("start", "<u8"), ("end", "<u8")])
+REGION_RESULT_DTYPE = np.dtype([("first", "<u8"), ("records", "<u8"), ("chunks", "<u8"), ("scanned", "<u8"),
+                                ("members", "<u8"), ("inflated_bytes", "<u8"), ("status", "<i4"),
+                                ("reserved", "<u4"), ("error_position", "<u8")])
+INDEX_BAI, INDEX_CSI = 1, 2
+
+
+class BinningIndex:
+    """A BAI or CSI index read into host memory (include/frhip_index.h): the
+    reference's bai::fs::read / csi::fs::read and BinningIndex::query
+    (crates/fritillaria-csi/src/binning_index/index.rs:93-121)."""
+
+    def __init__(self, data):
+        d = _u8(data)
+        h = _vp()
+        e = _Err()
+        _raise(lib().frhip_index_read(_ptr(d), d.size, C.byref(h), C.byref(e)), e)
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def format(self) -> str:
+        return {INDEX_BAI: "bai", INDEX_CSI: "csi"}[lib().frhip_index_format(self._h, None, None)]
+
+    @property
+    def binning(self):
+        """(min_shift, depth)."""
+        a, b = C.c_uint32(), C.c_uint32()
+        lib().frhip_index_format(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def references(self) -> int:
+        return int(lib().frhip_index_references(self._h))
+
+    def unplaced_unmapped(self):
+        n = _u64()
+        return n.value if lib().frhip_index_unplaced_unmapped(self._h, C.byref(n)) else None
+
+    def metadata(self, ref_id: int):
+        out = np.zeros(4, np.uint64)
+        return tuple(int(v) for v in out) if lib().frhip_index_metadata(self._h, ref_id, _ptr(out)) else None
+
+    def last_first_record_start(self):
+        v = _u64()
+        return v.value if lib().frhip_index_last_first_record_start(self._h, C.byref(v)) else None
+
+    def query(self, ref_id: int, start=None, end=None) -> np.ndarray:
+        """Merged chunks (CHUNK_DTYPE) a query of [start, end] (1-based,
+        inclusive; None = open) on reference ref_id reads."""
+        n = _u64()
+        e = _Err()
+        _raise(lib().frhip_index_query(self._h, ref_id, start or 0, end or 0, None, 0, C.byref(n), C.byref(e)), e)
+        out = np.zeros(n.value, CHUNK_DTYPE)
+        if n.value:
+            _raise(lib().frhip_index_query(self._h, ref_id, start or 0, end or 0, _ptr(out), n.value, C.byref(n),
+                                           C.byref(e)), e)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _regions(regions) -> np.ndarray:
+    """(ref_id, start, end) triples, None for an open bound."""
+    out = np.zeros(len(regions), REGION_DTYPE)
+    for i, (rid, a, b) in enumerate(regions):
+        out[i] = (rid, 0, a or 0, b or 0)
+    return out
+
+
 class QueryServer:
     """Serves [vstart, vend) virtual-offset ranges of a BAM resident in HBM, many
     queries per launch (noodles csi Query semantics,
-    crates/fritillaria-csi/src/io/query.rs:66-82)."""
+    crates/fritillaria-csi/src/io/query.rs:66-82), and region queries through
+    a BAI/CSI index (crates/fritillaria-bam/src/io/reader/query.rs:104-152)."""
 
     def __init__(self, context: HipContext, file_bytes, slots: int = 2):
         self.ctx = context
@@ -1295,6 +1393,36 @@ class QueryServer:
         e = _Err()
         _raise(lib().frhip_server_finish(self._h, slot, _ptr(res), C.byref(e)), e)
         return res
+
+    def launch_regions(self, slot: int, index: BinningIndex, regions) -> None:
+        """Maps the (ref_id, start, end) regions through `index` and enqueues
+        their chunks as one batch in `slot`, with the overlap filter."""
+        r = _regions(regions)
+        e = _Err()
+        _raise(lib().frhip_server_launch_regions(self._h, slot, index.handle, _ptr(r) if len(r) else None, len(r),
+                                                 C.byref(e)), e)
+        self._launched[slot] = len(r)
+
+    def finish_regions(self, slot: int) -> np.ndarray:
+        """Waits for the slot's region batch; its REGION_RESULT_DTYPE results."""
+        res = np.zeros(self._launched.pop(slot, 0), REGION_RESULT_DTYPE)
+        e = _Err()
+        _raise(lib().frhip_server_finish_regions(self._h, slot, _ptr(res) if len(res) else None, C.byref(e)), e)
+        return res
+
+    def run_regions(self, index: BinningIndex, regions, slot: int = 0) -> np.ndarray:
+        self.launch_regions(slot, index, regions)
+        return self.finish_regions(slot)
+
+    def selection(self, slot: int = 0) -> DeviceBuffer:
+        """Kept rows (u64 indices into the slot's columns) of its last region batch."""
+        return DeviceBuffer(lib().frhip_server_slot_selection(self._h, slot), owned=False, keepalive=self)
+
+    def selection_to_host(self, slot: int = 0) -> np.ndarray:
+        buf = self.selection(slot)
+        out = np.empty(buf.byte_len() // 8, np.uint64)
+        buf.copy_to_host(out)
+        return out
 
     def batch_interval(self, slot: int):
         """(start, end) in ms from the server's creation of the slot's last batch on the device."""

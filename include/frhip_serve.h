@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "frhip.h"
+#include "frhip_index.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -86,6 +87,52 @@ const frhip_buf *frhip_server_slot_column(const frhip_server *s, int slot, int c
 const frhip_buf *frhip_server_slot_inflated(const frhip_server *s, int slot);
 const frhip_buf *frhip_server_column(const frhip_server *s, int column);
 const frhip_buf *frhip_server_inflated(const frhip_server *s);
+/* Region queries (crates/fritillaria-bam/src/io/reader.rs:373-390 with
+   crates/fritillaria-bam/src/io/reader/query.rs:104-152): each region is
+   mapped through a BAI/CSI index (frhip_index_query) to its merged chunks,
+   every chunk becomes one range query of the batch, and after the decode a
+   device pass keeps the records that overlap their region: same reference
+   and, for a bounded region, an alignment [pos + 1, pos + span] (span from
+   the raw CIGAR, at least 1) meeting the 1-based closed interval. The kept
+   rows (indices into the slot's columns, u64, file order within a region)
+   are one selection per batch; region i owns [first, first + records) of it.
+   start/end 0 leave that bound open; both 0 keep every record of the
+   reference, placed or not. */
+typedef struct {
+    int32_t ref_id;   /* reference sequence id */
+    uint32_t reserved;
+    uint64_t start;   /* 1-based, inclusive; 0 = from the start */
+    uint64_t end;     /* 1-based, inclusive; 0 = to the end */
+} frhip_region;
+
+typedef struct {
+    uint64_t first;          /* index of the region's first kept row in the selection */
+    uint64_t records;        /* rows kept */
+    uint64_t chunks;         /* chunks the index mapped the region to */
+    uint64_t scanned;        /* records read from those chunks */
+    uint64_t members;        /* BGZF members inflated for them */
+    uint64_t inflated_bytes;
+    int32_t status;          /* FRHIP_OK, or the error kind of this region */
+    uint32_t reserved;
+    /* FRHIP_E_MALFORMED: position in the failing chunk's inflated bytes, or,
+       for a record failing the region check (reference id or position below
+       -1, CIGAR op kind above 8), that record's ordinal among the scanned */
+    uint64_t error_position;
+} frhip_region_result;
+
+/* Maps the regions through ix and launches them as one batch in the slot
+   (see frhip_server_launch). A region the index rejects (FRHIP_E_IO: a
+   reference id it does not have, a bound past its maximum position) fails
+   alone. */
+int frhip_server_launch_regions(frhip_server *s, int slot, const frhip_index *ix, const frhip_region *r, uint64_t n,
+                                frhip_error *err);
+/* Waits for the slot's region batch and fills res[0..n); the slot's columns
+   then hold every record the chunks read, and its selection the kept rows. */
+int frhip_server_finish_regions(frhip_server *s, int slot, frhip_region_result *res, frhip_error *err);
+/* The selection of the slot's last finished region batch (u64 rows). Owned by
+   the server; valid until the slot's next launch. */
+const frhip_buf *frhip_server_slot_selection(const frhip_server *s, int slot);
+
 /* hipMemGetInfo of the context's device. */
 int frhip_device_memory(frhip_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, frhip_error *err);
 

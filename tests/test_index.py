@@ -1,0 +1,381 @@
+"""BAI/CSI indexes and region queries on the CPU: the oracle pinned by the
+reference's own unit vectors, by the samtools-written
+testdata/htslib_multiblock.bam.bai and by brute force; then the product's
+C-ABI index reader and chunk query against the oracle."""
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import golden_bytes
+from helpers import bam_header, bgzf, build_record
+
+from oracle import index as oi
+
+
+def vp(c, u=0):
+    return (c << 16) | u
+
+
+# ---------------------------------------------------------------------------
+# the reference's unit vectors
+
+
+def test_reg2bin_vectors():
+    # fritillaria-csi/src/binning_index/index/reference_sequence.rs test_reg2bin
+    assert oi.reg2bin(8, 8, 4, 2) == 9
+    assert oi.reg2bin(8, 13, 4, 2) == 9
+    assert oi.reg2bin(8, 16, 4, 2) == 9
+    assert oi.reg2bin(8, 17, 4, 2) == 1
+    assert oi.reg2bin(8, 143, 4, 2) == 0
+
+
+def test_reg2bins_against_brute_force():
+    # every bin whose interval overlaps [start, end], by enumerating bins
+    def bins_brute(start, end, min_shift, depth):
+        out = set()
+        t = 0
+        for l in range(depth + 1):
+            width = 1 << (min_shift + 3 * (depth - l))
+            for k in range(1 << (3 * l)):
+                lo, hi = k * width + 1, (k + 1) * width
+                if lo <= end and start <= hi:
+                    out.add(t + k)
+            t += 1 << (3 * l)
+        return out
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        a = int(rng.integers(1, 1 << 10))
+        b = int(rng.integers(a, (1 << 10) + 1))
+        assert oi.reg2bins(a, b, 4, 2) == bins_brute(a, b, 4, 2)
+    for _ in range(100):
+        a = int(rng.integers(1, 1 << 29))
+        b = min(a + int(rng.integers(0, 1 << 20)), (1 << 29) - 1)
+        got = oi.reg2bins(a, b, 14, 5)
+        # the single bin of a record spanning [a, b] is among them
+        assert oi.reg2bin(a, b, 14, 5) in got
+
+
+def test_optimize_and_merge_chunks_vectors():
+    # fritillaria-csi/src/binning_index.rs doc examples
+    chunks = [(2, 3), (5, 8), (7, 13), (21, 34)]
+    assert oi.optimize_chunks(chunks, 0) == [(2, 3), (5, 13), (21, 34)]
+    assert oi.optimize_chunks(chunks, 5) == [(5, 13), (21, 34)]
+    # touching chunks merge; an empty list stays empty
+    assert oi.optimize_chunks([(1, 4), (4, 9)], 0) == [(1, 9)]
+    assert oi.optimize_chunks([], 0) == []
+
+
+def test_min_offset_vectors():
+    # binned_index.rs test_min_offset: min_shift 4, depth 2, {1: 233}
+    ix = oi.Index("csi", 4, 2, [oi.RefIndex(loffsets={1: 233})], None)
+    assert oi.min_offset(ix, ix.refs[0], 8) == 233
+    assert oi.min_offset(ix, ix.refs[0], 144) == 0
+    # linear: window (start - 1) >> min_shift, 0 past the end
+    ix = oi.Index("bai", 14, 5, [oi.RefIndex(linear=[8, 8, 13])], None)
+    assert oi.min_offset(ix, ix.refs[0], 1) == 8
+    assert oi.min_offset(ix, ix.refs[0], 32769) == 13
+    assert oi.min_offset(ix, ix.refs[0], 49153) == 0
+
+
+def test_indexer_build_vector():
+    # indexer.rs test_build
+    ix = oi.build_index([(0, 8, 13, True, 0, 9), (0, 121393, 196418, False, 9, 3473408)], n_ref=1)
+    r = ix.refs[0]
+    assert r.bins == {4681: [(0, 9)], 73: [(9, 3473408)]}
+    assert r.linear == [0] + [9] * 11
+    assert r.metadata == (0, 3473408, 1, 1)
+    assert ix.unplaced_unmapped == 0
+    assert len(oi.build_index([], n_ref=2).refs) == 2
+
+
+def test_optimize_vector():
+    # reference_sequence.rs test_optimize: bins 9 and 1 fold into 0
+    ctx = []
+    ix = oi.Index("bai", 14, 5, [oi.RefIndex()], 0)
+    bins = {0: [(vp(0), vp(16384))], 1: [(vp(0), vp(32768))], 2: [(vp(0), vp(131072))],
+            9: [(vp(0), vp(16384))], 10: [(vp(0), vp(131072))]}
+    # run the fold through build_index's optimize by building from contexts
+    # is not possible for hand-made bins, so apply the same fold directly
+    work = {k: list(v) for k, v in bins.items()}
+    kept = {}
+    for bid in sorted(work, reverse=True):
+        cs = work.pop(bid)
+        pid = oi.parent_id(bid)
+        lo = min(cs, key=lambda c: c[0])
+        hi = max(cs, key=lambda c: c[0])
+        if pid is not None and (hi[1] >> 16) - (lo[0] >> 16) < (1 << 16) and pid in work:
+            work[pid].extend(cs)
+        else:
+            kept[bid] = cs
+    kept = {k: oi._merge_chunks_by_block(v) for k, v in kept.items()}
+    assert kept == {10: [(vp(0), vp(131072))], 2: [(vp(0), vp(131072))], 0: [(vp(0), vp(32768))]}
+    del ctx, ix
+
+
+def test_read_bins_bytes():
+    # bai/io/reader/index/reference_sequences/bins.rs test_read_bins: a bin 0
+    # without chunks and the metadata pseudo-bin 37450
+    body = bytes([0x02, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0x4A, 0x92, 0, 0, 0x02, 0, 0, 0]) + \
+        struct.pack("<QQQQ", 610, 1597, 55, 0)
+    data = b"BAI\x01" + struct.pack("<I", 1) + body + struct.pack("<I", 0)
+    ix = oi.read_bai(data)
+    assert ix.refs[0].bins == {0: []}
+    assert ix.refs[0].metadata == (610, 1597, 55, 0)
+    assert ix.unplaced_unmapped is None
+    assert oi.read_bai(data + struct.pack("<Q", 8)).unplaced_unmapped == 8
+    with pytest.raises(oi.IndexError_):
+        oi.read_bai(data[:-2])
+    bad_meta = data.replace(bytes([0x4A, 0x92, 0, 0, 0x02]), bytes([0x4A, 0x92, 0, 0, 0x03]))
+    with pytest.raises(oi.IndexError_):
+        oi.read_bai(bad_meta)
+
+
+# ---------------------------------------------------------------------------
+# whole-file decode and the samtools-written fixture index
+
+
+def decode_file(oracle_mod, data):
+    """Records, virtual offsets (start, end) and index fields of a whole BAM."""
+    d = oracle_mod.discover(data)
+    r = oracle_mod.inflate_batch(data, d.spans)
+    assert r.kind == 0
+    u = np.frombuffer(r.data, np.uint8)
+    rc, start, _, _ = oracle_mod.bam_parse_header(u)
+    assert rc == 0
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, start)
+    assert rc == 0 and tail == u.size
+    rec = np.asarray(rec, dtype=np.int64)
+    ends = rec + 4 + u[rec[:, None] + np.arange(4)].copy().view("<u4").reshape(-1).astype(np.int64) \
+        if len(rec) else rec
+    co = d.spans["compressed_offset"]
+    vo = oracle_mod.virtual_offsets(r.offsets, co, rec)
+    vo_end = oracle_mod.virtual_offsets(r.offsets, co, ends)
+    return u, rec, vo, vo_end, oi.record_fields(u, rec)
+
+
+@pytest.fixture(scope="module")
+def multiblock(oracle_mod):
+    data = golden_bytes("htslib_multiblock.bam")
+    return (data,) + decode_file(oracle_mod, data)
+
+
+def test_fixture_index_is_rebuilt_by_the_indexer(multiblock):
+    """The oracle's indexer over htslib_multiblock.bam rebuilds samtools'
+    index: bins, their chunks, the linear index, metadata and the unplaced
+    count."""
+    data, u, rec, vo, vo_end, f = multiblock
+    got = oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=1)
+    want = oi.read_index(golden_bytes("htslib_multiblock.bam.bai"))
+    assert want.kind == "bai" and len(want.refs) == 1 and len(rec) == 4000
+    assert got.refs[0].bins == want.refs[0].bins
+    assert got.refs[0].linear == want.refs[0].linear
+    assert got.refs[0].metadata == want.refs[0].metadata
+    assert got.unplaced_unmapped == want.unplaced_unmapped
+    # and written back, it reads the same
+    again = oi.read_bai(oi.write_bai(got))
+    assert again.refs[0].bins == want.refs[0].bins and again.refs[0].linear == want.refs[0].linear
+
+
+def random_regions(rng, n, ref_len, n_ref=1):
+    out = []
+    for _ in range(n):
+        rid = int(rng.integers(0, n_ref))
+        kind = rng.integers(0, 10)
+        if kind == 0:
+            out.append((rid, None, None))
+        elif kind == 1:
+            out.append((rid, int(rng.integers(1, ref_len)), None))
+        elif kind == 2:
+            out.append((rid, None, int(rng.integers(1, ref_len))))
+        else:
+            a = int(rng.integers(1, ref_len))
+            w = int(rng.choice([1, 10, 300, 5000, 100_000, 1_000_000]))
+            out.append((rid, a, min(a + int(rng.integers(0, w)), (1 << 29) - 1)))
+    return out
+
+
+def brute_force(f, rid, start, end):
+    keep, bad = oi.intersects(f, rid, start, end)
+    return np.flatnonzero(keep), bool(bad.any())
+
+
+def test_fixture_queries_equal_a_filtered_full_scan(multiblock):
+    data, u, rec, vo, vo_end, f = multiblock
+    ix = oi.read_index(golden_bytes("htslib_multiblock.bam.bai"))
+    rng = np.random.default_rng(11)
+    hits = 0
+    # the records sit in the first 400 kb of chr1's 5 Mb
+    for rid, a, b in random_regions(rng, 300, 420_000) + random_regions(rng, 100, 5_000_000):
+        rows, bad = oi.region_records(ix, f, vo, rid, a, b)
+        want, wbad = brute_force(f, rid, a, b)
+        assert not bad and not wbad
+        assert np.array_equal(rows, want), (rid, a, b)
+        hits += len(rows) > 0
+    assert hits > 200
+    # the whole reference, and regions past the end
+    assert len(oi.region_records(ix, f, vo, 0, 1, 5_000_000)[0]) == 4000
+    assert len(oi.region_records(ix, f, vo, 0, 6_000_000, 7_000_000)[0]) == 0
+    with pytest.raises(oi.IndexError_):
+        oi.query(ix, 1, 1, 10)
+    with pytest.raises(oi.IndexError_):
+        oi.query(ix, 0, 1, 1 << 29)
+
+
+def multi_ref_bam(seed=3, per_ref=(700, 0, 900), ref_len=400_000, unmapped=40, cap=20_000):
+    """A coordinate-sorted BAM over three references (the second empty), with
+    placed-unmapped records, long deletions and skips, and a tail of unplaced
+    records, framed in small BGZF members."""
+    rng = np.random.default_rng(seed)
+    recs = []
+    refs = [("chr%d" % (i + 1), ref_len) for i in range(len(per_ref))]
+    k = 0
+    for rid, n in enumerate(per_ref):
+        pos = np.sort(rng.integers(0, ref_len - 2000, size=n))
+        for p in pos:
+            L = int(rng.integers(20, 120))
+            seq = bytes(rng.choice(list(b"ACGT"), size=L).tolist())
+            kind = rng.integers(0, 12)
+            if kind == 0:
+                flag, cig = 4, []  # placed unmapped, no CIGAR
+            elif kind == 1:
+                cig = [(L // 2 << 4) | 0, (int(rng.integers(1, 30_000)) << 4) | 3, ((L - L // 2) << 4) | 0]
+                flag = 0
+            elif kind == 2:
+                cig = [(5 << 4) | 4, ((L - 5) << 4) | 0, (int(rng.integers(1, 50)) << 4) | 2]
+                flag = 16
+            else:
+                cig, flag = [(L << 4) | 0], 0
+            recs.append(build_record(b"r%06d" % k, seq, ref_id=rid, pos=int(p), flag=flag, cigar=cig,
+                                     next_ref=-1, next_pos=-1, tlen=0))
+            k += 1
+    for _ in range(unmapped):
+        recs.append(build_record(b"u%06d" % k, b"ACGT", ref_id=-1, pos=-1, flag=4, cigar=[],
+                                 next_ref=-1, next_pos=-1, tlen=0))
+        k += 1
+    raw = bam_header(refs, text=b"@HD\tVN:1.6\tSO:coordinate\n") + b"".join(recs)
+    return bgzf(raw, cap=cap), refs
+
+
+@pytest.fixture(scope="module")
+def multi(oracle_mod):
+    data, refs = multi_ref_bam()
+    out = decode_file(oracle_mod, data)
+    u, rec, vo, vo_end, f = out
+    ix = oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=len(refs))
+    return (data, refs, ix) + out
+
+
+@pytest.mark.parametrize("kind", ["bai", "csi"])
+def test_synthetic_queries_equal_a_filtered_full_scan(multi, kind):
+    data, refs, ix, u, rec, vo, vo_end, f = multi
+    if kind == "csi":
+        ix = oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=len(refs), kind="csi")
+        ix = oi.read_csi(oi.write_csi(ix))
+    else:
+        ix = oi.read_bai(oi.write_bai(ix))
+    assert ix.unplaced_unmapped == 40
+    rng = np.random.default_rng(17)
+    hits = 0
+    for rid, a, b in random_regions(rng, 300, 400_000, n_ref=3):
+        rows, bad = oi.region_records(ix, f, vo, rid, a, b)
+        want, _ = brute_force(f, rid, a, b)
+        assert not bad
+        assert np.array_equal(rows, want), (kind, rid, a, b)
+        hits += len(rows) > 0
+    assert hits > 100
+    assert oi.query(ix, 1, 1, 1000) == []
+
+
+def test_a_bad_cigar_op_in_a_read_record_is_an_error(oracle_mod):
+    recs = [build_record(b"a", b"ACGT", pos=10, cigar=[(4 << 4) | 0]),
+            build_record(b"b", b"ACGT", pos=20, cigar=[(4 << 4) | 9]),
+            build_record(b"c", b"ACGT", pos=30, cigar=[(4 << 4) | 0])]
+    data = bgzf(bam_header(text=b"@HD\tVN:1.6\tSO:coordinate\n") + b"".join(recs))
+    u, rec, vo, vo_end, f = decode_file(oracle_mod, data)
+    assert f["bad_op"].tolist() == [False, True, False]
+    keep, bad = oi.intersects(f, 0, 1, 100)
+    assert bad.tolist() == [False, True, False]
+    # unbounded: the CIGAR is never read
+    keep, bad = oi.intersects(f, 0)
+    assert keep.all() and not bad.any()
+    # another reference: never read either
+    keep, bad = oi.intersects(f, 1, 1, 100)
+    assert not keep.any() and not bad.any()
+
+
+# ---------------------------------------------------------------------------
+# the product's index reader and chunk query (C ABI, no GPU needed)
+
+
+def chunks_of(ix_prod, rid, a, b):
+    return [(int(c["start"]), int(c["end"])) for c in ix_prod.query(rid, a, b)]
+
+
+def test_product_reads_the_fixture_index():
+    import fritillaria_amd as fa
+    data = golden_bytes("htslib_multiblock.bam.bai")
+    ix = fa.BinningIndex(data)
+    want = oi.read_index(data)
+    assert ix.format == "bai" and ix.binning == (14, 5) and ix.references() == 1
+    assert ix.unplaced_unmapped() == want.unplaced_unmapped
+    assert ix.metadata(0) == want.refs[0].metadata
+    assert ix.metadata(1) is None
+    assert ix.last_first_record_start() == want.refs[0].linear[-1]
+    rng = np.random.default_rng(23)
+    for rid, a, b in random_regions(rng, 300, 420_000) + random_regions(rng, 50, 5_000_000):
+        assert chunks_of(ix, rid, a, b) == oi.query(want, rid, a, b), (rid, a, b)
+    with pytest.raises(fa.IoError):
+        ix.query(1, 1, 10)
+    with pytest.raises(fa.IoError):
+        ix.query(0, 1, 1 << 29)
+    with pytest.raises(fa.IoError):
+        ix.query(0, 1 << 29, None)
+    ix.close()
+
+
+@pytest.mark.parametrize("kind", ["bai", "csi"])
+def test_product_chunk_queries_equal_the_oracle(multi, kind):
+    import fritillaria_amd as fa
+    data, refs, ix, u, rec, vo, vo_end, f = multi
+    if kind == "csi":
+        built = oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=len(refs), kind="csi")
+        built.aux = b"\x00" * 28  # an aux block the reader must skip
+        raw = oi.write_csi(built)
+    else:
+        raw = oi.write_bai(ix)
+    want = oi.read_index(raw)
+    got = fa.BinningIndex(raw)
+    assert got.format == kind and got.references() == 3 and got.unplaced_unmapped() == 40
+    for r in range(3):
+        assert got.metadata(r) == want.refs[r].metadata
+    rng = np.random.default_rng(29)
+    for rid, a, b in random_regions(rng, 300, 400_000, n_ref=3):
+        assert chunks_of(got, rid, a, b) == oi.query(want, rid, a, b), (kind, rid, a, b)
+    # a region whose start is past its end still reads bin 0's chunks, as reg2bins does
+    assert chunks_of(got, 0, 300_000, 100) == oi.query(want, 0, 300_000, 100)
+
+
+def test_product_rejects_malformed_indexes():
+    import fritillaria_amd as fa
+    good = oi.write_bai(oi.build_index([(0, 8, 13, True, 0, 9), (0, 121393, 196418, False, 9, 3473408)], n_ref=1))
+    fa.BinningIndex(good).close()
+    bad = [
+        b"",                                   # empty
+        b"BAM\x01" + good[4:],                 # wrong magic
+        good[:-5],                             # truncated
+        good[:8] + struct.pack("<I", 1 << 30) + good[12:],  # a bin count the bytes cannot hold
+        good[:-8] + b"\x01\x02\x03",           # a partial unplaced count
+    ]
+    # a duplicate bin: the first bin written twice
+    n_bin = struct.unpack_from("<I", good, 8)[0]
+    first_bin = good[12:12 + 8 + 16]
+    dup = good[:8] + struct.pack("<I", n_bin + 1) + first_bin + good[12:]
+    bad.append(dup)
+    for b in bad:
+        with pytest.raises(fa.IoError):
+            fa.BinningIndex(b)
+        with pytest.raises(oi.IndexError_):
+            oi.read_index(b)
+    # bytes after the unplaced count are not read, as in the reference
+    fa.BinningIndex(good + b"\x01\x02\x03").close()
