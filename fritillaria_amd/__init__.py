@@ -15,4 +15,5 @@ from .hip import (  # noqa: F401
     BcfFastqBatch, decode_bcf_fastq, decode_bcf_fastq_joined, join_bcf_fastq,
     InflateTimings, DecodeTimings, event_wait, event_synchronize, PinnedBuffer,
     BinningIndex, IoError, CHUNK_DTYPE, REGION_DTYPE, REGION_RESULT_DTYPE,
+    Dialect, TextScanner, TextRecordBatch, TEXT_COLUMNS, FastaCompactor, DeviceReference, FASTA_COLUMNS,
 )

@@ -419,6 +419,13 @@ hipError_t launch_scan_u32_u64(const uint32_t *in, uint64_t *out, const uint64_t
     return hipGetLastError();
 }
 
+hipError_t launch_scan_u64_u64(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL((scan_kernel<uint64_t, uint64_t>), dim3(1), dim3(1024), 0, s, in, out, n_dev, n_host);
+    return hipGetLastError();
+}
+
 hipError_t launch_verify(const uint32_t *status, const uint32_t *produced, const uint32_t *crc, const PipeMembers &m,
                          const uint64_t *n_dev, uint64_t cap, unsigned long long *first_bad, hipStream_t s)
 {

@@ -25,6 +25,8 @@ hipError_t launch_spans(const uint8_t *f, uint64_t len, const uint64_t *cand, co
                         uint64_t *bad, hipStream_t s);
 hipError_t launch_scan_u32_u64(const uint32_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
                                hipStream_t s);
+hipError_t launch_scan_u64_u64(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
+                               hipStream_t s);
 hipError_t launch_verify(const uint32_t *status, const uint32_t *produced, const uint32_t *crc, const PipeMembers &m,
                          const uint64_t *n_dev, uint64_t cap, unsigned long long *first_bad, hipStream_t s);
 hipError_t launch_bam_header(const uint8_t *u, const uint64_t *u_len, uint64_t *records_start, uint64_t *status,

@@ -209,6 +209,17 @@ def lib():
         "frhip_bcf_parse_header": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_uint32), _ep]),
         "frhip_fastq_decode": (C.c_int, [_vp, _vp, _u64, C.POINTER(_vp), _ep]),
+        "frhip_text_scan": (C.c_int, [_vp, _vp, _u64, C.c_uint32, C.c_uint32, C.POINTER(_vp), _ep]),
+        "frhip_text_batch_len": (_u64, [_vp]),
+        "frhip_text_batch_headers": (_u64, [_vp]),
+        "frhip_text_batch_tail": (_u64, [_vp]),
+        "frhip_text_batch_column": (_vp, [_vp, C.c_int]),
+        "frhip_text_batch_free": (None, [_vp]),
+        "frhip_fasta_compact": (C.c_int, [_vp, _vp, C.POINTER(_vp), _ep]),
+        "frhip_fasta_contigs": (_u64, [_vp]),
+        "frhip_fasta_total_bases": (_u64, [_vp]),
+        "frhip_fasta_reference_column": (_vp, [_vp, C.c_int]),
+        "frhip_fasta_free": (None, [_vp]),
         "frhip_bcf_fastq_decode": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, C.POINTER(_vp), _vp,
                                              C.POINTER(_u64), C.POINTER(_u64), C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_uint32), C.POINTER(_vp), C.POINTER(_vp), _ep]),
@@ -848,6 +859,145 @@ class FastqScanner:
         offs = d.column_to_host("record_offsets")
         n, surv = len(d), d.survivors()
         return SpeculativeScan(offs, d.tail(), surv, "tiled" if surv == n else "walked")
+
+
+# ---------------------------------------------------------------------------
+# Text framing and FASTA compaction
+# ---------------------------------------------------------------------------
+
+class Dialect:
+    """Which first byte makes a line a header
+    (crates/fritillaria-text/src/columnar/batch.rs:3-35)."""
+
+    def __init__(self, comment: bytes, secondary: Optional[bytes] = None):
+        self.comment = comment[0]
+        self.secondary = secondary[0] if secondary else None
+
+
+Dialect.SAM = Dialect(b"@")
+Dialect.VCF = Dialect(b"#")
+Dialect.GFF = Dialect(b"#")
+Dialect.BED = Dialect(b"#")
+
+TEXT_COLUMNS = ["record_offsets", "record_ends", "header_offsets", "tabs", "field_starts"]
+
+
+class TextRecordBatch:
+    """Text columns in device memory (crates/fritillaria-text/src/columnar/device.rs:13-160),
+    all u64 offsets into the inflated batch, which this object keeps alive."""
+
+    def __init__(self, handle, source):
+        self._h = handle
+        self._source = source
+
+    def __len__(self) -> int:
+        return int(lib().frhip_text_batch_len(self._h))
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def header_count(self) -> int:
+        return int(lib().frhip_text_batch_headers(self._h))
+
+    def tail(self) -> int:
+        return int(lib().frhip_text_batch_tail(self._h))
+
+    def column(self, name: str) -> DeviceBuffer:
+        return DeviceBuffer(lib().frhip_text_batch_column(self._h, TEXT_COLUMNS.index(name)), owned=False,
+                            keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        out = np.empty(buf.byte_len() // 8, dtype=np.uint64)
+        buf.copy_to_host(out)
+        return out
+
+    def to_host(self) -> dict:
+        return {name: self.column_to_host(name) for name in TEXT_COLUMNS}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_text_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class TextScanner:
+    """TextScanner (crates/fritillaria-cuda/src/text.rs:10-66)."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def scan(self, batch, start: int, dialect: Dialect) -> TextRecordBatch:
+        """`batch` is a DeviceInflateBatch or a DeviceBuffer of text bytes."""
+        data = batch.data() if isinstance(batch, DeviceInflateBatch) else batch
+        out = _vp()
+        e = _Err()
+        sec = 0x100 if dialect.secondary is None else dialect.secondary
+        _raise(lib().frhip_text_scan(self.ctx.handle, None if data is None else data._h, start, dialect.comment, sec,
+                                     C.byref(out), C.byref(e)), e)
+        return TextRecordBatch(out, batch)
+
+
+FASTA_COLUMNS = [("sequence", np.uint8), ("sequence_offsets", np.uint64), ("sequence_lengths", np.uint64),
+                 ("record_offsets", np.uint64)]
+
+
+class DeviceReference:
+    """A compacted FASTA reference in device memory
+    (crates/fritillaria-fasta/src/columnar/device.rs:13-120)."""
+
+    def __init__(self, handle, source):
+        self._h = handle
+        self._source = source
+
+    def __len__(self) -> int:
+        return int(lib().frhip_fasta_contigs(self._h))
+
+    def total_bases(self) -> int:
+        return int(lib().frhip_fasta_total_bases(self._h))
+
+    def column(self, name: str) -> DeviceBuffer:
+        idx = [n for n, _ in FASTA_COLUMNS].index(name)
+        return DeviceBuffer(lib().frhip_fasta_reference_column(self._h, idx), owned=False, keepalive=self)
+
+    def column_to_host(self, name: str) -> np.ndarray:
+        buf = self.column(name)
+        dt = dict(FASTA_COLUMNS)[name]
+        out = np.empty(buf.byte_len() // np.dtype(dt).itemsize, dtype=dt)
+        buf.copy_to_host(out)
+        return out
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().frhip_fasta_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FastaCompactor:
+    """FastaCompactor (crates/fritillaria-cuda/src/fasta.rs:10-67)."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def compact(self, batch) -> DeviceReference:
+        data = batch.data() if isinstance(batch, DeviceInflateBatch) else batch
+        out = _vp()
+        e = _Err()
+        _raise(lib().frhip_fasta_compact(self.ctx.handle, None if data is None else data._h, C.byref(out),
+                                         C.byref(e)), e)
+        return DeviceReference(out, batch)
 
 
 # ---------------------------------------------------------------------------

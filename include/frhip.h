@@ -337,6 +337,67 @@ uint64_t frhip_fastq_batch_survivors(const frhip_fastq_batch *b);
 const frhip_buf *frhip_fastq_batch_column(const frhip_fastq_batch *b, int column);
 void frhip_fastq_batch_free(frhip_fastq_batch *b);
 
+/* ---- Text framing: TextScanner::scan (crates/fritillaria-cuda/src/text.rs:49-66, 196-375) ---- */
+
+typedef struct frhip_text_batch frhip_text_batch;
+
+/* Column indices of the text DeviceRecordBatch
+   (crates/fritillaria-text/src/columnar/device.rs:5-11); every column is u64.
+   field_starts has len + 1 entries (none without records): record r's tabs
+   are tabs[field_starts[r] .. field_starts[r + 1]). */
+typedef enum {
+    FRHIP_TEXT_RECORD_OFFSETS = 0,
+    FRHIP_TEXT_RECORD_ENDS = 1,   /* the record's newline, excluded */
+    FRHIP_TEXT_HEADER_OFFSETS = 2,
+    FRHIP_TEXT_TABS = 3,
+    FRHIP_TEXT_FIELD_STARTS = 4,
+    FRHIP_TEXT_N_COLUMNS = 5
+} frhip_text_column;
+
+/* A dialect without a secondary header byte. */
+#define FRHIP_TEXT_NO_SECONDARY 0x100u
+
+/* Lines and fields of the inflated batch from `start`, as
+   crates/fritillaria-text/src/columnar/batch.rs:56-86, 236-258 decodes them: a
+   line starts at `start` or after a newline; blank lines are skipped; a line
+   whose first byte is `comment` (or `secondary`, when below 256) is a header,
+   any other a record; a final line without a newline is not decoded and its
+   start is the tail (else the tail is the batch length). NULL or empty data,
+   or start >= its length: no lines, tail = min(start, length), as
+   text.rs:209-215. Quotes are data (no format here quotes a tab). */
+int frhip_text_scan(frhip_ctx *ctx, const frhip_buf *data, uint64_t start, uint32_t comment, uint32_t secondary,
+                    frhip_text_batch **out, frhip_error *err);
+uint64_t frhip_text_batch_len(const frhip_text_batch *b);
+uint64_t frhip_text_batch_headers(const frhip_text_batch *b);
+uint64_t frhip_text_batch_tail(const frhip_text_batch *b);
+const frhip_buf *frhip_text_batch_column(const frhip_text_batch *b, int column);
+void frhip_text_batch_free(frhip_text_batch *b);
+
+/* ---- FASTA: FastaCompactor::compact (crates/fritillaria-cuda/src/fasta.rs:45-67, 203-304) ---- */
+
+typedef struct frhip_fasta_reference frhip_fasta_reference;
+
+/* Columns of the DeviceReference (crates/fritillaria-fasta/src/columnar/device.rs:5-11). */
+typedef enum {
+    FRHIP_FASTA_SEQUENCE = 0,         /* u8: every contig's bases, back to back */
+    FRHIP_FASTA_SEQUENCE_OFFSETS = 1, /* u64: each contig's first base in SEQUENCE */
+    FRHIP_FASTA_SEQUENCE_LENGTHS = 2, /* u64 */
+    FRHIP_FASTA_RECORD_OFFSETS = 3,   /* u64: the contig's '>' in the batch */
+    FRHIP_FASTA_N_COLUMNS = 4
+} frhip_fasta_column;
+
+/* Contigs are '>' at a line start, from the first one on; a contig's
+   sequence runs from after its definition line to the next contig, and its
+   bases are those bytes without the newlines
+   (crates/fritillaria-fasta/src/columnar/record.rs:58-126); a last '>' whose
+   definition line has no newline is not a contig. NULL or empty data, or no
+   '>' at a line start: no contigs. */
+int frhip_fasta_compact(frhip_ctx *ctx, const frhip_buf *data, frhip_fasta_reference **out, frhip_error *err);
+uint64_t frhip_fasta_contigs(const frhip_fasta_reference *r);
+uint64_t frhip_fasta_total_bases(const frhip_fasta_reference *r);
+const frhip_buf *frhip_fasta_reference_column(const frhip_fasta_reference *r, int column);
+void frhip_fasta_free(frhip_fasta_reference *r);
+
 /* ---- BCF: BcfScanner::{scan, decode} (crates/fritillaria-cuda/src/bcf.rs:94-143, 451-470) ---- */
 
 /* Column indices of the BCF DeviceRecordBatch, in the order of
