@@ -356,6 +356,93 @@ def bcf_fastq_leg(fa, ctx, np, steps, schedule):
                                        f"({t_scan:.2f} s)"}}
 
 
+def text_fasta_leg(fa, ctx, np, steps):
+    """Text framing and FASTA compaction, device-resident. Text: a SAM-like
+    file of 1.73 GB, a 34.5 MB block of 100,000 records of 12 fields (varied
+    name, position, CIGAR, sequence and quality lengths) repeated 50 times; the
+    oracle decodes the block and every column sum is checked against it scaled
+    to the copies. FASTA: a 1 Gbp assembly, 16 contigs of 64 Mbp wrapped at 60
+    columns; the compacted sequence is compared byte for byte with the bases
+    it was written from."""
+    from oracle import text as ot
+    rng = np.random.default_rng(20260)
+    t0 = time.time()
+    recs = []
+    for i in range(100_000):
+        L = int(rng.integers(50, 250))
+        seq = b"ACGT" * (L // 4 + 1)
+        recs.append(b"read%07d\t%d\tchr%d\t%d\t60\t%dM\t*\t0\t0\t%s\t%s" %
+                    (i, int(rng.integers(0, 4096)), i % 22 + 1, i * 97, L, seq[:L], b"I" * L))
+    block = b"\n".join(recs) + b"\n"
+    copies = 50
+    text = np.frombuffer(b"@HD\tVN:1.6\tSO:unsorted\n" + block * copies, np.uint8)
+    hdr_len = len(b"@HD\tVN:1.6\tSO:unsorted\n")
+    want = ot.text_decode(block, 0, ot.SAM)
+    nrec, ntab, L = len(want.record_offsets), len(want.tabs), len(block)
+    dev = ctx.alloc(text.size)
+    dev.copy_from_host(text)
+    scanner = fa.TextScanner(ctx)
+    b = scanner.scan(dev, 0, fa.Dialect.SAM)
+    cols = b.to_host()
+    require(len(b) == nrec * copies and b.header_count() == 1 and b.tail() == text.size,
+            "text framing: record / header counts or tail")
+    shift = lambda col, n: int(col.astype(np.uint64).sum()) * copies + n * (hdr_len * copies + L * copies * (copies - 1) // 2)
+    for k, n in (("record_offsets", nrec), ("record_ends", nrec), ("tabs", ntab)):
+        require(int(cols[k].sum(dtype=np.uint64)) == shift(getattr(want, k), n) % (1 << 64),
+                f"text framing: {k} sum differs from the oracle's")
+    require(np.array_equal(cols["tabs"][:ntab], want.tabs + hdr_len), "text framing: first block's tabs")
+    del b, cols
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        scanner.scan(dev, 0, fa.Dialect.SAM).free()
+    ctx.synchronize()
+    tdt = (time.perf_counter() - t1) / steps
+    dev.free()
+    text_bytes = int(text.size)
+    del text
+    # FASTA
+    contigs, clen, width = 16, 64 * 1024 * 1024, 60
+    bases = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, contigs * clen, dtype=np.uint8)]
+    rows = clen // width
+    parts = []
+    for c in range(contigs):
+        sq = bases[c * clen:(c + 1) * clen]
+        body = np.full((rows, width + 1), 10, np.uint8)
+        body[:, :width] = sq[:rows * width].reshape(rows, width)
+        tail = sq[rows * width:]
+        parts += [np.frombuffer(b">chr%d synthetic contig\n" % (c + 1), np.uint8), body.reshape(-1)]
+        if tail.size:
+            parts += [tail, np.array([10], np.uint8)]
+    fasta = np.concatenate(parts)
+    del parts
+    log(f"generated {text_bytes} bytes of SAM text and a {fasta.size}-byte FASTA in {time.time() - t0:.1f}s")
+    dev = ctx.alloc(fasta.size)
+    dev.copy_from_host(fasta)
+    comp = fa.FastaCompactor(ctx)
+    r = comp.compact(dev)
+    require(len(r) == contigs and r.total_bases() == bases.size, "FASTA: contig count or total bases")
+    require((r.column_to_host("sequence_lengths") == clen).all(), "FASTA: sequence lengths")
+    require(np.array_equal(r.column_to_host("sequence"), bases), "FASTA: compacted sequence differs from its bases")
+    del r
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        comp.compact(dev).free()
+    ctx.synchronize()
+    fdt = (time.perf_counter() - t1) / steps
+    dev.free()
+    log(f"text framing: {tdt * 1e3:.2f} ms per {text_bytes / 1e9:.2f} GB call ({text_bytes / tdt / 1e9:.1f} GB/s); "
+        f"FASTA compaction {fdt * 1e3:.2f} ms per {fasta.size / 1e9:.2f} GB call ({fasta.size / fdt / 1e9:.1f} GB/s)")
+    return {"workload": "text framing of a 1.73 GB SAM-like file (5,000,000 records of 12 fields) and FASTA "
+                        "compaction of a 1 Gbp assembly (16 x 64 Mbp, 60-column lines), device-resident",
+            "text": {"bytes": text_bytes, "records": nrec * copies, "tabs": ntab * copies,
+                     "ms_per_call": round(tdt * 1e3, 3), "gbps": round(text_bytes / tdt / 1e9, 3), "steps": steps},
+            "fasta": {"bytes": int(fasta.size), "bases": int(bases.size), "contigs": contigs,
+                      "ms_per_call": round(fdt * 1e3, 3), "gbps": round(fasta.size / fdt / 1e9, 3), "steps": steps},
+            "parity": {"text_column_sums_equal_oracle": True, "fasta_sequence_equal_bases": True}}
+
+
 def host_fed_leg(fa, ctx, np, compressed, steps, schedule, order, window):
     """The batched large BAM fed from host memory: the compressed file sits in
     pinned host memory and every step uploads it over PCIe before the pipeline
@@ -440,9 +527,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--workload", choices=["bam", "bcf-fastq"], default="bam",
                     help="bcf-fastq: the mixed BCF + bgzipped FASTQ configuration, decoded together per call")
-    ap.add_argument("--configs", default="1,3,4",
+    ap.add_argument("--configs", default="1,3,4,text",
                     help="other configurations measured after the headline one: 1 plain small BAM per call, "
-                         "3 long-read BAM (batched and per call), 4 mixed BCF + FASTQ; 'none' to skip")
+                         "3 long-read BAM (batched and per call), 4 mixed BCF + FASTQ, text: text framing and "
+                         "FASTA compaction; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -581,6 +669,8 @@ def main():
         configs["long_read_bam"] = long_read_leg(fa, ctx, np, steps=3, schedule=args.schedule)
     if "4" in want:
         configs["bcf_fastq"] = bcf_fastq_leg(fa, ctx, np, steps=3, schedule=args.schedule)
+    if "text" in want:
+        configs["text_fasta"] = text_fasta_leg(fa, ctx, np, steps=3)
 
     variant = fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None
     two_phase = variant is not None and variant[3] == 3

@@ -262,44 +262,126 @@ __global__ void spans_kernel(const uint8_t *f, uint64_t len, const uint64_t *can
     }
 }
 
-// Exclusive scan by one workgroup of 1024 threads, 8 elements per thread per
-// step; out[n] receives the total. in and out may alias.
-template <typename TI, typename TO>
-__global__ __launch_bounds__(1024) void scan_kernel(const TI *in, TO *out, const uint64_t *n_dev, uint64_t n_host)
+// Exclusive scan in three passes, no scratch beyond the output: blocks of
+// SCAN_B elements (256 threads x 8) first park their sum in their own first
+// output slot; one workgroup scans those slots in place (and writes the total
+// to out[n]); then each block writes its elements' exclusive prefixes from
+// that base. Grids are fixed and stride over blocks, so a count that is only
+// known on the device (n_dev) needs no host round trip. in and out must not
+// alias.
+constexpr uint32_t SCAN_T = 256, SCAN_PER = 8, SCAN_B = SCAN_T * SCAN_PER;
+
+template <typename T> __device__ inline T scan_wave_incl(T v)
 {
-    __shared__ TO sums[1024];
-    __shared__ TO carry_s;
-    const uint64_t n = n_dev ? *n_dev : n_host;
-    if (threadIdx.x == 0) carry_s = 0;
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        T x = __shfl_up(v, o, 64);
+        if (lane >= o) v += x;
+    }
+    return v;
+}
+
+// Exclusive prefix of v over the workgroup's NW waves; *total: the sum.
+template <uint32_t NW, typename T> __device__ inline T scan_block_excl(T v, T *lds, T *total)
+{
+    const T inc = scan_wave_incl(v);
+    const uint32_t w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 63) lds[w] = inc;
     __syncthreads();
-    for (uint64_t base = 0; base < n; base += 1024 * 8) {
-        TO v[8];
-        TO s = 0;
-        uint64_t i0 = base + (uint64_t)threadIdx.x * 8;
+    T before = 0, all = 0;
+    for (uint32_t k = 0; k < NW; ++k) {
+        if (k < w) before += lds[k];
+        all += lds[k];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(const TI *in, TO *out, const uint64_t *n_dev,
+                                                             uint64_t n_host)
+{
+    __shared__ TO lds[SCAN_T / 64];
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    const uint64_t nb = (n + SCAN_B - 1) / SCAN_B;
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint64_t i0 = b * SCAN_B + (uint64_t)threadIdx.x * SCAN_PER;
+        TO sum = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (uint32_t k = 0; k < SCAN_PER; ++k)
+            if (i0 + k < n) sum += (TO)in[i0 + k];
+        TO all;
+        scan_block_excl<SCAN_T / 64>(sum, lds, &all);
+        if (threadIdx.x == 0) out[b * SCAN_B] = all;
+    }
+}
+
+template <typename TO>
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(TO *out, const uint64_t *n_dev, uint64_t n_host)
+{
+    __shared__ TO lds[16];
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    const uint64_t nb = (n + SCAN_B - 1) / SCAN_B;
+    TO run = 0;
+    for (uint64_t c0 = 0; c0 < nb; c0 += 1024) {
+        const uint64_t b = c0 + threadIdx.x;
+        const TO v = b < nb ? out[b * SCAN_B] : (TO)0;
+        TO all;
+        const TO ex = scan_block_excl<16>(v, lds, &all);
+        if (b < nb) out[b * SCAN_B] = run + ex;
+        run += all;
+    }
+    if (threadIdx.x == 0) out[n] = run;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(SCAN_T) void scan_down_kernel(const TI *in, TO *out, const uint64_t *n_dev,
+                                                           uint64_t n_host)
+{
+    __shared__ TO lds[SCAN_T / 64];
+    __shared__ TO base_s;
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    const uint64_t nb = (n + SCAN_B - 1) / SCAN_B;
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        if (threadIdx.x == 0) base_s = out[b * SCAN_B];
+        const uint64_t i0 = b * SCAN_B + (uint64_t)threadIdx.x * SCAN_PER;
+        TO v[SCAN_PER];
+        TO sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SCAN_PER; ++k) {
             v[k] = i0 + k < n ? (TO)in[i0 + k] : (TO)0;
-            s += v[k];
+            sum += v[k];
         }
-        sums[threadIdx.x] = s;
-        __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1) {
-            TO add = threadIdx.x >= off ? sums[threadIdx.x - off] : (TO)0;
-            __syncthreads();
-            sums[threadIdx.x] += add;
-            __syncthreads();
-        }
-        TO run = carry_s + sums[threadIdx.x] - s;
+        TO all;
+        TO run = scan_block_excl<SCAN_T / 64>(sum, lds, &all); // its barriers also publish base_s
+        run += base_s;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (uint32_t k = 0; k < SCAN_PER; ++k) {
             if (i0 + k < n) out[i0 + k] = run;
             run += v[k];
         }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry_s += sums[1023];
-        __syncthreads();
+        __syncthreads(); // base_s is rewritten for the next block
     }
-    if (threadIdx.x == 0) out[n] = carry_s;
+}
+
+template <typename TI, typename TO>
+hipError_t launch_scan(const TI *in, TO *out, const uint64_t *n_dev, uint64_t n_host, hipStream_t s)
+{
+    uint64_t grid = 1024;
+    if (!n_dev) {
+        grid = (n_host + SCAN_B - 1) / SCAN_B;
+        if (grid > 1024) grid = 1024;
+        if (grid == 0) grid = 1;
+    }
+    hipLaunchKernelGGL((scan_reduce_kernel<TI, TO>), dim3((uint32_t)grid), dim3(SCAN_T), 0, s, in, out, n_dev, n_host);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL((scan_blocks_kernel<TO>), dim3(1), dim3(1024), 0, s, out, n_dev, n_host);
+    r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL((scan_down_kernel<TI, TO>), dim3((uint32_t)grid), dim3(SCAN_T), 0, s, in, out, n_dev, n_host);
+    return hipGetLastError();
 }
 
 // First failing member in file order, with the check that failed
@@ -415,15 +497,13 @@ hipError_t launch_spans(const uint8_t *f, uint64_t len, const uint64_t *cand, co
 hipError_t launch_scan_u32_u64(const uint32_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL((scan_kernel<uint32_t, uint64_t>), dim3(1), dim3(1024), 0, s, in, out, n_dev, n_host);
-    return hipGetLastError();
+    return launch_scan(in, out, n_dev, n_host, s);
 }
 
 hipError_t launch_scan_u64_u64(const uint64_t *in, uint64_t *out, const uint64_t *n_dev, uint64_t n_host,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL((scan_kernel<uint64_t, uint64_t>), dim3(1), dim3(1024), 0, s, in, out, n_dev, n_host);
-    return hipGetLastError();
+    return launch_scan(in, out, n_dev, n_host, s);
 }
 
 hipError_t launch_verify(const uint32_t *status, const uint32_t *produced, const uint32_t *crc, const PipeMembers &m,

@@ -184,29 +184,86 @@ __global__ __launch_bounds__(THREADS) void text_count_kernel(TextArgs a)
     }
 }
 
-// One workgroup over the tiles, 1024 at a time: the carried line kind, the
-// adjusted record-tab counts, and exclusive prefixes of records, headers and
-// record tabs; the totals and the last line start of the batch.
-__global__ __launch_bounds__(1024) void text_tiles_kernel(TextArgs a)
+// The tile prefixes in three passes over groups of 1024 tiles (one tile per
+// thread): each group's totals, then one workgroup over the groups (the line
+// kind open at each group's start, and group bases), then each tile's kind
+// carried in and its record, header and record-tab bases.
+constexpr uint32_t GROUP = 1024;
+
+// A group's view of its tiles: the kind carried in from earlier tiles of the
+// group (TX_NONE before its first line start) and each tile's record tabs
+// when that kind is known.
+struct GroupTile {
+    uint32_t carry;           // in-group carry key (0: none)
+    uint64_t rec, hdr, known, pending, lsp;
+};
+
+__device__ inline GroupTile group_tile(const TextArgs &a, uint64_t i, uint32_t *l32, uint32_t *kall)
+{
+    GroupTile g{};
+    const bool in = i < a.n_tiles;
+    const uint32_t lk = in ? a.t_last_kind[i] : 0u;
+    const uint32_t key = lk ? ((threadIdx.x + 1) << 2 | lk) : 0u;
+    g.carry = block_excl_max<GROUP / 64>(key, l32, kall);
+    if (in) {
+        g.rec = a.t_rec[i];
+        g.hdr = a.t_hdr[i];
+        const uint32_t k = g.carry & 3u;
+        g.known = a.t_tabs_rec[i] + (k == TX_RECORD ? a.t_tabs_head[i] : 0u);
+        g.pending = k == TX_NONE ? a.t_tabs_head[i] : 0u;
+        g.lsp = a.t_last_ls[i] == ~0ull ? 0ull : a.t_last_ls[i] + 1;
+    }
+    return g;
+}
+
+__global__ __launch_bounds__(GROUP) void text_groups_reduce_kernel(TextArgs a)
+{
+    __shared__ uint64_t l64[GROUP / 64];
+    __shared__ uint32_t l32[GROUP / 64];
+    const uint64_t ng = (a.n_tiles + GROUP - 1) / GROUP;
+    for (uint64_t g = blockIdx.x; g < ng; g += gridDim.x) {
+        uint32_t kall = 0;
+        const GroupTile t = group_tile(a, g * GROUP + threadIdx.x, l32, &kall);
+        uint64_t s_rec, s_hdr, s_known, s_pending, s_ls;
+        block_excl_sum<GROUP / 64>(t.rec, l64, &s_rec);
+        block_excl_sum<GROUP / 64>(t.hdr, l64, &s_hdr);
+        block_excl_sum<GROUP / 64>(t.known, l64, &s_known);
+        block_excl_sum<GROUP / 64>(t.pending, l64, &s_pending);
+        block_excl_max<GROUP / 64>(t.lsp, l64, &s_ls);
+        if (threadIdx.x == 0) {
+            a.g_rec[g] = s_rec;
+            a.g_hdr[g] = s_hdr;
+            a.g_known[g] = s_known;
+            a.g_pending[g] = s_pending;
+            a.g_last_kind[g] = kall & 3u;
+            a.g_last_ls[g] = s_ls;
+        }
+    }
+}
+
+// One workgroup over the groups: the kind carried into each, its bases, the
+// batch totals and the last line start.
+__global__ __launch_bounds__(1024) void text_groups_scan_kernel(TextArgs a)
 {
     __shared__ uint64_t l64[16];
     __shared__ uint32_t l32[16];
+    const uint64_t ng = (a.n_tiles + GROUP - 1) / GROUP;
     uint64_t run_rec = 0, run_hdr = 0, run_tabs = 0, run_ls = 0;
     uint32_t run_kind = TX_NONE;
-    for (uint64_t c0 = 0; c0 < a.n_tiles; c0 += 1024) {
-        const uint64_t i = c0 + threadIdx.x;
-        const bool in = i < a.n_tiles;
-        const uint32_t lk = in ? a.t_last_kind[i] : 0u;
+    for (uint64_t c0 = 0; c0 < ng; c0 += 1024) {
+        const uint64_t g = c0 + threadIdx.x;
+        const bool in = g < ng;
+        const uint32_t lk = in ? a.g_last_kind[g] : 0u;
         const uint32_t key = lk ? ((threadIdx.x + 1) << 2 | lk) : 0u;
         uint32_t kall = 0;
         const uint32_t kex = block_excl_max<16>(key, l32, &kall);
         const uint32_t carry = kex ? (kex & 3u) : run_kind;
-        uint64_t tabs = 0, rec = 0, hdr = 0, lsp = 0;
+        uint64_t rec = 0, hdr = 0, tabs = 0, lsp = 0;
         if (in) {
-            tabs = a.t_tabs_rec[i] + (carry == TX_RECORD ? a.t_tabs_head[i] : 0u);
-            rec = a.t_rec[i];
-            hdr = a.t_hdr[i];
-            lsp = a.t_last_ls[i] == ~0ull ? 0ull : a.t_last_ls[i] + 1;
+            rec = a.g_rec[g];
+            hdr = a.g_hdr[g];
+            tabs = a.g_known[g] + (carry == TX_RECORD ? a.g_pending[g] : 0ull);
+            lsp = a.g_last_ls[g];
         }
         uint64_t s_rec, s_hdr, s_tabs, s_ls;
         const uint64_t e_rec = block_excl_sum<16>(rec, l64, &s_rec);
@@ -214,10 +271,10 @@ __global__ __launch_bounds__(1024) void text_tiles_kernel(TextArgs a)
         const uint64_t e_tabs = block_excl_sum<16>(tabs, l64, &s_tabs);
         block_excl_max<16>(lsp, l64, &s_ls);
         if (in) {
-            a.t_carry[i] = (uint8_t)carry;
-            a.rec_base[i] = run_rec + e_rec;
-            a.hdr_base[i] = run_hdr + e_hdr;
-            a.tab_base[i] = run_tabs + e_tabs;
+            a.g_carry[g] = carry;
+            a.g_rec[g] = run_rec + e_rec;    // totals become bases in place
+            a.g_hdr[g] = run_hdr + e_hdr;
+            a.g_known[g] = run_tabs + e_tabs;
         }
         run_rec += s_rec;
         run_hdr += s_hdr;
@@ -234,6 +291,30 @@ __global__ __launch_bounds__(1024) void text_tiles_kernel(TextArgs a)
         a.totals[TX_T_TABS] = run_tabs;
         a.totals[TX_T_LAST_LS] = run_ls ? run_ls - 1 : ~0ull;
         a.totals[TX_T_LAST_KIND] = run_kind;
+    }
+}
+
+__global__ __launch_bounds__(GROUP) void text_groups_down_kernel(TextArgs a)
+{
+    __shared__ uint64_t l64[GROUP / 64];
+    __shared__ uint32_t l32[GROUP / 64];
+    const uint64_t ng = (a.n_tiles + GROUP - 1) / GROUP;
+    for (uint64_t g = blockIdx.x; g < ng; g += gridDim.x) {
+        const uint64_t i = g * GROUP + threadIdx.x;
+        uint32_t kall = 0;
+        const GroupTile t = group_tile(a, i, l32, &kall);
+        const uint32_t carry = t.carry ? (t.carry & 3u) : a.g_carry[g];
+        const uint64_t tabs = t.known + (carry == TX_RECORD ? t.pending : 0ull);
+        uint64_t tot;
+        const uint64_t e_rec = block_excl_sum<GROUP / 64>(t.rec, l64, &tot);
+        const uint64_t e_hdr = block_excl_sum<GROUP / 64>(t.hdr, l64, &tot);
+        const uint64_t e_tabs = block_excl_sum<GROUP / 64>(tabs, l64, &tot);
+        if (i < a.n_tiles) {
+            a.t_carry[i] = (uint8_t)carry;
+            a.rec_base[i] = a.g_rec[g] + e_rec;
+            a.hdr_base[i] = a.g_hdr[g] + e_hdr;
+            a.tab_base[i] = a.g_known[g] + e_tabs;
+        }
     }
 }
 
@@ -285,7 +366,15 @@ hipError_t launch_text_count(const TextArgs &a, hipStream_t s)
 
 hipError_t launch_text_tiles(const TextArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(text_tiles_kernel, dim3(1), dim3(1024), 0, s, a);
+    uint64_t ng = text_groups(a.n_tiles);
+    const uint32_t grid = (uint32_t)(ng < 1024 ? (ng ? ng : 1) : 1024);
+    hipLaunchKernelGGL(text_groups_reduce_kernel, dim3(grid), dim3(GROUP), 0, s, a);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(text_groups_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+    r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(text_groups_down_kernel, dim3(grid), dim3(GROUP), 0, s, a);
     return hipGetLastError();
 }
 

@@ -103,6 +103,8 @@ extern "C" int frhip_text_scan(frhip_ctx *ctx, const frhip_buf *data, uint64_t s
     const uint64_t o_rec = take(T * 4), o_hdr = take(T * 4), o_tr = take(T * 4), o_th = take(T * 4),
                    o_lk = take(T), o_ls = take(T * 8), o_cy = take(T), o_rb = take((T + 1) * 8),
                    o_hb = take((T + 1) * 8), o_tb = take((T + 1) * 8), o_tot = take(TX_T_N * 8);
+    const uint64_t G = text_groups(T);
+    const uint64_t o_g64 = take(G * 8 * 5), o_g32 = take(G * 4 * 2);
     int rc;
     uint8_t *s = (uint8_t *)ctx->tx_scratch.get(p, err, &rc);
     if (!s) return rc;
@@ -117,6 +119,13 @@ extern "C" int frhip_text_scan(frhip_ctx *ctx, const frhip_buf *data, uint64_t s
     a.hdr_base = (uint64_t *)(s + o_hb);
     a.tab_base = (uint64_t *)(s + o_tb);
     a.totals = (uint64_t *)(s + o_tot);
+    a.g_rec = (uint64_t *)(s + o_g64);
+    a.g_hdr = a.g_rec + G;
+    a.g_known = a.g_hdr + G;
+    a.g_pending = a.g_known + G;
+    a.g_last_ls = a.g_pending + G;
+    a.g_last_kind = (uint32_t *)(s + o_g32);
+    a.g_carry = a.g_last_kind + G;
     hipError_t r = launch_text_count(a, st);
     if (r == hipSuccess) r = launch_text_tiles(a, st);
     uint64_t tot[TX_T_N] = {};

@@ -10,6 +10,7 @@ namespace frhip {
 // tile t covering [t * 4096, (t + 1) * 4096) of the buffer.
 constexpr uint32_t TX_TILE = 4096;
 inline uint64_t tx_tiles(uint64_t len) { return (len + TX_TILE - 1) / TX_TILE; }
+inline uint64_t text_groups(uint64_t tiles) { return (tiles + 1023) / 1024; }
 
 enum { TX_NONE = 0, TX_RECORD = 1, TX_HEADER = 2 };
 
@@ -27,6 +28,12 @@ struct TextArgs {
     uint64_t *t_last_ls;          // last line start in the tile, or ~0
     uint8_t *t_carry;             // kind of the line open at the tile's start
     uint64_t *rec_base, *hdr_base, *tab_base; // n_tiles + 1 each
+    // per group of 1024 tiles: records, headers, record tabs whose line kind is
+    // known in the group and those waiting on the kind carried in, the group's
+    // last line kind and last line start (+1, 0 none); the scan turns the first
+    // three into bases and sets the carried kind
+    uint64_t *g_rec, *g_hdr, *g_known, *g_pending, *g_last_ls;
+    uint32_t *g_last_kind, *g_carry;
     uint64_t *totals;             // TX_T_N
     // outputs
     uint64_t *record_offsets, *record_ends, *header_offsets, *tabs, *field_starts;
