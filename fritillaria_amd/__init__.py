@@ -16,4 +16,6 @@ from .hip import (  # noqa: F401
     InflateTimings, DecodeTimings, event_wait, event_synchronize, PinnedBuffer,
     BinningIndex, IoError, CHUNK_DTYPE, REGION_DTYPE, REGION_RESULT_DTYPE,
     Dialect, TextScanner, TextRecordBatch, TEXT_COLUMNS, FastaCompactor, DeviceReference, FASTA_COLUMNS,
+    HipCompressor, CompressedBatch, DeviceBgzfWriter, framed_size, MAX_COMPRESSIBLE_PAYLOAD, DEFAULT_PAYLOAD_SIZE,
+    EOF_BLOCK,
 )

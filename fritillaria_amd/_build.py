@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["inflate.hip", "bam.hip", "pipeline.hip", "fastq.hip", "soa.hip", "noalloc.hip", "api.cpp", "bam_host.cpp", "pipeline_host.cpp",
-               "fastq_host.cpp", "serve_host.cpp", "region.hip", "index_host.cpp", "text.hip", "fasta.hip", "text_host.cpp",
+               "fastq_host.cpp", "serve_host.cpp", "region.hip", "index_host.cpp", "text.hip", "fasta.hip", "text_host.cpp", "deflate.hip", "writer_host.cpp",
                "bcf_host.cpp", "mixed_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

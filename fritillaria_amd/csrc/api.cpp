@@ -279,6 +279,7 @@ void ctx_release(frhip_ctx *c)
     c->fq_big.release();
     c->rec_scratch.release();
     c->tx_scratch.release();
+    c->wr_scratch.release();
     c->pinned.release();
     if (c->x2n) (void)hipFree(c->x2n);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);

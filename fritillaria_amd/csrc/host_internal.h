@@ -83,7 +83,7 @@ struct frhip_ctx {
     int inflate_schedule = 16; // index into INFLATE_SCHEDULES (DEFAULT_INFLATE_SCHEDULE)
     uint32_t two_phase_min = 8192; // smaller batches of a two-phase schedule run the one-wave form
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
-        rec_scratch, tx_scratch;
+        rec_scratch, tx_scratch, wr_scratch;
     frhip::PinnedScratch pinned;
     // device address of this GPU's no-allocation scratch array (noalloc.hip)
     void *noalloc_scratch = nullptr;

@@ -142,6 +142,8 @@ _lib = None
 _vp = C.c_void_p
 _u64 = C.c_uint64
 _ep = C.POINTER(_Err)
+# frhip_sink: int (*)(void *user, const uint8_t *p, uint64_t n)
+_SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint8), C.c_uint64)
 
 
 def lib():
@@ -220,6 +222,18 @@ def lib():
         "frhip_fasta_total_bases": (_u64, [_vp]),
         "frhip_fasta_reference_column": (_vp, [_vp, C.c_int]),
         "frhip_fasta_free": (None, [_vp]),
+        "frhip_compress_batch_device": (C.c_int, [_vp, _vp, _vp, _u64, C.POINTER(_vp), _vp, _vp, _ep]),
+        "frhip_framed_size": (_u64, [_u64, _u64]),
+        "frhip_bgzf_writer_create": (C.c_int, [_vp, _SINK, _vp, _u64, _u64, C.POINTER(_vp), _ep]),
+        "frhip_bgzf_writer_write_batch": (C.c_int, [_vp, _vp, _vp, _u64, _ep]),
+        "frhip_bgzf_writer_write_all": (C.c_int, [_vp, _vp, _ep]),
+        "frhip_bgzf_writer_finish": (C.c_int, [_vp, _ep]),
+        "frhip_bgzf_writer_blocks_written": (_u64, [_vp]),
+        "frhip_bgzf_writer_batches_run": (_u64, [_vp]),
+        "frhip_bgzf_writer_compressed_bytes": (_u64, [_vp]),
+        "frhip_bgzf_writer_uncompressed_bytes": (_u64, [_vp]),
+        "frhip_bgzf_writer_virtual_position": (_u64, [_vp]),
+        "frhip_bgzf_writer_free": (None, [_vp]),
         "frhip_bcf_fastq_decode": (C.c_int, [_vp, _vp, _u64, _vp, _u64, _u64, C.POINTER(_vp), _vp,
                                              C.POINTER(_u64), C.POINTER(_u64), C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_uint32), C.POINTER(_vp), C.POINTER(_vp), _ep]),
@@ -998,6 +1012,149 @@ class FastaCompactor:
         _raise(lib().frhip_fasta_compact(self.ctx.handle, None if data is None else data._h, C.byref(out),
                                          C.byref(e)), e)
         return DeviceReference(out, batch)
+
+
+# ---------------------------------------------------------------------------
+# Writing BGZF from device payloads (include/frhip_write.h)
+# ---------------------------------------------------------------------------
+
+MAX_COMPRESSIBLE_PAYLOAD = 65505
+DEFAULT_PAYLOAD_SIZE = 65280
+DEFAULT_CHUNKS_PER_BATCH = 1024
+FRAMING_DEFLATED, FRAMING_STORED, FRAMING_EMPTY = 0, 1, 2
+EOF_BLOCK = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+
+
+def framed_size(payload_len: int, deflate_len: int) -> int:
+    """framed_size (crates/fritillaria-core/src/compress.rs:133-142)."""
+    return int(lib().frhip_framed_size(payload_len, deflate_len))
+
+
+class CompressedBatch:
+    """Framed BGZF blocks back to back (crates/fritillaria-core/src/compress.rs:157-247),
+    held in device memory; `offsets` (n + 1) are the block boundaries."""
+
+    def __init__(self, device: DeviceBuffer, offsets: np.ndarray, framing: np.ndarray):
+        self.device = device
+        self._offsets = offsets
+        self.framing = framing
+
+    def __len__(self) -> int:
+        return len(self._offsets) - 1
+
+    def offsets(self) -> np.ndarray:
+        return self._offsets
+
+    def byte_len(self) -> int:
+        return int(self._offsets[-1])
+
+    def data(self) -> bytes:
+        out = np.empty(self.byte_len(), np.uint8)
+        if out.size:
+            self.device.copy_to_host(out)
+        return out.tobytes()
+
+    def block(self, i: int) -> bytes:
+        return self.data()[int(self._offsets[i]):int(self._offsets[i + 1])]
+
+
+class HipCompressor:
+    """DeviceBlockCompressor (crates/fritillaria-core/src/device.rs:350-372) on
+    this library's own DEFLATE kernels."""
+
+    def __init__(self, context: Optional[HipContext] = None, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+
+    def name(self) -> str:
+        return "hip"
+
+    def device_ordinal(self) -> int:
+        return self.ctx.ordinal
+
+    def compress_batch_device(self, data: DeviceBuffer, bounds) -> CompressedBatch:
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        n = max(b.size - 1, 0)
+        offs = np.zeros(n + 1, np.uint64)
+        frm = np.zeros(max(n, 1), np.uint8)
+        out = _vp()
+        e = _Err()
+        _raise(lib().frhip_compress_batch_device(self.ctx.handle, data._h if data is not None else None,
+                                                 _ptr(b) if b.size else None, b.size, C.byref(out), _ptr(offs),
+                                                 _ptr(frm), C.byref(e)), e)
+        return CompressedBatch(DeviceBuffer(out), offs, frm[:n])
+
+
+class DeviceBgzfWriter:
+    """DeviceBgzfWriter (crates/fritillaria-bgzf/src/device_write.rs:70-259):
+    writes BGZF to `sink` (an object with write(bytes), default an in-memory
+    bytearray) from payloads in device memory. finish() writes the EOF block
+    and returns the sink."""
+
+    def __init__(self, sink=None, context: Optional[HipContext] = None, chunks_per_batch: int = 0,
+                 payload_size: int = 0, ordinal: int = 0):
+        self.ctx = context or HipContext(ordinal)
+        self.sink = bytearray() if sink is None else sink
+        self._error = None
+
+        def write(_user, p, n):
+            try:
+                chunk = C.string_at(p, n)
+                if isinstance(self.sink, bytearray):
+                    self.sink += chunk
+                else:
+                    self.sink.write(chunk)
+                return 0
+            except Exception as exc:  # reported by the library as FRHIP_E_IO
+                self._error = exc
+                return 1
+
+        self._cb = _SINK(write)
+        h = _vp()
+        e = _Err()
+        _raise(lib().frhip_bgzf_writer_create(self.ctx.handle, self._cb, None, chunks_per_batch, payload_size,
+                                              C.byref(h), C.byref(e)), e)
+        self._h = h
+
+    def write_batch(self, data: DeviceBuffer, bounds) -> None:
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        e = _Err()
+        _raise(lib().frhip_bgzf_writer_write_batch(self._h, data._h if data is not None else None,
+                                                   _ptr(b) if b.size else None, b.size, C.byref(e)), e)
+
+    def write_all_device(self, data: DeviceBuffer) -> None:
+        e = _Err()
+        _raise(lib().frhip_bgzf_writer_write_all(self._h, data._h if data is not None else None, C.byref(e)), e)
+
+    def blocks_written(self) -> int:
+        return int(lib().frhip_bgzf_writer_blocks_written(self._h))
+
+    def batches_run(self) -> int:
+        return int(lib().frhip_bgzf_writer_batches_run(self._h))
+
+    def compressed_bytes(self) -> int:
+        return int(lib().frhip_bgzf_writer_compressed_bytes(self._h))
+
+    def uncompressed_bytes(self) -> int:
+        return int(lib().frhip_bgzf_writer_uncompressed_bytes(self._h))
+
+    def virtual_position(self) -> int:
+        return int(lib().frhip_bgzf_writer_virtual_position(self._h))
+
+    def finish(self):
+        e = _Err()
+        _raise(lib().frhip_bgzf_writer_finish(self._h, C.byref(e)), e)
+        return self.sink
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().frhip_bgzf_writer_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,187 @@
+"""BGZF written from device payloads: the compressor's blocks decoded by zlib
+and by this library's inflate, the framing choice, byte-exact stored and
+empty blocks, determinism, the ratio against htslib's own output on the
+reference's fixtures, and the writer's contract
+(crates/fritillaria-bgzf/tests/device_write.rs,
+crates/fritillaria-cuda/tests/nvcomp_compress.rs)."""
+import numpy as np
+import pytest
+
+from conftest import golden_bytes
+
+from oracle import bgzf_write as ow
+
+pytestmark = pytest.mark.gpu
+
+# crates/fritillaria-cuda/tests/nvcomp_compress.rs:219-259 holds the device
+# compressor to 8% above htslib's bytes on these fixtures (the host one, in
+# crates/fritillaria-bgzf/tests/compress.rs:185-211, to 10%)
+TOLERANCE = 0.08
+
+
+def device_bytes(ctx, data: bytes):
+    buf = ctx.alloc(len(data))
+    if data:
+        buf.copy_from_host(np.frombuffer(data, np.uint8))
+    return buf
+
+
+def inflated_with_bounds(oracle_mod, raw: bytes):
+    d = oracle_mod.discover(raw)
+    r = oracle_mod.inflate_batch(raw, d.spans)
+    assert r.kind == 0
+    return bytes(r.data), np.asarray(r.offsets, dtype=np.uint64)
+
+
+def check_stream(stream: bytes, data: bytes, bounds, codec=None):
+    ms = ow.members(stream)
+    assert len(ms) == len(bounds) - 1, "one block per chunk"
+    for i, (_, size, payload, _) in enumerate(ms):
+        assert payload == data[int(bounds[i]):int(bounds[i + 1])], i
+    if codec is not None:
+        import fritillaria_amd as fa
+        spans, _ = fa.discover(stream)
+        back = codec.inflate_batch(stream, spans)
+        assert back.data.tobytes() == data[int(bounds[0]):int(bounds[-1])]
+    return ms
+
+
+@pytest.fixture(scope="module")
+def compressor(hip_ctx):
+    import fritillaria_amd as fa
+    return fa.HipCompressor(hip_ctx)
+
+
+@pytest.mark.parametrize("name", ["pacbio_hifi.bam", "ont_ultralong.bam", "kg_phase3.bcf", "htslib_multiblock.bam",
+                                  "giab_hg002.bcf"])
+def test_fixture_round_trip_and_ratio(hip_ctx, compressor, codec, oracle_mod, name):
+    print(f"[gpu] write: {name}", flush=True)
+    raw = golden_bytes(name)
+    data, bounds = inflated_with_bounds(oracle_mod, raw)
+    out = compressor.compress_batch_device(device_bytes(hip_ctx, data), bounds)
+    stream = out.data()
+    assert np.array_equal(np.diff(out.offsets()), [m[1] for m in check_stream(stream, data, bounds, codec)])
+    # the framing of every block is the one choose_framing gives its sizes
+    for i, m in enumerate(ow.members(stream)):
+        n = int(bounds[i + 1] - bounds[i])
+        body = m[1] - 26
+        assert out.framing[i] == ow.choose_framing(n, body if out.framing[i] == ow.DEFLATED else 1 << 40)
+    excess = len(stream) / len(raw) - 1.0
+    print(f"[gpu] write: {name} htslib {len(raw)} ours {len(stream)} ({excess * 100:+.1f}%)", flush=True)
+    assert excess <= TOLERANCE, f"{name}: {excess * 100:.1f}% larger than htslib"
+
+
+def test_stored_and_empty_blocks_are_byte_exact(hip_ctx, compressor):
+    print("[gpu] write: stored and empty blocks", flush=True)
+    rng = np.random.default_rng(5)
+    sizes = [1, 100, 65505, 0, 65280, 0, 3, 4096]
+    chunks = [rng.integers(0, 256, n, dtype=np.uint8).tobytes() for n in sizes]
+    data = b"".join(chunks)
+    bounds = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    out = compressor.compress_batch_device(device_bytes(hip_ctx, data), bounds)
+    for i, c in enumerate(chunks):
+        got = out.block(i)
+        if not c:
+            assert got == ow.EOF_BLOCK and out.framing[i] == ow.EMPTY
+        else:
+            # random bytes do not compress: stored, exactly as the framing kernel writes it
+            assert out.framing[i] == ow.STORED and got == ow.stored_block(c), i
+    assert out.byte_len() == sum(ow.framed_size(n, 1 << 40) for n in sizes)
+
+
+def test_deterministic_and_mixed_content(hip_ctx, compressor, codec):
+    print("[gpu] write: determinism over mixed content", flush=True)
+    rng = np.random.default_rng(6)
+    parts = []
+    for k in range(40):
+        kind = k % 5
+        n = int(rng.integers(0, 65506))
+        if kind == 0:
+            parts.append(bytes([int(rng.integers(0, 256))]) * n)       # one byte repeated
+        elif kind == 1:
+            parts.append(rng.integers(0, 4, n, dtype=np.uint8).tobytes())  # 2 bits of entropy
+        elif kind == 2:
+            unit = rng.integers(0, 256, int(rng.integers(1, 3000)), dtype=np.uint8).tobytes()
+            parts.append((unit * (n // len(unit) + 1))[:n])             # long repeats
+        elif kind == 3:
+            parts.append(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+        else:
+            parts.append(b"ACGT" * (n // 4) + b"A" * (n % 4))
+    data = b"".join(parts)
+    bounds = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    dev = device_bytes(hip_ctx, data)
+    a = compressor.compress_batch_device(dev, bounds).data()
+    b = compressor.compress_batch_device(dev, bounds).data()
+    assert a == b
+    check_stream(a, data, bounds, codec)
+    assert len(a) < len(data) * 0.7
+
+
+def test_writer_contract(hip_ctx, codec):
+    """crates/fritillaria-bgzf/tests/device_write.rs: boundaries survive, the
+    output does not depend on the batch size, the EOF block ends every file,
+    counters and virtual positions, write_all chunking, refused inputs."""
+    print("[gpu] write: the writer", flush=True)
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(7)
+    sizes = [int(rng.integers(0, 5000)) for _ in range(41)]
+    data = b"".join(rng.integers(0, 8, n, dtype=np.uint8).tobytes() for n in sizes)
+    bounds = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    dev = device_bytes(hip_ctx, data)
+    outputs = {}
+    for cpb in [1, 3, 7, 1024]:
+        w = fa.DeviceBgzfWriter(context=hip_ctx, chunks_per_batch=cpb)
+        w.write_batch(dev, bounds)
+        assert w.batches_run() == -(-41 // cpb)
+        assert w.blocks_written() == 41 and w.uncompressed_bytes() == len(data)
+        stream = bytes(w.finish())
+        assert stream.endswith(ow.EOF_BLOCK)
+        assert w.compressed_bytes() == len(stream) - 28
+        outputs[cpb] = stream
+    assert len(set(outputs.values())) == 1, "the output depends on the batch size"
+    ms = check_stream(outputs[1][:-28], data, bounds, codec)
+    # virtual positions name the next block, block-aligned
+    w = fa.DeviceBgzfWriter(context=hip_ctx)
+    starts = []
+    for i in range(5):
+        starts.append(w.virtual_position())
+        w.write_batch(dev, bounds[i:i + 2])
+    assert starts == [m[0] << 16 for m in ms[:5]]
+    w.close()
+    # an empty file is just the EOF block; no chunks is a no-op
+    w = fa.DeviceBgzfWriter(context=hip_ctx)
+    w.write_batch(dev, [0])
+    assert w.blocks_written() == 0 and bytes(w.finish()) == ow.EOF_BLOCK
+    # write_all chunks at the payload size: 1000 + 1000 + 500
+    w = fa.DeviceBgzfWriter(context=hip_ctx, payload_size=1000)
+    w.write_all_device(device_bytes(hip_ctx, data[:2500]))
+    assert w.blocks_written() == 3
+    stream = bytes(w.finish())
+    assert [len(m[2]) for m in ow.members(stream)] == [1000, 1000, 500, 0]
+    # refused: a chunk too large to frame, bounds past the buffer, no bounds
+    big = device_bytes(hip_ctx, bytes(70000))
+    w = fa.DeviceBgzfWriter(context=hip_ctx)
+    with pytest.raises(fa.Malformed):
+        w.write_batch(big, [0, 65506])
+    with pytest.raises(fa.Malformed):
+        w.write_batch(device_bytes(hip_ctx, bytes(10)), [0, 11])
+    with pytest.raises(fa.Malformed):
+        w.write_batch(device_bytes(hip_ctx, bytes(10)), [])
+    w.write_batch(big, [0, 65505])  # the maximum is accepted
+    assert len(ow.members(bytes(w.finish()))) == 2
+
+
+def test_synthetic_bam_round_trip_and_ratio(hip_ctx, compressor, codec, oracle_mod):
+    """200,000 short-read records: a reader's batch written back at its member
+    boundaries, against zlib level 6 on the same members."""
+    print("[gpu] write: 200k-record BAM", flush=True)
+    from fritillaria_amd import synth
+    s = synth.config_bam(1, n_records=200_000)
+    raw = s.compressed.tobytes()
+    data, bounds = inflated_with_bounds(oracle_mod, raw)
+    out = compressor.compress_batch_device(device_bytes(hip_ctx, data), bounds)
+    stream = out.data()
+    check_stream(stream, data, bounds, codec)
+    excess = len(stream) / len(raw) - 1.0
+    print(f"[gpu] write: synthetic BAM zlib-6 {len(raw)} ours {len(stream)} ({excess * 100:+.1f}%)", flush=True)
+    assert excess <= TOLERANCE
