@@ -443,6 +443,87 @@ def text_fasta_leg(fa, ctx, np, steps):
             "parity": {"text_column_sums_equal_oracle": True, "fasta_sequence_equal_bases": True}}
 
 
+def write_leg(fa, ctx, np, steps, n_records=5_000_000):
+    """The device BGZF writer: an uncompressed BAM stream of the batched-large
+    generator (n_records reads of 100-250 bp, indels at p=0.01), resident in
+    HBM, chunked at 65,280 bytes (htslib's payload size) and compressed into
+    framed BGZF blocks in one compress_batch_device call. Parity: every block
+    inflates with system zlib to its chunk, and its trailer holds the chunk's
+    CRC32 and length. CPU baseline: zlib level 6 on the same chunks, one chunk
+    per task on every host core (htslib's multithreaded bgzf write)."""
+    import zlib
+    from concurrent.futures import ThreadPoolExecutor
+
+    from fritillaria_amd import synth
+    t0 = time.time()
+    u, _ = synth.bam_stream(n_records, synth.CONFIG_SEED + 2, read_range=(100, 250), indel_p=0.01,
+                            ref_len=1_000_000)
+    U = int(u.size)
+    P = 65280
+    bounds = np.append(np.arange(0, U, P, dtype=np.uint64), np.uint64(U))
+    n_blocks = bounds.size - 1
+    log(f"generated a {U}-byte BAM stream ({n_records} records) for the writer in {time.time() - t0:.1f}s")
+    dev = ctx.alloc(U)
+    dev.copy_from_host(u)
+    comp = fa.HipCompressor(ctx)
+    cb = comp.compress_batch_device(dev, bounds)
+    data = np.frombuffer(cb.data(), np.uint8)
+    offs = cb.offsets().astype(np.int64)
+    framing = np.bincount(cb.framing, minlength=3)
+    del cb
+
+    def check(i):
+        o, e = int(offs[i]), int(offs[i + 1])
+        blk = data[o:e].tobytes()
+        lo, hi = int(bounds[i]), int(bounds[i + 1])
+        chunk = u[lo:hi].tobytes()
+        bsize = int.from_bytes(blk[16:18], "little") + 1
+        ok = blk[:4] == b"\x1f\x8b\x08\x04" and bsize == e - o
+        ok = ok and zlib.decompress(blk[18:-8], -15) == chunk
+        ok = ok and int.from_bytes(blk[-8:-4], "little") == zlib.crc32(chunk)
+        return ok and int.from_bytes(blk[-4:], "little") == len(chunk)
+
+    cores = cpu_cores()
+    with ThreadPoolExecutor(cores) as ex:
+        bad = n_blocks - sum(ex.map(check, range(n_blocks)))
+    require(bad == 0, f"device BGZF writer: {bad} of {n_blocks} blocks do not inflate to their chunks")
+    C_dev = int(offs[-1])
+    del data
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        comp.compress_batch_device(dev, bounds).device.free()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t1) / steps
+    dev.free()
+    # CPU: zlib level 6 over a bounded sample of the chunks, every core
+    sample = min(n_blocks, 4096)
+
+    def deflate(i):
+        co = zlib.compressobj(6, zlib.DEFLATED, -15)
+        return len(co.compress(u[int(bounds[i]):int(bounds[i + 1])].tobytes()) + co.flush()) + 26
+
+    t1 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        zsizes = list(ex.map(deflate, range(sample)))
+    cdt = time.perf_counter() - t1
+    cpu_bytes = int(bounds[sample])
+    dev_sample = int(offs[sample])
+    log(f"device BGZF writer: {dt * 1e3:.2f} ms per {U / 1e9:.2f} GB call ({U / dt / 1e9:.2f} GB/s in), "
+        f"{C_dev} bytes out ({C_dev / U:.3f} of input); zlib -6 on {cores} threads {cpu_bytes / cdt / 1e9:.2f} GB/s, "
+        f"device output {dev_sample / sum(zsizes) - 1:+.1%} against zlib -6 over the first {sample} blocks")
+    return {"workload": f"BGZF write of an uncompressed BAM stream resident in HBM ({n_records:,} reads of the "
+                        "batched-large generator), 65,280-byte chunks, one compress_batch_device call",
+            "input_bytes": U, "blocks": n_blocks, "output_bytes": C_dev, "ratio": round(C_dev / U, 4),
+            "framing": {"deflated": int(framing[0]), "stored": int(framing[1]), "empty": int(framing[2])},
+            "ms_per_call": round(dt * 1e3, 3), "gbps_in": round(U / dt / 1e9, 3), "steps": steps,
+            "size_vs_zlib6": round(dev_sample / sum(zsizes) - 1, 4),
+            "parity": {"every_block_inflates_to_its_chunk": True, "crc_and_isize_trailers_match": True},
+            "cpu_baseline": {"value": round(cpu_bytes / cdt / 1e9, 3), "unit": "GB/s in", "cores": cores,
+                             "kind": "port", "sample": f"zlib 1.2.11 level 6 on the first {sample} chunks "
+                                                       f"({cpu_bytes} bytes), {cdt:.2f} s"}}
+
+
 def host_fed_leg(fa, ctx, np, compressed, steps, schedule, order, window):
     """The batched large BAM fed from host memory: the compressed file sits in
     pinned host memory and every step uploads it over PCIe before the pipeline
@@ -507,6 +588,36 @@ def bcf_fastq_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def leg_bench(args):
+    """One leg of the default run alone (--workload plain-small | long-read |
+    text-fasta | write): the same measurement, parity checks and CPU baseline,
+    printed as one JSON line whose value is the leg's own GB/s figure."""
+    import numpy as np
+
+    import fritillaria_amd as fa
+
+    ctx = fa.HipContext(0)
+    schedule = args.schedule if args.schedule >= 0 else 16
+    steps = max(1, args.steps)
+    if args.workload == "plain-small":
+        leg = plain_small_leg(fa, ctx, np, steps=steps)
+        metric, value, ms = "BAM GB/s inflated→HBM columns, plain per-call path, small file", leg["gbps"], leg["ms_per_call"]
+    elif args.workload == "long-read":
+        leg = long_read_leg(fa, ctx, np, steps=steps, schedule=schedule)
+        metric, value, ms = ("long-read BAM GB/s inflated→HBM columns, batched (bit-exact)", leg["batched"]["gbps"],
+                             leg["batched"]["ms_per_step"])
+    elif args.workload == "text-fasta":
+        leg = text_fasta_leg(fa, ctx, np, steps=steps)
+        metric, value, ms = "text framing GB/s (SAM-like lines and tabs)", leg["text"]["gbps"], leg["text"]["ms_per_call"]
+    else:
+        leg = write_leg(fa, ctx, np, steps=steps)
+        metric, value, ms = "BGZF write GB/s of input, device-resident", leg["gbps_in"], leg["ms_per_call"]
+    line = {"metric": metric, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic", "config": {"workload": leg["workload"]}, "leg": leg}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -525,12 +636,14 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (range query) measurement")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--workload", choices=["bam", "bcf-fastq"], default="bam",
-                    help="bcf-fastq: the mixed BCF + bgzipped FASTQ configuration, decoded together per call")
-    ap.add_argument("--configs", default="1,3,4,text",
+    ap.add_argument("--workload", choices=["bam", "bcf-fastq", "plain-small", "long-read", "text-fasta", "write"],
+                    default="bam",
+                    help="bcf-fastq: the mixed BCF + bgzipped FASTQ configuration, decoded together per call; "
+                         "plain-small, long-read, text-fasta, write: that leg of the default run alone")
+    ap.add_argument("--configs", default="1,3,4,text,write",
                     help="other configurations measured after the headline one: 1 plain small BAM per call, "
                          "3 long-read BAM (batched and per call), 4 mixed BCF + FASTQ, text: text framing and "
-                         "FASTA compaction; 'none' to skip")
+                         "FASTA compaction, write: the device BGZF writer; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -542,6 +655,8 @@ def main():
 
     if args.workload == "bcf-fastq":
         return bcf_fastq_bench(args)
+    if args.workload != "bam":
+        return leg_bench(args)
 
     import numpy as np
 
@@ -671,6 +786,8 @@ def main():
         configs["bcf_fastq"] = bcf_fastq_leg(fa, ctx, np, steps=3, schedule=args.schedule)
     if "text" in want:
         configs["text_fasta"] = text_fasta_leg(fa, ctx, np, steps=3)
+    if "write" in want:
+        configs["bgzf_write"] = write_leg(fa, ctx, np, steps=3)
 
     variant = fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None
     two_phase = variant is not None and variant[3] == 3
