@@ -893,6 +893,8 @@ hipError_t launch_bcf_decode(const BcfDecodeArgs &a, hipStream_t s)
 hipError_t launch_bam_spec(const BamSpecArgs &a, hipStream_t s)
 {
     if (a.nb == 0) return hipSuccess;
+    // the walk writes each member's offsets 16 bytes at a time into its slab
+    if (((uintptr_t)a.rel_offs & 15) != 0) return hipErrorInvalidValue;
     // one-wave workgroups: the walks are latency-bound chains, spread over every CU
     const uint32_t wgrid = (a.nb + 63) / 64;
     if (a.format == CHAIN_BCF) {

@@ -40,6 +40,9 @@ inline BamScratchLayout bam_layout(uint64_t nb)
     L.o_fscan = take((nb + 1) * 8);
     L.o_fstate = take(4 * 4);
     L.bytes = p;
+    // the offsets slab takes 16-byte stores: its base must be 16-byte aligned
+    // within an allocation that is itself at least 256-byte aligned
+    static_assert(BAM_CAPB % 8 == 0, "member slabs of u16 offsets must be a multiple of 16 bytes");
     return L;
 }
 

@@ -52,6 +52,10 @@ constexpr uint64_t BAM_NONE = ~0ull;          // block has no entry
 // most records that can start in one member (BCF's are >= 32 B), rounded up
 // to a multiple of 8 so that each member's u16 offsets start 16-byte aligned
 constexpr uint32_t BAM_CAPB = (65536 / 32 + 1 + 7) & ~7u;
+// the walk stores offsets eight at a time as one 16-byte store into each
+// member's slab: the slab must hold a member's records and stay 16-byte aligned
+static_assert(BAM_CAPB % 8 == 0, "member slabs of u16 offsets must be a multiple of 16 bytes");
+static_assert(BAM_CAPB >= 65536 / 32 + 1, "a member slab must hold every record a member can start");
 constexpr uint32_t BAM_MODE_FALLBACK = 255;
 
 // Length-prefixed record chains the boundary scan handles.
