@@ -288,86 +288,32 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
     for (uint32_t j = head + 4 * nw; j < len; ++j) dst[j] = src[j];
 }
 
-// dst[0, len) = src[0, len) by the 32 lanes of a half-wave (gl = lane in the
-// half): aligned 4-byte words of dst from consecutive lanes (one or two cache
-// lines per store), each from two aligned source words funnel-shifted, and
-// the up to 3 head and 3 tail bytes around them by single lanes. No source
-// word read lies past the word of the last source byte.
-__device__ __forceinline__ void group_copy(uint8_t *dst, const uint8_t *src, uint32_t len, uint32_t gl)
+// The 16 bytes at p (any alignment): one 16-byte load from the aligned word
+// at or below p, plus the next word when p is unaligned (it holds the last
+// wanted byte, so it never crosses a page), funnel-shifted.
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+__device__ __forceinline__ uint4 ld16u(const uint8_t *p)
 {
-    const uintptr_t d = (uintptr_t)dst, e = d + len;
-    const uintptr_t D0 = (d + 3) & ~(uintptr_t)3; // first whole word
-    const uintptr_t D1 = e & ~(uintptr_t)3;       // end of the whole words
-    const uintptr_t hend = D0 < e ? D0 : e;
-    if (gl < 3 && d + gl < hend) dst[gl] = src[gl];
-    const uintptr_t T0 = D1 > D0 ? D1 : D0;       // tail bytes [T0, e)
-    if (gl >= 3 && gl < 6) {
-        const uintptr_t b = T0 + (gl - 3);
-        if (b < e) dst[b - d] = src[b - d];
-    }
-    if (D1 <= D0) return;
-    const uint32_t nw = (uint32_t)((D1 - D0) >> 2);
-    const uintptr_t s0 = (uintptr_t)src + (D0 - d);
-    const uint32_t sh = (uint32_t)(s0 & 3);
-    const uint32_t *sw = (const uint32_t *)(s0 - sh);
-    uint32_t *dw = (uint32_t *)D0;
-    if (sh == 0) {
-        for (uint32_t k = gl; k < nw; k += 32) dw[k] = sw[k];
-    } else {
-        for (uint32_t k = gl; k < nw; k += 32) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
-    }
+    const uintptr_t ad = (uintptr_t)p;
+    const uint32_t sh = (uint32_t)(ad & 3) * 8;
+    const uint32_t *w = (const uint32_t *)(ad & ~(uintptr_t)3);
+    const u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(w);
+    const uint32_t w4 = sh ? w[4] : 0u;
+    uint4 o;
+    o.x = __builtin_amdgcn_alignbit(v.y, v.x, sh);
+    o.y = __builtin_amdgcn_alignbit(v.z, v.y, sh);
+    o.z = __builtin_amdgcn_alignbit(v.w, v.z, sh);
+    o.w = __builtin_amdgcn_alignbit(w4, v.w, sh);
+    return o;
 }
 
-// group_copy split into phases so that several records' loads are in flight
-// before any store: plan, then load (the half-wave's first 32 words and the
-// head/tail bytes), then store them, then rest (words from 32 on, long
-// records only).
-struct CopyPlan {
-    uint32_t *dw;
-    const uint32_t *sw;
-    uint8_t *bd;
-    const uint8_t *bs;
-    uint32_t nw, sh, gl, wv, bv;
-    bool has_w, has_b;
-
-    __device__ __forceinline__ void plan(uint8_t *dst, const uint8_t *src, uint32_t len, uint32_t lane_in_half)
-    {
-        gl = lane_in_half;
-        const uintptr_t d = (uintptr_t)dst, e = d + len;
-        const uintptr_t D0 = (d + 3) & ~(uintptr_t)3, D1 = e & ~(uintptr_t)3;
-        const uintptr_t hend = D0 < e ? D0 : e, T0 = D1 > D0 ? D1 : D0;
-        // one byte for lanes 0-2 (head) or 3-5 (tail)
-        uintptr_t b = gl < 3 ? d + gl : T0 + (gl - 3);
-        has_b = len && ((gl < 3 && b < hend) || (gl >= 3 && gl < 6 && b < e));
-        bd = dst + (b - d);
-        bs = src + (b - d);
-        nw = len && D1 > D0 ? (uint32_t)((D1 - D0) >> 2) : 0;
-        const uintptr_t s0 = (uintptr_t)src + (D0 - d);
-        sh = (uint32_t)(s0 & 3);
-        sw = (const uint32_t *)(s0 - sh);
-        dw = (uint32_t *)D0;
-        has_w = gl < nw;
-    }
-    __device__ __forceinline__ void load()
-    {
-        if (has_w) wv = sh ? __builtin_amdgcn_alignbyte(sw[gl + 1], sw[gl], sh) : sw[gl];
-        if (has_b) bv = *bs;
-    }
-    __device__ __forceinline__ void store()
-    {
-        if (has_w) dw[gl] = wv;
-        if (has_b) *bd = (uint8_t)bv;
-    }
-    __device__ __forceinline__ void rest()
-    {
-        if (nw <= 32) return;
-        if (sh == 0) {
-            for (uint32_t k = gl + 32; k < nw; k += 32) dw[k] = sw[k];
-        } else {
-            for (uint32_t k = gl + 32; k < nw; k += 32) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
-        }
-    }
-};
+// Bytes j = at .. at + 3 of a chunk: from a where j < k, else from b.
+__device__ __forceinline__ uint32_t merge4(uint32_t a, uint32_t b, uint32_t k, uint32_t at)
+{
+    const uint32_t n = k > at ? k - at : 0; // bytes of this word from a
+    const uint32_t m = n >= 4 ? ~0u : (1u << (8 * n)) - 1;
+    return (a & m) | (b & ~m);
+}
 
 __device__ __forceinline__ void store_entry(AuxEntry *at, const AuxEntry &e)
 {
@@ -382,6 +328,8 @@ __device__ __forceinline__ void store_entry(AuxEntry *at, const AuxEntry &e)
 __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
 {
     __shared__ uint64_t wsum[3][4];
+    __shared__ uint64_t s_rel[SOA_TILE + 1]; // each record's sequence offset in the tile's stretch, and its end
+    __shared__ uint64_t s_src[SOA_TILE];     // each record's packed sequence
     const uint64_t n_all = a.n_dev ? *a.n_dev : a.n;
     if (n_all > a.cap_rec) return;
     const uint64_t n = n_all;
@@ -475,27 +423,63 @@ __global__ __launch_bounds__(256) void soa_fill_kernel(SoaArgs a)
             }
             seq_src = (uint64_t)(uintptr_t)(r + a.seq_start[i]);
         }
-        // packed sequences: each half-wave copies every other record of its
-        // wave, consecutive lanes on consecutive words, four records per step
-        // with every load issued before the stores
-        const uint32_t len = live && fits ? sb : 0;
-        for (uint32_t it = 0; it < 32; it += 4) {
-            CopyPlan c[4];
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const int src_lane = (int)(2 * (it + u) + (lane >> 5));
-                const uint32_t L = __shfl(len, src_lane, 64);
-                const uint64_t sp = __shfl(seq_src, src_lane, 64);
-                const uint64_t dp = __shfl(o1, src_lane, 64);
-                c[u].plan(a.seq + dp, (const uint8_t *)(uintptr_t)sp, L, lane & 31);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) c[u].load();
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) c[u].store();
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) c[u].rest();
+        // Packed sequences: the tile's records' sequences are one stretch
+        // [T0, T1) of the column. The workgroup writes it in 16-byte chunks
+        // aligned in memory (coalesced stores), each lane finding its chunk's
+        // record by a binary search over the tile's offsets in LDS: a chunk
+        // inside one record is one unaligned 16-byte load, a chunk across one
+        // record boundary two loads merged by byte, anything else (the tile's
+        // ends, records shorter than a chunk) byte by byte.
+        if (live) {
+            s_rel[threadIdx.x] = o1 - a.tile_base[3 * t + 1];
+            s_src[threadIdx.x] = seq_src;
         }
+        const uint32_t nt = n - t * SOA_TILE < SOA_TILE ? (uint32_t)(n - t * SOA_TILE) : SOA_TILE;
+        if (live && threadIdx.x == nt - 1) s_rel[nt] = o1 + sb - a.tile_base[3 * t + 1];
+        __syncthreads();
+        if (fits) {
+            const uint64_t T0 = a.tile_base[3 * t + 1], T1 = T0 + s_rel[nt];
+            const uint32_t mis = (uint32_t)((uintptr_t)a.seq & 15);
+            // chunk g holds column bytes [16 g - mis, 16 g - mis + 16)
+            const uint64_t g0 = (T0 + mis) / 16, g1 = (T1 + mis + 15) / 16;
+            for (uint64_t g = g0 + threadIdx.x; g < g1; g += SOA_TILE) {
+                const uint64_t c0 = 16 * g - mis; // wraps for the column's first chunk: clamped below
+                const uint64_t lo = g == 0 || c0 < T0 ? T0 : c0;
+                const uint64_t hi = c0 + 16 < T1 ? c0 + 16 : T1;
+                const uint64_t xlo = lo - T0, xhi = hi - T0;
+                if (xlo >= xhi) continue; // a tile whose records hold no sequence
+                // r: the last record whose sequence starts at or before xlo
+                uint32_t lo_i = 0, hi_i = nt; // s_rel[lo_i] <= xlo < s_rel[hi_i]
+                while (hi_i - lo_i > 1) {
+                    const uint32_t mid = (lo_i + hi_i) / 2;
+                    if (s_rel[mid] <= xlo) lo_i = mid;
+                    else hi_i = mid;
+                }
+                uint32_t r = lo_i;
+                while (s_rel[r + 1] <= xlo) ++r; // past zero-length records
+                const bool full = g != 0 && c0 >= T0 && hi == c0 + 16;
+                const uint8_t *src_r = (const uint8_t *)(uintptr_t)s_src[r] + (xlo - s_rel[r]);
+                if (full && s_rel[r + 1] >= xhi) {
+                    *reinterpret_cast<uint4 *>(a.seq + lo) = ld16u(src_r);
+                } else if (full && r + 1 < nt && s_rel[r + 1] < xhi && s_rel[r + 2] >= xhi) {
+                    const uint32_t k = (uint32_t)(s_rel[r + 1] - xlo); // bytes of record r, 1..15
+                    const uint4 v1 = ld16u(src_r);
+                    const uint4 v2 = ld16u((const uint8_t *)(uintptr_t)s_src[r + 1] - k);
+                    uint4 v;
+                    v.x = merge4(v1.x, v2.x, k, 0);
+                    v.y = merge4(v1.y, v2.y, k, 4);
+                    v.z = merge4(v1.z, v2.z, k, 8);
+                    v.w = merge4(v1.w, v2.w, k, 12);
+                    *reinterpret_cast<uint4 *>(a.seq + lo) = v;
+                } else {
+                    for (uint64_t x = xlo; x < xhi; ++x) {
+                        while (s_rel[r + 1] <= x) ++r;
+                        a.seq[T0 + x] = ((const uint8_t *)(uintptr_t)s_src[r])[x - s_rel[r]];
+                    }
+                }
+            }
+        }
+        __syncthreads(); // before the next tile's offsets replace these
     }
 }
 
