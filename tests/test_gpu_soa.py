@@ -78,6 +78,23 @@ def test_long_cigar_placeholder_and_errors(codec, decoder, oracle_mod):
     assert ei.value.position == int(recs.column_to_host("record_offsets")[7])
 
 
+def test_sequences_of_every_short_length(codec, decoder, oracle_mod):
+    """Packed sequences copied in 16-byte chunks of the column: records with no
+    sequence (a whole 256-record tile of them), sequences shorter than a chunk,
+    chunks across one and several record boundaries, and long sequences, in
+    random order, against the oracle."""
+    print("[gpu] soa short and empty sequences", flush=True)
+    from helpers import random_records
+    rng = np.random.default_rng(20261204)
+    recs = random_records(rng, 300, min_len=0, max_len=0)
+    recs += random_records(rng, 2000, min_len=0, max_len=40, name_prefix=b"s")
+    recs += random_records(rng, 200, min_len=200, max_len=3000, name_prefix=b"l")
+    order = rng.permutation(len(recs) - 300) + 300
+    batch, start = _batch(codec, recs[:300] + [recs[i] for i in order])
+    got = check(oracle_mod, decoder, batch, start)
+    assert len(got["cigar_offsets"]) == len(recs) + 1
+
+
 def test_empty_batch(codec, decoder):
     import fritillaria_amd as fa
     recs = decoder.decode(fa.DeviceInflateBatch(), 0)
