@@ -527,6 +527,132 @@ def write_leg(fa, ctx, np, steps, n_records=5_000_000):
                                                        f"({cpu_bytes} bytes), {cdt:.2f} s"}}
 
 
+def regions_leg(fa, ctx, np, steps, n_records=1_000_000, n_regions=512, batch_sizes=(64, 512)):
+    """Region queries through a BAI: a coordinate-sorted BAM of n_records short
+    reads (150 bp on a 1 Mbp reference, level 6, noodles framing) resident in
+    HBM, and regions of 1 bp to 50 kb at random starts, served batch by batch
+    by the query server (each region mapped to merged chunks through the
+    index, the chunks run as range queries of one batch, the records that
+    overlap their region kept by a device pass), two batches in flight on the
+    server's two slots, at 64 and at 512 regions per batch. The reference ships an index
+    only for its small fixture, so the BAI is built here by the oracle's
+    indexer (the one that rebuilds samtools' index of that fixture exactly).
+    Parity: every region's kept records (virtual offsets) equal the oracle's
+    index query filtered by its overlap rule. CPU baseline: the same queries
+    answered the way noodles' IndexedReader does it -- the index's chunks,
+    their members inflated with zlib, the records scanned and filtered -- one
+    thread, over a bounded sample of the regions."""
+    import oracle
+    from oracle import index as oi
+    from fritillaria_amd import synth
+    t0 = time.time()
+    s = synth.config_bam(1, n_records=n_records)
+    data = s.compressed
+    d = oracle.discover(data)
+    inf = oracle.inflate_batch(data, d.spans)
+    require(inf.kind == 0, "regions: the oracle could not inflate the file")
+    u = np.frombuffer(inf.data, np.uint8)
+    rc, start, _, _ = oracle.bam_parse_header(u)
+    rc2, rec, tail, _ = oracle.bam_scan_blocked(u, inf.offsets, start)
+    require(rc == 0 and rc2 == 0 and tail == u.size, "regions: the oracle could not scan the file")
+    rec = np.asarray(rec, dtype=np.int64)
+    ends = rec + 4 + u[rec[:, None] + np.arange(4)].copy().view("<u4").reshape(-1).astype(np.int64)
+    co = d.spans["compressed_offset"].astype(np.uint64)
+    vo = oracle.virtual_offsets(inf.offsets, co, rec)
+    vo_end = oracle.virtual_offsets(inf.offsets, co, ends)
+    fields = oi.record_fields(u, rec)
+    bai = oi.write_bai(oi.build_index(oi.alignment_contexts(fields, vo, vo_end), n_ref=1))
+    del inf, u
+    log(f"regions: {n_records} sorted records ({data.size} compressed bytes) and their BAI ({len(bai)} bytes) "
+        f"in {time.time() - t0:.1f}s")
+    ix, oix = fa.BinningIndex(bai), oi.read_index(bai)
+    server = fa.QueryServer(ctx, data)
+    rng = np.random.default_rng(0x5EED0006)
+    starts = rng.integers(1, 1_000_000, n_regions).tolist()
+    widths = rng.choice([1, 150, 2000, 50_000], n_regions).tolist()
+    regions = [(0, int(a), int(a + w)) for a, w in zip(starts, widths)]
+    by_batch = {}
+    for batch in batch_sizes:
+        batches = [regions[i:i + batch] for i in range(0, n_regions, batch)]
+        kept = scanned = inflated = 0
+        want_res = []
+        for regs in batches:
+            res = server.run_regions(ix, regs)
+            sel = server.selection_to_host(0)
+            col_vo = server.slot_column_to_host(0, "virtual_offsets")
+            for k, reg in enumerate(regs):
+                want, bad = oi.region_records(oix, fields, vo, *reg)
+                rows = sel[int(res[k]["first"]):int(res[k]["first"] + res[k]["records"])]
+                require(int(res[k]["status"]) == 0 and not bad and np.array_equal(col_vo[rows], vo[want]),
+                        f"regions: region {reg} differs from the oracle's index query")
+                kept += len(want)
+            scanned += int(res["scanned"].sum())
+            inflated += int(res["inflated_bytes"].sum())
+            want_res.append(res)
+        ctx.synchronize()
+        lat = []
+        t1 = time.perf_counter()
+        for step in range(steps):
+            pending = None
+            for i, regs in enumerate(batches + [None]):
+                if regs is not None:
+                    tb = time.perf_counter()
+                    server.launch_regions(i % 2, ix, regs)
+                if pending is not None:
+                    res = server.finish_regions(pending[0])
+                    lat.append((time.perf_counter() - pending[1]) * 1e3)
+                    require(step > 0 or np.array_equal(res, want_res[i - 1]),
+                            f"regions: batch {i - 1} of {batch} differs with two batches in flight")
+                pending = (i % 2, tb)
+        dt = (time.perf_counter() - t1) / steps
+        lat = np.array(lat)
+        log(f"regions: {n_regions} regions in batches of {batch}, two in flight: {dt * 1e3:.2f} ms per pass "
+            f"({n_regions / dt:.0f} regions/s, {kept / dt / 1e6:.1f} M kept records/s, {inflated / dt / 1e9:.2f} GB/s "
+            f"inflated); batch p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms")
+        by_batch[str(batch)] = {
+            "ms_per_pass": round(dt * 1e3, 3), "regions_per_s": round(n_regions / dt, 1),
+            "kept_records": kept, "scanned_records": scanned, "inflated_bytes": inflated,
+            "kept_records_per_s": round(kept / dt, 1), "inflated_gbps": round(inflated / dt / 1e9, 3),
+            "batch_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "batch_p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    server.close()
+    # CPU: the same queries through the index, members inflated with zlib
+    spans = d.spans
+    sample = regions[:128]
+    t1 = time.perf_counter()
+    cpu_kept = 0
+    for reg in sample:
+        for a, b in oi.query(oix, *reg):
+            m0 = int(np.searchsorted(co, np.uint64(a >> 16)))
+            # through the member after the end's block: a record starting before the end may run on into it
+            m1 = min(int(np.searchsorted(co, np.uint64(b >> 16), side="right")) + 1, len(spans))
+            part = oracle.inflate_batch(data, spans[m0:m1])
+            pu = np.frombuffer(part.data, np.uint8)
+            _, offs, _ = oracle.bam_scan_records(pu, a & 0xFFFF)
+            offs = offs.astype(np.int64)
+            mem = np.searchsorted(part.offsets[1:].astype(np.int64), offs, side="right")
+            rvo = (co[m0 + mem] << np.uint64(16)) | (offs - part.offsets[mem].astype(np.int64)).astype(np.uint64)
+            offs = offs[rvo < np.uint64(b)]
+            keep, _ = oi.intersects(oi.record_fields(pu, offs), *reg)
+            cpu_kept += int(keep.sum())
+    cdt = time.perf_counter() - t1
+    want_sample = sum(len(oi.region_records(oix, fields, vo, *reg)[0]) for reg in sample)
+    require(cpu_kept == want_sample, f"regions: the CPU path kept {cpu_kept} records, the oracle {want_sample}")
+    best = max(by_batch.values(), key=lambda r: r["regions_per_s"])
+    log(f"regions: CPU {len(sample) / cdt:.1f} regions/s")
+    return {"workload": f"region queries through a BAI over a coordinate-sorted BAM of {n_records:,} x 150 bp reads "
+                        f"(1 Mbp reference, level 6), {n_regions} regions of 1 bp-50 kb, batches of "
+                        f"{'/'.join(str(b) for b in batch_sizes)} with two in flight, device-resident",
+            "records": n_records, "compressed_bytes": int(data.size), "regions": n_regions,
+            "regions_per_s": best["regions_per_s"], "ms_per_pass": best["ms_per_pass"], "batches": by_batch,
+            "two_phase_min_members": ctx.two_phase_min_members(), "steps": steps,
+            "parity": {"every_region_equals_oracle_index_query": True, "same_results_two_in_flight": True,
+                       "cpu_path_kept_equal": True},
+            "cpu_baseline": {"value": round(len(sample) / cdt, 2), "unit": "regions/s", "cores": 1, "kind": "port",
+                             "sample": f"the first {len(sample)} regions: index chunks, their members inflated with "
+                                       f"zlib 1.2.11, records scanned and filtered, {cdt:.2f} s"}}
+
+
 def host_fed_leg(fa, ctx, np, compressed, steps, schedule, order, window):
     """The batched large BAM fed from host memory: the compressed file sits in
     pinned host memory and every step uploads it over PCIe before the pipeline
@@ -612,10 +738,14 @@ def leg_bench(args):
     elif args.workload == "text-fasta":
         leg = text_fasta_leg(fa, ctx, np, steps=steps)
         metric, value, ms = "text framing GB/s (SAM-like lines and tabs)", leg["text"]["gbps"], leg["text"]["ms_per_call"]
+    elif args.workload == "regions":
+        leg = regions_leg(fa, ctx, np, steps=steps)
+        metric, value, ms = "region queries per second through a BAI, batched", leg["regions_per_s"], leg["ms_per_pass"]
     else:
         leg = write_leg(fa, ctx, np, steps=steps)
         metric, value, ms = "BGZF write GB/s of input, device-resident", leg["gbps_in"], leg["ms_per_call"]
-    line = {"metric": metric, "value": value, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+    unit = "regions/s" if args.workload == "regions" else "GB/s"
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": 1,
             "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
             "data": "synthetic", "config": {"workload": leg["workload"]}, "leg": leg}
     print(json.dumps(line), flush=True)
@@ -641,14 +771,15 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (range query) measurement")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--workload", choices=["bam", "bcf-fastq", "plain-small", "long-read", "text-fasta", "write"],
+    ap.add_argument("--workload", choices=["bam", "bcf-fastq", "plain-small", "long-read", "text-fasta", "write",
+                                           "regions"],
                     default="bam",
                     help="bcf-fastq: the mixed BCF + bgzipped FASTQ configuration, decoded together per call; "
-                         "plain-small, long-read, text-fasta, write: that leg of the default run alone")
-    ap.add_argument("--configs", default="1,3,4,text,write",
+                         "plain-small, long-read, text-fasta, write, regions: that leg of the default run alone")
+    ap.add_argument("--configs", default="1,3,4,text,write,regions",
                     help="other configurations measured after the headline one: 1 plain small BAM per call, "
                          "3 long-read BAM (batched and per call), 4 mixed BCF + FASTQ, text: text framing and "
-                         "FASTA compaction, write: the device BGZF writer; 'none' to skip")
+                         "FASTA compaction, write: the device BGZF writer, regions: BAI region queries; 'none' to skip")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="after the bench, run one plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -828,6 +959,8 @@ def main():
         configs["text_fasta"] = text_fasta_leg(fa, ctx, np, steps=3)
     if "write" in want:
         configs["bgzf_write"] = write_leg(fa, ctx, np, steps=3)
+    if "regions" in want:
+        configs["regions"] = regions_leg(fa, ctx, np, steps=3)
 
     variant = fa.hip.inflate_schedules()[args.schedule] if args.schedule >= 0 else None
     two_phase = variant is not None and variant[3] == 3

@@ -81,7 +81,7 @@ struct frhip_ctx {
     uint32_t *x2n = nullptr;
     uint32_t bam_window = 2048;
     int inflate_schedule = 16; // index into INFLATE_SCHEDULES (DEFAULT_INFLATE_SCHEDULE)
-    uint32_t two_phase_min = 8192; // smaller batches of a two-phase schedule run the one-wave form
+    uint32_t two_phase_min = 4096; // smaller batches of a two-phase schedule run the one-wave form (set per device)
     frhip::Scratch in_scratch, meta_scratch, status_scratch, bam_scratch, bcf_scratch, soa_scratch, fq_small, fq_big,
         rec_scratch, tx_scratch, wr_scratch;
     frhip::PinnedScratch pinned;

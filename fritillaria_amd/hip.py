@@ -165,6 +165,7 @@ def lib():
         "frhip_ctx_synchronize": (C.c_int, [_vp, _ep]),
         "frhip_ctx_set_inflate_schedule": (C.c_int, [_vp, C.c_int]),
         "frhip_ctx_set_two_phase_min_members": (C.c_int, [_vp, C.c_uint32]),
+        "frhip_ctx_two_phase_min_members": (C.c_uint32, [_vp]),
         "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
         "frhip_codec_name": (C.c_char_p, []),
         "frhip_device_codec_name": (C.c_char_p, []),
@@ -375,8 +376,12 @@ class HipContext:
             raise ValueError(f"no inflate schedule {schedule}")
 
     def set_two_phase_min_members(self, n: int) -> None:
-        """Batches of fewer members than n run a two-phase schedule's one-wave form (default 8192)."""
+        """Batches of fewer members than n run a two-phase schedule's one-wave form
+        (default: the one-wave form's resident waves, 4096 on a 256-CU device)."""
         lib().frhip_ctx_set_two_phase_min_members(self._h, int(n))
+
+    def two_phase_min_members(self) -> int:
+        return int(lib().frhip_ctx_two_phase_min_members(self._h))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         """A device buffer of nbytes (contents undefined)."""

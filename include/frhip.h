@@ -117,9 +117,12 @@ int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
 /* With a two-phase schedule (token form 3: one member per lane, then one wave
    per member), batches of fewer than min_members members run the one-wave
    lane-speculative form of the same ring size instead, which finishes a small
-   batch sooner (default 8192). Results do not depend on it. No reference
+   batch sooner (default: the one-wave form's resident waves on the device,
+   n_cu x 16 = 4096 on MI355X). Results do not depend on it. No reference
    counterpart. */
 int frhip_ctx_set_two_phase_min_members(frhip_ctx *ctx, uint32_t min_members);
+/* The current value of that threshold. No reference counterpart. */
+uint32_t frhip_ctx_two_phase_min_members(const frhip_ctx *ctx);
 /* Writes up to cap entries of (ring bytes, waves per workgroup, resident waves
    per CU, token form: 0 symbols straight into the ring, 1 token lists resolved
    64 at a time, 2 lane-speculative token lists, 3 two-phase); returns the

@@ -230,6 +230,7 @@ def test_every_schedule_matches_oracle(hip_ctx, oracle_mod, schedule):
     print(f"[gpu] inflate schedule {schedule}", flush=True)
     import fritillaria_amd as fa
     hip_ctx.set_inflate_schedule(schedule)
+    default_min = hip_ctx.two_phase_min_members()
     hip_ctx.set_two_phase_min_members(0)  # two-phase schedules run as such on these small batches
     try:
         codec = fa.HipCodec(hip_ctx)
@@ -251,4 +252,4 @@ def test_every_schedule_matches_oracle(hip_ctx, oracle_mod, schedule):
             assert ei.value.code == want, name
     finally:
         hip_ctx.set_inflate_schedule(16)
-        hip_ctx.set_two_phase_min_members(8192)
+        hip_ctx.set_two_phase_min_members(default_min)

@@ -71,6 +71,7 @@ def test_synthetic_bcf_and_fastq_together(hip_ctx, codec, oracle_mod, two_phase)
 
 def run_and_compare(hip_ctx, codec, oracle_mod, b, f, two_phase):
     import fritillaria_amd as fa
+    default_min = hip_ctx.two_phase_min_members()
     if two_phase:
         hip_ctx.set_inflate_schedule(16)
         hip_ctx.set_two_phase_min_members(0)
@@ -78,7 +79,7 @@ def run_and_compare(hip_ctx, codec, oracle_mod, b, f, two_phase):
         r, gb, gf = run_pair(hip_ctx, oracle_mod, b.compressed.tobytes(), f.compressed.tobytes())
     finally:
         hip_ctx.set_inflate_schedule(16)
-        hip_ctx.set_two_phase_min_members(8192)
+        hip_ctx.set_two_phase_min_members(default_min)
     assert len(gb["record_offsets"]) == b.n_records and len(gf["record_offsets"]) == f.n_records
     bcf_alone = fa.BcfScanner(hip_ctx).decode(
         codec.inflate_batch_device(b.compressed, fa.discover(b.compressed)[0]), b.records_start, b.samples,

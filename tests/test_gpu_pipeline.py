@@ -194,13 +194,14 @@ def test_two_phase_pipeline_matches_oracle(hip_ctx, oracle_mod, schedule, order)
     print(f"[gpu] two-phase pipeline schedule {schedule} order {order}", flush=True)
     import fritillaria_amd as fa
     from fritillaria_amd import synth
+    default_min = hip_ctx.two_phase_min_members()
     hip_ctx.set_two_phase_min_members(0)  # captured into the graph at creation
     try:
         s = synth.config_bam(2, n_records=60000)
         pipe = fa.BatchedPipeline(hip_ctx, s.compressed, inflate_schedule=schedule, order=order, use_graph=True,
                                   soa=True)
     finally:
-        hip_ctx.set_two_phase_min_members(8192)
+        hip_ctx.set_two_phase_min_members(default_min)
     for rep in range(3):
         pipe.run()
         check_pipeline(pipe, oracle_mod, s.compressed, check_bytes=(rep == 0), soa=True)
