@@ -117,6 +117,39 @@ def test_deterministic_and_mixed_content(hip_ctx, compressor, codec):
     assert len(a) < len(data) * 0.7
 
 
+def test_matches_at_the_window_edge(hip_ctx, compressor, codec):
+    """Repeats at exactly 32,768 bytes are matched (the largest distance
+    DEFLATE allows, RFC 1951 3.2.5; zlib's own compressor stops 262 bytes
+    short of it), repeats one byte farther are not: zlib's decoder rejects a
+    distance past the window, so any such match fails the check."""
+    print("[gpu] write: matches at the window edge", flush=True)
+    rng = np.random.default_rng(8)
+    n = 65505
+    parts = []
+    for period in (32768, 32769, 32767, 32768 - 258, 40000):
+        unit = rng.integers(0, 256, period, dtype=np.uint8).tobytes()
+        parts.append((unit * 2)[:n])
+    # an 8,000-byte repeat at distance 32,768 inside otherwise random bytes
+    r = bytearray(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+    r[40000:48000] = r[40000 - 32768:48000 - 32768]
+    parts.append(bytes(r))
+    data = b"".join(parts)
+    bounds = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    out = compressor.compress_batch_device(device_bytes(hip_ctx, data), bounds)
+    check_stream(out.data(), data, bounds, codec)
+    size = [len(out.block(i)) for i in range(len(parts))]
+    # periods within the window: the second copy is matches
+    for k in (0, 2, 3):
+        assert out.framing[k] == ow.DEFLATED and size[k] < 0.56 * n, (k, size[k])
+    # periods past it: nothing to match, stored
+    for k in (1, 4):
+        assert out.framing[k] == ow.STORED, (k, size[k])
+    # random bytes cost more than stored as Huffman literals (a dynamic
+    # block every 4,096 symbols): deflated, and smaller, only when the repeat
+    # at distance 32,768 was matched
+    assert out.framing[5] == ow.DEFLATED and size[5] < n - 6000, size[5]
+
+
 def test_writer_contract(hip_ctx, codec):
     """crates/fritillaria-bgzf/tests/device_write.rs: boundaries survive, the
     output does not depend on the batch size, the EOF block ends every file,
