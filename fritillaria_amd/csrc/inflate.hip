@@ -1169,6 +1169,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
 // the ring is written back with the CRC folded in.
 
 constexpr uint32_t REC_SKIP = 511;
+// Decode steps per pass of the lanes kernel. On the 20M-record BAM: 3, 5 and
+// 6 within 0.3 ms of 4 (36.0 ms); 8 slower (37.9 ms: lanes that use up their
+// staged input mid-pass skip the rest of it).
+constexpr uint32_t LSTEPS = 4;
 constexpr uint32_t LQ = 14; // packed limits for code lengths 1..14
 
 // First record slot of member m. Each record covers at least 3 output bytes,
@@ -1486,10 +1490,10 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     uint8_t *const out_w = out - h; // word-aligned: byte r of the windows is out_w[r]
     uint32_t lo = h;                 // first window byte a word store may cover
     uint32_t wcur = NONE, wv = 0;    // the open window and its bytes
-    uint32_t ls_i[4], ls_v[4];       // windows closed in step j of the pass
-    uint32_t rs_i[4], rs_v[4];       // record written in step j of the pass
+    uint32_t ls_i[LSTEPS], ls_v[LSTEPS]; // windows closed in step j of the pass
+    uint32_t rs_i[LSTEPS], rs_v[LSTEPS]; // record written in step j of the pass
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
+    for (uint32_t j = 0; j < LSTEPS; ++j) {
         ls_i[j] = NONE;
         ls_v[j] = 0;
         rs_i[j] = NONE;
@@ -1508,7 +1512,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
     };
     auto flush_pass = [&]() __attribute__((always_inline)) {
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
+        for (uint32_t j = 0; j < LSTEPS; ++j) {
             if (ls_i[j] != NONE) put_window(ls_i[j], ls_v[j], NONE);
             if (rs_i[j] != NONE) rec[rs_i[j]] = rs_v[j];
             ls_i[j] = NONE;
@@ -1761,7 +1765,7 @@ __global__ __launch_bounds__(64 * NW) void fr_inflate_lanes_kernel(InflateArgs a
             }
         }
 #pragma unroll
-        for (uint32_t sub = 0; sub < 4; ++sub) {
+        for (uint32_t sub = 0; sub < LSTEPS; ++sub) {
             LB_REFILL_SEL(b); // harmless for a lane at a header
             // a symbol needs up to 20 bits, then (a match) a refill and 28
             // (a second literal 15); a lane without them waits for the next
