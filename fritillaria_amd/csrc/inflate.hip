@@ -822,6 +822,54 @@ __device__ void inflate_member(WaveMem<RING> &sm, const uint32_t *crctab, const 
                 const uint32_t ev = sm.plut[fl & ((1u << PBITS) - 1)];
                 const uint32_t dv = sm.dlut[fl & ((1u << DBITS) - 1)];
                 uint32_t p = 0;
+                // Fast hops while every code this round can read (a symbol
+                // starting below bit 64 takes at most 48 bits with its
+                // distance) ends before the payload does, so no truncation
+                // check applies: a symbol is taken here only when its codes
+                // are in the tables and every check passes. Anything else
+                // (a long code, the block's end, an error) leaves the symbol
+                // to the general loop below, which decodes it again in full.
+                if (B + 128 <= total) {
+                    while (p < 64 && ntok < 62 && o.pos - chunk_pos < G::TOK_LIMIT) {
+                        const uint32_t e = uni(__builtin_amdgcn_readlane(ev, p));
+                        const uint32_t L = e & 15, sym = (e >> 4) & 511;
+                        if (L == 0) break;
+                        if (sym < 256) {
+                            if ((e & PAIR) && o.pos + 2 <= o.cap) {
+                                tokv = put_lane((e >> 4) & 255, ntok, tokv);
+                                tokv = put_lane((e >> 18) & 255, ntok + 1, tokv);
+                                ntok += 2;
+                                o.pos += 2;
+                                p += (e >> 14) & 15;
+                                continue;
+                            }
+                            if (o.pos >= o.cap) break;
+                            tokv = put_lane(sym, ntok, tokv);
+                            ++ntok;
+                            ++o.pos;
+                            p += L;
+                            continue;
+                        }
+                        const uint32_t lc = sym - 257;
+                        if (lc >= 29) break;
+                        const uint32_t f = uni(__builtin_amdgcn_readlane(fl, p));
+                        const uint32_t lx = len_extra(lc);
+                        const uint32_t len = len_base(lc) + ((f >> L) & ((1u << lx) - 1));
+                        const uint32_t q = p + L + lx;
+                        if (q >= 64) break;
+                        const uint32_t ed = uni(__builtin_amdgcn_readlane(dv, q));
+                        const uint32_t Ld = ed & 15, ds = ed >> 4;
+                        if (Ld == 0 || ds >= 30) break;
+                        const uint32_t g = uni(__builtin_amdgcn_readlane(fl, q));
+                        const uint32_t dx = dist_extra(ds);
+                        const uint32_t dist = dist_base(ds) + ((g >> Ld) & ((1u << dx) - 1));
+                        if (dist > o.pos || o.pos + len > o.cap) break;
+                        tokv = put_lane(TOK_MATCH | (len << 16) | dist, ntok, tokv);
+                        ++ntok;
+                        o.pos += len;
+                        p = q + Ld + dx;
+                    }
+                }
                 while (p < 64 && ntok < 62 && o.pos - chunk_pos < G::TOK_LIMIT) {
                     const uint32_t e = uni(__builtin_amdgcn_readlane(ev, p));
                     const uint32_t f = uni(__builtin_amdgcn_readlane(fl, p));
