@@ -137,11 +137,24 @@ def test_tampered_trailers(codec, oracle_mod):
     assert ei.value.code == 4
 
 
-def test_corrupted_payload_bytes_fail_like_the_oracle(codec, oracle_mod):
+@pytest.mark.parametrize("form", ["one-wave", "two-phase"])
+def test_corrupted_payload_bytes_fail_like_the_oracle(hip_ctx, codec, oracle_mod, form):
+    """One flipped bit anywhere in a 60 KB member's payload (mostly far from
+    its end, where the decoders take their fast paths): the same bytes, or the
+    same error, as the oracle; in the one-wave form and in the two-phase form."""
     import fritillaria_amd as fa
+    default_min = hip_ctx.two_phase_min_members()
+    hip_ctx.set_two_phase_min_members(1 << 30 if form == "one-wave" else 0)
+    try:
+        corrupted_trials(codec, oracle_mod, fa)
+    finally:
+        hip_ctx.set_two_phase_min_members(default_min)
+
+
+def corrupted_trials(codec, oracle_mod, fa):
     rng = np.random.default_rng(77)
     data = b"".join(random_records(rng, 200))[:60000]
-    for trial in range(24):
+    for trial in range(96):
         cd = bytearray(deflate_raw(data, int(rng.integers(1, 10))))
         k = int(rng.integers(0, len(cd)))
         cd[k] ^= 1 << int(rng.integers(0, 8))
