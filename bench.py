@@ -1013,7 +1013,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": traffic,
                      "kernel": inflate_kernel, "algorithmic_bytes_per_launch": alg_bytes,
-                     "launch_ms": round(inflate_ms, 3)},
+                     "launch_ms": round(inflate_ms, 3), "compute_units": ctx.compute_units(),
+                     "members_per_cu_ms": round(n_members / ctx.compute_units() / inflate_ms, 3)},
         "pipeline_roofline": {"algorithmic_bytes": pipe_bytes,
                               "achieved_gbps": round(pipe_bytes / (ms_per_step / 1e3) / 1e9, 2),
                               "frac": round(pipe_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBPS, 5)},

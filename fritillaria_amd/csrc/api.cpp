@@ -314,6 +314,7 @@ extern "C" int frhip_ctx_set_two_phase_min_members(frhip_ctx *c, uint32_t min_me
     return FRHIP_OK;
 }
 extern "C" uint32_t frhip_ctx_two_phase_min_members(const frhip_ctx *c) { return c->two_phase_min; }
+extern "C" uint32_t frhip_ctx_compute_units(const frhip_ctx *c) { return (uint32_t)c->n_cu; }
 extern "C" int frhip_inflate_schedules(uint32_t *ring, uint32_t *waves_per_group, uint32_t *waves_per_cu,
                                        uint32_t *tokens, int cap)
 {

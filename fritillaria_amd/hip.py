@@ -166,6 +166,7 @@ def lib():
         "frhip_ctx_set_inflate_schedule": (C.c_int, [_vp, C.c_int]),
         "frhip_ctx_set_two_phase_min_members": (C.c_int, [_vp, C.c_uint32]),
         "frhip_ctx_two_phase_min_members": (C.c_uint32, [_vp]),
+        "frhip_ctx_compute_units": (C.c_uint32, [_vp]),
         "frhip_inflate_schedules": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
         "frhip_codec_name": (C.c_char_p, []),
         "frhip_device_codec_name": (C.c_char_p, []),
@@ -382,6 +383,9 @@ class HipContext:
 
     def two_phase_min_members(self) -> int:
         return int(lib().frhip_ctx_two_phase_min_members(self._h))
+
+    def compute_units(self) -> int:
+        return int(lib().frhip_ctx_compute_units(self._h))
 
     def alloc(self, nbytes: int) -> "DeviceBuffer":
         """A device buffer of nbytes (contents undefined)."""

@@ -123,6 +123,9 @@ int frhip_ctx_set_inflate_schedule(frhip_ctx *ctx, int schedule);
 int frhip_ctx_set_two_phase_min_members(frhip_ctx *ctx, uint32_t min_members);
 /* The current value of that threshold. No reference counterpart. */
 uint32_t frhip_ctx_two_phase_min_members(const frhip_ctx *ctx);
+/* Compute units of the context's device (per-CU throughput figures). No
+   reference counterpart. */
+uint32_t frhip_ctx_compute_units(const frhip_ctx *ctx);
 /* Writes up to cap entries of (ring bytes, waves per workgroup, resident waves
    per CU, token form: 0 symbols straight into the ring, 1 token lists resolved
    64 at a time, 2 lane-speculative token lists, 3 two-phase); returns the
