@@ -104,14 +104,17 @@ def test_a_bad_cigar_op_fails_its_region_only(hip_ctx, oracle_mod):
     server.close()
 
 
-def test_sorted_200k_records(hip_ctx, oracle_mod):
-    print("[gpu] regions: 200,000 sorted records, BAI", flush=True)
+@pytest.mark.parametrize("kind,min_shift,depth", [("bai", 14, 5), ("csi", 14, 5), ("csi", 12, 6)])
+def test_sorted_200k_records(hip_ctx, oracle_mod, kind, min_shift, depth):
+    print(f"[gpu] regions: 200,000 sorted records, {kind} min_shift {min_shift} depth {depth}", flush=True)
     import fritillaria_amd as fa
     from fritillaria_amd import synth
     s = synth.config_bam(1, n_records=200_000)
     u, rec, vo, vo_end, f = decode_file(oracle_mod, s.compressed)
-    raw = oi.write_bai(oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=1))
+    built = oi.build_index(oi.alignment_contexts(f, vo, vo_end), n_ref=1, min_shift=min_shift, depth=depth, kind=kind)
+    raw = oi.write_bai(built) if kind == "bai" else oi.write_csi(built)
     ix, oix = fa.BinningIndex(raw), oi.read_index(raw)
+    assert ix.format == kind and ix.binning == (min_shift, depth)
     server = fa.QueryServer(hip_ctx, s.compressed)
     rng = np.random.default_rng(47)
     regions = [(0, a, a + w) for a, w in zip(rng.integers(1, 1_000_000, 400).tolist(),
