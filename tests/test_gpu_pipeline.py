@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import golden_bytes
-from helpers import bam_header, bgzf, bgzf_member, random_records
+from helpers import bam_header, bgzf, bgzf_member, build_record, random_records
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +75,28 @@ def test_pipeline_matches_oracle(hip_ctx, oracle_mod, config, n, graph):
     if not graph:
         ms = pipe.stage_ms()
         assert ms["total"] > 0 and ms["inflate"] > 0
+
+
+def small_files():
+    recs = [build_record(b"r%d" % i, b"ACGTN" * (i + 1), pos=10 * i, cigar=[((5 * (i + 1)) << 4) | 0]) for i in range(3)]
+    head = bam_header()
+    return [("header only", bgzf(head)),
+            ("header only, no EOF member", bgzf(head, eof=False)),
+            ("three records in one member, no EOF member", bgzf(head + b"".join(recs), eof=False)),
+            ("one record, header and record in separate members", bgzf_member(head) + bgzf(recs[0]))]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_pipeline_small_files(hip_ctx, oracle_mod, graph):
+    """Files with no records, a single member, or no EOF member, against the oracle."""
+    import fritillaria_amd as fa
+    for name, data in small_files():
+        print(f"[gpu] pipeline small file: {name}, graph={graph}", flush=True)
+        pipe = fa.BatchedPipeline(hip_ctx, np.frombuffer(data, np.uint8), use_graph=graph, soa=True)
+        for rep in range(2):
+            pipe.run()
+            check_pipeline(pipe, oracle_mod, data, check_bytes=True, soa=True)
+        pipe.close()
 
 
 @pytest.mark.parametrize("name", ["htslib.bam", "htslib_multiblock.bam", "ont_ultralong.bam", "pacbio_hifi.bam"])
