@@ -38,7 +38,11 @@ typedef struct {
 
 enum { FRHIP_INDEX_BAI = 1, FRHIP_INDEX_CSI = 2 };
 
-/* Reads a BAI ("BAI\1") or CSI ("CSI\1") index held in memory. */
+/* Reads a BAI ("BAI\1") or CSI ("CSI\1") index held in memory. One
+   deviation: the reference reads a CSI of any min_shift and depth from 0 to
+   255, whose maximum position 2^(min_shift + 3 depth) - 1 then overflows its
+   64-bit arithmetic in a query; such an index (min_shift + 3 depth > 62, or
+   depth > 10) is refused here when read, as unsupported. */
 int frhip_index_read(const uint8_t *data, uint64_t len, frhip_index **out, frhip_error *err);
 void frhip_index_free(frhip_index *ix);
 /* FRHIP_INDEX_BAI or FRHIP_INDEX_CSI, with the binning parameters (BAI: 14, 5). */
