@@ -83,3 +83,68 @@ def test_no_device_means_unavailable_not_fallback():
         pass
     else:
         raise AssertionError("a context without a device must raise")
+
+
+def damaged(rng, good: bytes, n: int):
+    """n damaged copies: flipped bits, overwritten 16/32-bit fields, truncations, insertions."""
+    out = []
+    for _ in range(n):
+        b = bytearray(good)
+        kind = int(rng.integers(0, 4))
+        k = int(rng.integers(0, len(b)))
+        if kind == 0:
+            for _ in range(int(rng.integers(1, 4))):
+                j = int(rng.integers(0, len(b)))
+                b[j] ^= 1 << int(rng.integers(0, 8))
+        elif kind == 1:
+            w = int(rng.choice([2, 4]))
+            v = int(rng.integers(0, 1 << (8 * w)))
+            b[k:k + w] = v.to_bytes(w, "little")[:len(b[k:k + w])]
+        elif kind == 2:
+            b = b[:k]
+        else:
+            b[k:k] = rng.integers(0, 256, int(rng.integers(1, 30)), dtype=np.uint8).tobytes()
+        out.append(bytes(b))
+    return out
+
+
+def test_discover_agrees_with_the_oracle_on_damaged_streams(oracle_mod):
+    """Damaged BGZF streams (headers, BSIZE, XLEN, subfields, trailers, cuts):
+    the host walk finds the same members and stops at the same position as the
+    oracle, or refuses the stream at the same offset."""
+    import fritillaria_amd as fa
+    good = golden_bytes("htslib_multiblock.bam")[:200_000]
+    rng = np.random.default_rng(43)
+    refused = 0
+    for i, b in enumerate(damaged(rng, good, 1500)):
+        want = oracle_mod.discover(b)
+        try:
+            spans, pos = fa.discover(b)
+        except fa.InvalidBlock as e:
+            assert want.err != 0 and e.offset == want.err_offset, (i, want.err, want.err_offset, e.offset)
+            refused += 1
+            continue
+        assert want.err == 0, (i, want.err, want.err_offset)
+        assert pos == want.position and np.array_equal(spans, want.spans), i
+    assert refused > 100
+
+
+def test_bam_header_agrees_with_the_oracle_on_damaged_headers(oracle_mod):
+    """Damaged BAM headers (magic, l_text, n_ref, name lengths, cuts): the same
+    records start, or a refusal where the oracle refuses."""
+    import fritillaria_amd as fa
+    buf = golden_bytes("htslib_multiblock.bam")
+    d = oracle_mod.discover(buf)
+    u = oracle_mod.inflate_batch(buf, d.spans).data
+    _, rs, _, _ = oracle_mod.bam_parse_header(u)
+    good = bytes(u[:rs + 64])
+    rng = np.random.default_rng(47)
+    for i, b in enumerate(damaged(rng, good[:rs], 1500)):
+        b = b + good[rs:]
+        rc, start, _, nref = oracle_mod.bam_parse_header(np.frombuffer(b, np.uint8))
+        try:
+            got = fa.bam_parse_header(b)
+        except fa.Malformed:
+            assert rc != 0, i
+            continue
+        assert rc == 0 and got == (start, nref), (i, rc, got, start, nref)
