@@ -135,6 +135,39 @@ def test_malformed_record_position_matches_oracle(codec, decoder, oracle_mod):
     assert rc == 1 and ei.value.position == tail
 
 
+def test_corrupted_block_sizes_walk_like_the_oracle(codec, decoder, oracle_mod):
+    """A record's block_size overwritten with random values (below the core,
+    inside the stream so the chain walks on through other records' bytes, past
+    the end): the device finds the same records, or fails at the same
+    position, as the oracle's serial walk."""
+    import fritillaria_amd as fa
+    rng = np.random.default_rng(24)
+    recs = random_records(rng, 300)
+    failed = 0
+    for trial in range(48):
+        k = int(rng.integers(0, len(recs)))
+        bad = bytearray(recs[k])
+        v = int(rng.choice([int(rng.integers(0, 32)), int(rng.integers(32, 2000)), int(rng.integers(2000, 60000)),
+                            int(rng.integers(60000, 1 << 31))]))
+        bad[0:4] = v.to_bytes(4, "little")
+        data = b"".join(recs[:k]) + bytes(bad) + b"".join(recs[k + 1:])
+        buf = bgzf(data, cap=int(rng.choice([900, 7000, 65495])))
+        spans, _ = fa.discover(buf)
+        batch = codec.inflate_batch_device(buf, spans)
+        u = np.frombuffer(data, np.uint8)
+        rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, batch.offsets(), 0)
+        print(f"[gpu] block_size trial {trial}: record {k} -> {v}, oracle rc {rc}", flush=True)
+        if rc != 0:
+            failed += 1
+            with pytest.raises(fa.Malformed) as ei:
+                decoder.decode(batch, 0)
+            assert ei.value.position == tail and ei.value.format == "bam", trial
+        else:
+            records = decoder.decode(batch, 0)
+            check_batch(oracle_mod, records, u, batch.offsets(), batch.compressed_offsets(), 0)
+    assert 0 < failed < 48
+
+
 def test_empty_batch_decodes_to_no_records(codec, decoder):
     import fritillaria_amd as fa
     b = codec.inflate_batch_device(b"", np.zeros(0, dtype=fa.SPAN_DTYPE))
