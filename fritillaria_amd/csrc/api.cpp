@@ -11,6 +11,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -81,8 +83,9 @@ frhip_buf *buf_alloc(frhip_ctx *ctx, uint64_t len, frhip_error *err)
     b->ptr = nullptr;
     // zero-length allocations are padded to one byte with a logical length of 0
     // (crates/fritillaria-cuda/src/backend.rs:202-205,608-613)
-    hipError_t r = hipMalloc(&b->ptr, len ? len : 1);
-    if (r != hipSuccess) {
+    hipError_t r = hipSuccess;
+    b->ptr = dev_alloc(ctx->ordinal, len ? len : 1, &b->cap, &r);
+    if (!b->ptr) {
         delete b;
         hip_err(err, r, "hipMalloc");
         return nullptr;
@@ -97,16 +100,106 @@ void buf_free(frhip_buf *b)
     if (b->owner)
         owner_release(b->owner);
     else if (b->ptr)
-        (void)hipFree(b->ptr);
+        dev_free(b->ordinal, b->ptr, b->cap);
     delete b;
 }
 
 void owner_release(BufOwner *o)
 {
     if (o && o->refs.fetch_sub(1) == 1) {
-        if (o->ptr) (void)hipFree(o->ptr);
+        if (o->ptr) dev_free(o->ordinal, o->ptr, o->cap);
         delete o;
     }
+}
+
+namespace {
+// Blocks are sized in 64 KiB steps (4 KiB at least); a request takes the
+// smallest kept block that holds it and is at most twice its size. At most
+// BLOCK_KEEP_MAX bytes are kept per device (the largest go first beyond it).
+constexpr uint64_t BLOCK_KEEP_MAX = 8ull << 30;
+constexpr int KEEP_DEVICES = 64;
+struct BlockKeep {
+    std::mutex m;
+    std::multimap<uint64_t, void *> blocks[KEEP_DEVICES];
+    uint64_t bytes[KEEP_DEVICES] = {};
+};
+BlockKeep &block_keep()
+{
+    static BlockKeep k;
+    return k;
+}
+uint64_t block_bytes(uint64_t n) { return n <= 4096 ? 4096 : (n + 65535) & ~65535ull; }
+} // namespace
+
+void *dev_alloc(int ordinal, uint64_t n, uint64_t *cap, hipError_t *r)
+{
+    const uint64_t want = block_bytes(n);
+    *r = hipSuccess;
+    if (ordinal >= 0 && ordinal < KEEP_DEVICES) {
+        BlockKeep &k = block_keep();
+        std::lock_guard<std::mutex> g(k.m);
+        auto &bl = k.blocks[ordinal];
+        auto it = bl.lower_bound(want);
+        if (it != bl.end() && it->first <= 2 * want) {
+            void *p = it->second;
+            *cap = it->first;
+            k.bytes[ordinal] -= it->first;
+            bl.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    *r = hipMalloc(&p, want);
+    if (*r != hipSuccess) {
+        // the kept blocks may be what fills the device: release them, once
+        (void)hipGetLastError();
+        dev_flush(ordinal);
+        *r = hipMalloc(&p, want);
+    }
+    if (*r != hipSuccess) {
+        *cap = 0;
+        return nullptr;
+    }
+    *cap = want;
+    return p;
+}
+
+void dev_free(int ordinal, void *p, uint64_t cap)
+{
+    if (!p) return;
+    if (cap == 0 || ordinal < 0 || ordinal >= KEEP_DEVICES) {
+        (void)hipFree(p);
+        return;
+    }
+    std::vector<void *> release;
+    {
+        BlockKeep &k = block_keep();
+        std::lock_guard<std::mutex> g(k.m);
+        auto &bl = k.blocks[ordinal];
+        bl.emplace(cap, p);
+        k.bytes[ordinal] += cap;
+        while (k.bytes[ordinal] > BLOCK_KEEP_MAX && !bl.empty()) {
+            auto last = std::prev(bl.end());
+            release.push_back(last->second);
+            k.bytes[ordinal] -= last->first;
+            bl.erase(last);
+        }
+    }
+    for (void *q : release) (void)hipFree(q);
+}
+
+void dev_flush(int ordinal)
+{
+    if (ordinal < 0 || ordinal >= KEEP_DEVICES) return;
+    std::vector<void *> release;
+    {
+        BlockKeep &k = block_keep();
+        std::lock_guard<std::mutex> g(k.m);
+        for (auto &kv : k.blocks[ordinal]) release.push_back(kv.second);
+        k.blocks[ordinal].clear();
+        k.bytes[ordinal] = 0;
+    }
+    for (void *q : release) (void)hipFree(q);
 }
 
 EventRef *event_record(hipStream_t s)
@@ -273,6 +366,7 @@ void ctx_release(frhip_ctx *c)
     if (!c || __atomic_sub_fetch(&c->refs, 1, __ATOMIC_ACQ_REL) != 0) return;
     (void)hipSetDevice(c->ordinal);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dev_flush(c->ordinal);
     c->in_scratch.release();
     c->meta_scratch.release();
     c->status_scratch.release();
@@ -526,28 +620,30 @@ extern "C" int frhip_discover(const uint8_t *buf, uint64_t len, uint64_t base, f
 
 namespace frhip {
 
-// Plan + upload + launch + verify. On success *out owns the output.
-int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
-                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, frhip_inflate_timings *t,
-                   uint64_t align_at)
+// Plan + upload + launch on run's stream, without waiting. On success *out
+// holds the output (its ready event recorded after the kernel), out_off the
+// members' output offsets, and *h_status (in run's pinned buffer) receives
+// status, produced and CRC per member once the stream passes the last copy.
+int inflate_launch(frhip_ctx *ctx, const InflateRun &run, const uint8_t *batch, uint64_t batch_len,
+                   const frhip_block_span *spans, uint64_t n, frhip_buf **out, std::vector<uint64_t> &out_off,
+                   const uint32_t **h_status, frhip_error *err, frhip_inflate_timings *t, uint64_t align_at)
 {
     *out = nullptr;
+    *h_status = nullptr;
     (void)hipSetDevice(ctx->ordinal);
     double t0 = t ? now_ms() : 0;
     // ends a timed phase: wait for the stream, add the host time since t0
     auto phase = [&](double &acc) {
         if (!t) return hipSuccess;
-        hipError_t r = hipStreamSynchronize(ctx->stream);
+        hipError_t r = hipStreamSynchronize(run.stream);
         const double t1 = now_ms();
         acc += t1 - t0;
         t0 = t1;
         return r;
     };
     // BatchPlan::new (backend.rs:95-133)
-    std::vector<uint64_t> meta(3 * n + 1);
-    uint64_t *in_off = meta.data();
-    uint64_t *out_off = meta.data() + n;
-    uint32_t *lens = reinterpret_cast<uint32_t *>(meta.data() + 2 * n); // in_len[n], out_cap[n]
+    std::vector<uint64_t> in_off(n);
+    out_off.assign(n, 0);
     std::vector<uint32_t> caps(n), inl(n);
     uint64_t total = 0;
     for (uint64_t i = 0; i < n; ++i) {
@@ -565,7 +661,6 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
         caps[i] = s.isize;
         total += s.isize;
     }
-    (void)lens;
     if (t) {
         t->blocks += n;
         for (uint64_t i = 0; i < n; ++i) t->compressed_bytes += spans[i].payload_len;
@@ -575,30 +670,30 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     frhip_buf *o = buf_alloc(ctx, total, err);
     if (!o) return err ? err->kind : FRHIP_E_DEVICE;
     if (n == 0) {
-        offsets[0] = 0;
         *out = o;
         return FRHIP_OK;
     }
     int rc;
     // compressed bytes, padded: the kernel stages 1 KiB-aligned chunks past the end
     const uint64_t pad = 4096;
-    uint8_t *d_in = (uint8_t *)ctx->in_scratch.get(batch_len + pad, err, &rc);
+    uint8_t *d_in = (uint8_t *)run.in->get(batch_len + pad, err, &rc);
     if (!d_in) { buf_free(o); return rc; }
-    // meta: in_off[n] u64, out_off[n] u64, in_len[n] u32, out_cap[n] u32
+    // meta: in_off[n] u64, out_off[n] u64, in_len[n] u32, out_cap[n] u32; the
+    // same pinned buffer later receives the status read-back (stream order)
     const uint64_t meta_bytes = n * 8 * 2 + n * 4 * 2;
-    uint8_t *h_meta = (uint8_t *)ctx->pinned.get(meta_bytes, err, &rc);
+    uint8_t *h_meta = (uint8_t *)run.pinned->get(n * 12 > meta_bytes ? n * 12 : meta_bytes, err, &rc);
     if (!h_meta) { buf_free(o); return rc; }
-    std::memcpy(h_meta, in_off, n * 8);
-    std::memcpy(h_meta + n * 8, out_off, n * 8);
+    std::memcpy(h_meta, in_off.data(), n * 8);
+    std::memcpy(h_meta + n * 8, out_off.data(), n * 8);
     std::memcpy(h_meta + n * 16, inl.data(), n * 4);
     std::memcpy(h_meta + n * 20, caps.data(), n * 4);
-    uint8_t *d_meta = (uint8_t *)ctx->meta_scratch.get(meta_bytes, err, &rc);
+    uint8_t *d_meta = (uint8_t *)run.meta->get(meta_bytes, err, &rc);
     if (!d_meta) { buf_free(o); return rc; }
-    uint32_t *d_st = (uint32_t *)ctx->status_scratch.get(n * 12, err, &rc);
+    uint32_t *d_st = (uint32_t *)run.status->get(n * 12, err, &rc);
     if (!d_st) { buf_free(o); return rc; }
-    hipError_t r = hipMemcpyAsync(d_in, batch, batch_len, hipMemcpyHostToDevice, ctx->stream);
-    if (r == hipSuccess) r = hipMemsetAsync(d_in + batch_len, 0, pad, ctx->stream);
-    if (r == hipSuccess) r = hipMemcpyAsync(d_meta, h_meta, meta_bytes, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t r = hipMemcpyAsync(d_in, batch, batch_len, hipMemcpyHostToDevice, run.stream);
+    if (r == hipSuccess) r = hipMemsetAsync(d_in + batch_len, 0, pad, run.stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(d_meta, h_meta, meta_bytes, hipMemcpyHostToDevice, run.stream);
     if (r == hipSuccess && t) r = phase(t->upload_ms);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "uploading the batch"); }
     InflateArgs a{};
@@ -617,51 +712,84 @@ int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, con
     a.x2n = ctx->x2n;
     void *d_rec = nullptr;
     if (const uint64_t sb = inflate_scratch_bytes(ctx->inflate_schedule, total, n)) {
-        d_rec = ctx->rec_scratch.get(sb, err, &rc);
+        d_rec = run.rec->get(sb, err, &rc);
         if (!d_rec) { buf_free(o); return rc; }
     }
     inflate_scratch_bind(a, d_rec, n);
     a.two_phase_min = ctx->two_phase_min;
-    r = launch_inflate(a, (uint32_t)n, ctx->stream, ctx->inflate_schedule);
+    r = launch_inflate(a, (uint32_t)n, run.stream, ctx->inflate_schedule);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "launching inflate"); }
     // the consumer's ordering point: the inflated bytes are final here
-    if (EventRef *ev = event_record(ctx->stream)) {
+    if (EventRef *ev = event_record(run.stream)) {
         buf_attach_event(o, ev);
         event_release(ev);
     }
     if (t) r = phase(t->kernel_ms);
-    uint32_t *h_st = (uint32_t *)ctx->pinned.get(n * 12 > meta_bytes ? n * 12 : meta_bytes, err, &rc);
-    if (!h_st) { buf_free(o); return rc; }
-    if (r == hipSuccess) r = hipMemcpyAsync(h_st, d_st, n * 12, hipMemcpyDeviceToHost, ctx->stream);
-    if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    if (r == hipSuccess) r = hipMemcpyAsync(h_meta, d_st, n * 12, hipMemcpyDeviceToHost, run.stream);
     if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "inflate"); }
-    // verify_and_offsets (backend.rs:221-256): status, then size, then CRC
-    const uint32_t *st = h_st, *produced = h_st + n, *crc = h_st + 2 * n;
+    *out = o;
+    *h_status = (const uint32_t *)h_meta;
+    return FRHIP_OK;
+}
+
+// verify_and_offsets (backend.rs:221-256) over read-back statuses: status,
+// then size, then CRC, member by member. Returns the index of the first
+// member that fails (err set) or n; offsets[0 .. that index] are filled.
+uint64_t inflate_verify(const frhip_block_span *spans, uint64_t n, const uint32_t *h_status,
+                        const std::vector<uint64_t> &out_off, uint64_t *offsets, frhip_error *err)
+{
+    const uint32_t *st = h_status, *produced = h_status + n, *crc = h_status + 2 * n;
     offsets[0] = 0;
     for (uint64_t i = 0; i < n; ++i) {
         if (st[i] != 0) {
-            buf_free(o);
-            int k = set_err(err, FRHIP_E_INFLATE, spans[i].compressed_offset, "%s", inflate_status_message(st[i]));
+            set_err(err, FRHIP_E_INFLATE, spans[i].compressed_offset, "%s", inflate_status_message(st[i]));
             if (err) err->code = st[i];
-            return k;
+            return i;
         }
         if (produced[i] != spans[i].isize) {
-            buf_free(o);
-            int k = set_err(err, FRHIP_E_SIZE_MISMATCH, spans[i].compressed_offset,
-                            "ISIZE says %u bytes, inflated %u", spans[i].isize, produced[i]);
+            set_err(err, FRHIP_E_SIZE_MISMATCH, spans[i].compressed_offset, "ISIZE says %u bytes, inflated %u",
+                    spans[i].isize, produced[i]);
             if (err) { err->expected = spans[i].isize; err->actual = produced[i]; }
-            return k;
+            return i;
         }
         if (crc[i] != spans[i].crc32) {
-            buf_free(o);
-            int k = set_err(err, FRHIP_E_CHECKSUM_MISMATCH, spans[i].compressed_offset,
-                            "expected crc32 %#010x, got %#010x", spans[i].crc32, crc[i]);
+            set_err(err, FRHIP_E_CHECKSUM_MISMATCH, spans[i].compressed_offset, "expected crc32 %#010x, got %#010x",
+                    spans[i].crc32, crc[i]);
             if (err) { err->expected = spans[i].crc32; err->actual = crc[i]; }
-            return k;
+            return i;
         }
         offsets[i + 1] = out_off[i] + produced[i];
     }
-    if (t) (void)phase(t->verify_ms);
+    return n;
+}
+
+// Plan + upload + launch + verify on the context's stream. On success *out
+// owns the output.
+int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
+                   uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, frhip_inflate_timings *t,
+                   uint64_t align_at)
+{
+    *out = nullptr;
+    const InflateRun run{ctx->stream, &ctx->in_scratch, &ctx->meta_scratch, &ctx->status_scratch, &ctx->rec_scratch,
+                         &ctx->pinned};
+    std::vector<uint64_t> out_off;
+    const uint32_t *h_st = nullptr;
+    frhip_buf *o = nullptr;
+    int rc = inflate_launch(ctx, run, batch, batch_len, spans, n, &o, out_off, &h_st, err, t, align_at);
+    if (rc) return rc;
+    if (n == 0) {
+        offsets[0] = 0;
+        *out = o;
+        return FRHIP_OK;
+    }
+    const double t0 = t ? now_ms() : 0;
+    hipError_t r = hipStreamSynchronize(ctx->stream);
+    if (r != hipSuccess) { buf_free(o); return hip_err(err, r, "inflate"); }
+    if (inflate_verify(spans, n, h_st, out_off, offsets, err) != n) {
+        buf_free(o);
+        return err ? err->kind : FRHIP_E_INFLATE;
+    }
+    if (t) t->verify_ms += now_ms() - t0;
     *out = o;
     return FRHIP_OK;
 }

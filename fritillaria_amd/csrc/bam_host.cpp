@@ -301,9 +301,14 @@ struct frhip_reader {
     const uint8_t *src = nullptr;
     uint64_t src_len = 0, src_pos = 0;
     uint64_t blocks_per_batch = 256;
-    std::vector<uint8_t> compressed;
+    // The current batch's compressed bytes are src[batch_offset, batch_offset
+    // + comp_len); the carry (a trailing partial member, or the members from
+    // the one holding a carried record on) is the last carry_len of them, and
+    // always ends at src_pos. Both are views of the caller's buffer, which
+    // outlives the reader: nothing is copied on the host.
+    uint64_t comp_len = 0;
     uint64_t batch_offset = 0;
-    std::vector<uint8_t> carry;
+    uint64_t carry_len = 0;
     uint64_t carry_start = 0;
     std::vector<frhip_block_span> spans;
     std::vector<uint64_t> offsets, coffsets;
@@ -314,6 +319,21 @@ struct frhip_reader {
     BufOwner *cache = nullptr;
     EventRef *cache_ready = nullptr;
     std::vector<uint64_t> cache_coff, cache_uoff;
+    uint64_t cache_cend = 0; // compressed end of the last cached member
+    // Window bounds (0: the defaults below).
+    uint64_t ra_members = 0, ra_bytes = 0;
+    // The next window, inflating on the reader's own stream with its own
+    // scratch while batches are cut from the current one; it becomes the
+    // cache when a batch needs it.
+    hipStream_t ahead_stream = nullptr;
+    Scratch ahead_in, ahead_meta, ahead_status, ahead_rec;
+    PinnedScratch ahead_pinned;
+    bool ahead_pending = false;
+    uint64_t ahead_c0 = 0;
+    frhip_buf *ahead_data = nullptr;
+    const uint32_t *ahead_status_h = nullptr;
+    std::vector<frhip_block_span> ahead_spans;
+    std::vector<uint64_t> ahead_out_off;
 };
 
 namespace {
@@ -322,9 +342,10 @@ namespace {
 // member per lane) to fill the device, and many batches' worth.
 uint64_t read_ahead_members(const frhip_reader *r)
 {
-    return std::max<uint64_t>(32768, 16 * r->blocks_per_batch);
+    return r->ra_members ? r->ra_members : std::max<uint64_t>(32768, 16 * r->blocks_per_batch);
 }
 constexpr uint64_t READ_AHEAD_MAX_BYTES = 1ull << 30; // compressed bytes per read-ahead launch
+uint64_t read_ahead_bytes(const frhip_reader *r) { return r->ra_bytes ? r->ra_bytes : READ_AHEAD_MAX_BYTES; }
 
 void drop_cache(frhip_reader *r)
 {
@@ -334,6 +355,17 @@ void drop_cache(frhip_reader *r)
     r->cache_ready = nullptr;
     r->cache_coff.clear();
     r->cache_uoff.clear();
+    r->cache_cend = 0;
+}
+
+// Waits for the window inflating ahead, if any, and discards it.
+void drop_ahead(frhip_reader *r)
+{
+    if (!r->ahead_pending) return;
+    (void)hipStreamSynchronize(r->ahead_stream);
+    buf_free(r->ahead_data);
+    r->ahead_data = nullptr;
+    r->ahead_pending = false;
 }
 
 // The current batch (r->spans) as a view into the read-ahead, or nullptr when
@@ -361,50 +393,145 @@ frhip_buf *cache_view(frhip_reader *r, frhip_error *err)
     return b;
 }
 
-// Inflates the read-ahead from the batch's first member on. A member that
-// fails verification ends the read-ahead before it, so that the batch holding
-// it is inflated on its own and reports the failure as it always has.
+// The whole members from compressed offset c0 on that one window holds
+// (discovery stops at a partial or bad member).
+void window_members(const frhip_reader *r, uint64_t c0, std::vector<frhip_block_span> &ra)
+{
+    ra.clear();
+    if (c0 >= r->src_len) return;
+    const uint64_t avail = std::min<uint64_t>(r->src_len - c0, read_ahead_bytes(r));
+    uint64_t consumed = 0;
+    frhip_error e{};
+    (void)discover(r->src + c0, avail, c0, ra, &consumed, &e);
+    if (ra.size() > read_ahead_members(r)) ra.resize(read_ahead_members(r));
+}
+
+// Compressed bytes of the window ra read from c0 (spans' payload offsets are
+// relative to c0).
+uint64_t window_bytes(const std::vector<frhip_block_span> &ra)
+{
+    return ra.empty() ? 0 : ra.back().payload_start + ra.back().payload_len + 8;
+}
+
+// A finished window becomes the cache: its members before `good`, the first
+// that failed verification (good == ra.size(): none failed), so that the
+// batch holding a bad member is inflated on its own and reports the failure
+// as it always has. Returns whether every member passed.
+bool adopt_window(frhip_reader *r, frhip_buf *data, uint64_t c0, const std::vector<frhip_block_span> &ra,
+                  uint64_t good, const uint64_t *offs)
+{
+    drop_cache(r);
+    if (good == 0) {
+        buf_free(data);
+        return false;
+    }
+    r->cache = new (std::nothrow) BufOwner;
+    if (!r->cache) {
+        buf_free(data);
+        return false;
+    }
+    r->cache->ptr = data->ptr;
+    r->cache->ordinal = data->ordinal;
+    r->cache->cap = data->cap;
+    r->cache_ready = data->ready;
+    data->ptr = nullptr;
+    data->ready = nullptr;
+    buf_free(data);
+    r->cache_uoff.assign(offs, offs + good + 1);
+    r->cache_coff.resize(good);
+    for (uint64_t i = 0; i < good; ++i) r->cache_coff[i] = ra[i].compressed_offset;
+    r->cache_cend = c0 + ra[good - 1].payload_start + ra[good - 1].payload_len + 8;
+    return good == ra.size();
+}
+
+// Launches the window after the cache on the reader's stream. It starts
+// `overlap` compressed bytes before the cache's end, so that a batch that
+// straddles that end, with its carry, lies wholly in the new window; a batch
+// that does not (a carry longer than that) is inflated by a refill instead.
+void launch_ahead(frhip_reader *r)
+{
+    if (r->ahead_pending || !r->cache || r->cache_coff.empty() || r->cache_cend >= r->src_len) return;
+    const uint64_t span = r->cache_cend - r->cache_coff[0];
+    const uint64_t overlap = std::min<uint64_t>(2 * r->blocks_per_batch * 65536ull + (8ull << 20), span / 2);
+    // the first cached member at or past cache_cend - overlap (the last one
+    // when that point lies inside it)
+    auto it = std::lower_bound(r->cache_coff.begin(), r->cache_coff.end(), r->cache_cend - overlap);
+    const uint64_t from = it == r->cache_coff.end() ? r->cache_coff.back() : *it;
+    window_members(r, from, r->ahead_spans);
+    // a window that would not reach past the cache adds nothing
+    if (r->ahead_spans.empty() || from + window_bytes(r->ahead_spans) <= r->cache_cend) return;
+    if (!r->ahead_stream && hipStreamCreateWithFlags(&r->ahead_stream, hipStreamNonBlocking) != hipSuccess) {
+        r->ahead_stream = nullptr;
+        return;
+    }
+    const InflateRun run{r->ahead_stream, &r->ahead_in, &r->ahead_meta, &r->ahead_status, &r->ahead_rec,
+                         &r->ahead_pinned};
+    frhip_error e{};
+    if (inflate_launch(r->ctx, run, r->src + from, window_bytes(r->ahead_spans), r->ahead_spans.data(),
+                       r->ahead_spans.size(), &r->ahead_data, r->ahead_out_off, &r->ahead_status_h, &e,
+                       nullptr) != FRHIP_OK) {
+        r->ahead_data = nullptr;
+        return;
+    }
+    r->ahead_c0 = from;
+    r->ahead_pending = true;
+}
+
+// Makes the window inflated ahead the cache when it holds every member of
+// the current batch (r->spans): waits for it, verifies it and launches the
+// next. Returns whether it was taken.
+bool take_ahead(frhip_reader *r)
+{
+    const uint64_t k = r->spans.size();
+    if (!r->ahead_pending || k == 0) return false;
+    auto holds = [&](uint64_t coff) {
+        auto it = std::lower_bound(r->ahead_spans.begin(), r->ahead_spans.end(), coff,
+                                   [](const frhip_block_span &s, uint64_t v) { return s.compressed_offset < v; });
+        return it != r->ahead_spans.end() && it->compressed_offset == coff;
+    };
+    if (!holds(r->spans[0].compressed_offset) || !holds(r->spans[k - 1].compressed_offset)) return false;
+    r->ahead_pending = false;
+    frhip_buf *data = r->ahead_data;
+    r->ahead_data = nullptr;
+    if (hipStreamSynchronize(r->ahead_stream) != hipSuccess) {
+        buf_free(data);
+        return false;
+    }
+    std::vector<uint64_t> offs(r->ahead_spans.size() + 1);
+    frhip_error e{};
+    const uint64_t good = inflate_verify(r->ahead_spans.data(), r->ahead_spans.size(), r->ahead_status_h,
+                                         r->ahead_out_off, offs.data(), &e);
+    if (adopt_window(r, data, r->ahead_c0, r->ahead_spans, good, offs.data())) launch_ahead(r);
+    return true;
+}
+
+// Inflates a window from the batch's first member on, on the context's
+// stream, waiting for it; then launches the next one ahead.
 void refill_cache(frhip_reader *r)
 {
     drop_cache(r);
     if (r->spans.empty()) return;
     const uint64_t c0 = r->spans[0].compressed_offset;
-    if (c0 >= r->src_len) return;
-    const uint8_t *base = r->src + c0;
-    uint64_t avail = std::min<uint64_t>(r->src_len - c0, READ_AHEAD_MAX_BYTES);
     std::vector<frhip_block_span> ra;
-    uint64_t consumed = 0;
+    window_members(r, c0, ra);
+    if (ra.size() <= r->spans.size()) return;
+    frhip_ctx *ctx = r->ctx;
+    const InflateRun run{ctx->stream, &ctx->in_scratch, &ctx->meta_scratch, &ctx->status_scratch, &ctx->rec_scratch,
+                         &ctx->pinned};
+    frhip_buf *data = nullptr;
+    std::vector<uint64_t> out_off;
+    const uint32_t *hst = nullptr;
     frhip_error e{};
-    (void)discover(base, avail, c0, ra, &consumed, &e); // stops at a partial or bad member
-    if (ra.size() > read_ahead_members(r)) ra.resize(read_ahead_members(r));
-    while (ra.size() > r->spans.size()) {
-        const uint64_t bytes = ra.back().payload_start + ra.back().payload_len + 8;
-        std::vector<uint64_t> offs(ra.size() + 1);
-        frhip_buf *data = nullptr;
-        frhip_error fe{};
-        const int rc = inflate_device(r->ctx, base, bytes, ra.data(), ra.size(), &data, offs.data(), &fe, nullptr);
-        if (rc == FRHIP_OK) {
-            r->cache = new (std::nothrow) BufOwner;
-            if (!r->cache) {
-                buf_free(data);
-                return;
-            }
-            r->cache->ptr = data->ptr;
-            r->cache_ready = data->ready;
-            data->ptr = nullptr;
-            data->ready = nullptr;
-            buf_free(data);
-            r->cache_uoff.assign(offs.begin(), offs.end());
-            r->cache_coff.resize(ra.size());
-            for (size_t i = 0; i < ra.size(); ++i) r->cache_coff[i] = ra[i].compressed_offset;
-            return;
-        }
-        if (rc != FRHIP_E_INFLATE && rc != FRHIP_E_SIZE_MISMATCH && rc != FRHIP_E_CHECKSUM_MISMATCH) return;
-        // keep the members before the failing one
-        size_t keep = 0;
-        while (keep < ra.size() && ra[keep].compressed_offset < fe.offset) ++keep;
-        ra.resize(keep);
+    if (inflate_launch(ctx, run, r->src + c0, window_bytes(ra), ra.data(), ra.size(), &data, out_off, &hst, &e,
+                       nullptr) != FRHIP_OK)
+        return;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        buf_free(data);
+        return;
     }
+    std::vector<uint64_t> offs(ra.size() + 1);
+    const uint64_t good = inflate_verify(ra.data(), ra.size(), hst, out_off, offs.data(), &e);
+    if (adopt_window(r, data, c0, ra, good, offs.data())) launch_ahead(r);
 }
 
 } // namespace
@@ -426,9 +553,26 @@ extern "C" int frhip_reader_open(frhip_ctx *ctx, const uint8_t *data, uint64_t l
 extern "C" void frhip_reader_free(frhip_reader *r)
 {
     if (!r) return;
+    drop_ahead(r);
     drop_cache(r);
+    if (r->ahead_stream) (void)hipStreamDestroy(r->ahead_stream);
+    r->ahead_in.release();
+    r->ahead_meta.release();
+    r->ahead_status.release();
+    r->ahead_rec.release();
+    r->ahead_pinned.release();
     ctx_release(r->ctx);
     delete r;
+}
+
+extern "C" int frhip_reader_set_read_ahead(frhip_reader *r, uint64_t members, uint64_t bytes, frhip_error *err)
+{
+    (void)err;
+    drop_ahead(r);
+    drop_cache(r);
+    r->ra_members = members;
+    r->ra_bytes = bytes;
+    return FRHIP_OK;
 }
 
 extern "C" int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_error *err)
@@ -444,7 +588,7 @@ extern "C" int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_err
     }
     uint64_t block = (uint64_t)(std::upper_bound(r->offsets.begin(), r->offsets.end(), tail) - r->offsets.begin()) - 1;
     uint64_t block_start = r->spans[block].compressed_offset - r->batch_offset;
-    r->carry.assign(r->compressed.begin() + (ptrdiff_t)block_start, r->compressed.end());
+    r->carry_len = r->comp_len - block_start;
     r->carry_start = tail - r->offsets[block];
     return FRHIP_OK;
 }
@@ -456,33 +600,34 @@ extern "C" int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_bu
     *has_batch = 0;
     *out_data = nullptr;
     for (;;) {
-        if (r->inner_exhausted && r->carry.empty()) return FRHIP_OK;
-        if (r->inner_exhausted && !r->compressed.empty() && r->carry.size() == r->compressed.size()) {
+        if (r->inner_exhausted && r->carry_len == 0) return FRHIP_OK;
+        if (r->inner_exhausted && r->comp_len != 0 && r->carry_len == r->comp_len) {
             int k = set_err(err, FRHIP_E_MALFORMED, r->batch_offset,
                             "stream ends inside a record: the trailing bytes are not a complete record");
             if (err) std::strcpy(err->format, "bgzf");
             return k;
         }
-        r->batch_offset += r->compressed.size();
-        r->batch_offset -= r->carry.size();
-        r->compressed.swap(r->carry);
-        r->carry.clear();
+        // the carry becomes the front of the next batch, which continues it
+        r->batch_offset = r->src_pos - r->carry_len;
+        r->comp_len = r->carry_len;
+        r->carry_len = 0;
         const uint64_t want = r->blocks_per_batch * 65536ull;
         const uint64_t remaining = r->src_len - r->src_pos;
         const uint64_t take = std::min(want, remaining);
-        r->compressed.insert(r->compressed.end(), r->src + r->src_pos, r->src + r->src_pos + take);
+        r->comp_len += take;
         r->src_pos += take;
         if (remaining < want) r->inner_exhausted = true;
+        const uint8_t *comp = r->src + r->batch_offset;
 
         r->spans.clear();
         uint64_t consumed = 0;
-        int rc = discover(r->compressed.data(), r->compressed.size(), r->batch_offset, r->spans, &consumed, err);
+        int rc = discover(comp, r->comp_len, r->batch_offset, r->spans, &consumed, err);
         if (rc) return rc;
-        r->carry.assign(r->compressed.begin() + (ptrdiff_t)consumed, r->compressed.end());
-        if (r->inner_exhausted && !r->carry.empty()) {
+        r->carry_len = r->comp_len - consumed;
+        if (r->inner_exhausted && r->carry_len != 0) {
             int k = set_err(err, FRHIP_E_MALFORMED, r->batch_offset + consumed,
                             "truncated BGZF: %llu trailing bytes are not a complete block",
-                            (unsigned long long)r->carry.size());
+                            (unsigned long long)r->carry_len);
             if (err) std::strcpy(err->format, "bgzf");
             return k;
         }
@@ -492,13 +637,14 @@ extern "C" int frhip_reader_next_batch(frhip_reader *r, int *has_batch, frhip_bu
         }
         r->offsets.assign(r->spans.size() + 1, 0);
         frhip_buf *data = cache_view(r, err);
+        if (!data && take_ahead(r)) data = cache_view(r, err);
         if (!data) {
             refill_cache(r);
             data = cache_view(r, err);
         }
         if (!data) {
-            rc = inflate_device(r->ctx, r->compressed.data(), r->compressed.size(), r->spans.data(),
-                                r->spans.size(), &data, r->offsets.data(), err, nullptr);
+            rc = inflate_device(r->ctx, comp, r->comp_len, r->spans.data(), r->spans.size(), &data,
+                                r->offsets.data(), err, nullptr);
             if (rc) return rc;
         }
         if (data->len == 0) {

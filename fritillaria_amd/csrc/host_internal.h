@@ -41,8 +41,19 @@ void event_release(EventRef *e);
 struct BufOwner {
     void *ptr = nullptr;
     std::atomic<int> refs{1};
+    int ordinal = 0;
+    uint64_t cap = 0; // bytes of the allocation (0: not from dev_alloc)
 };
 void owner_release(BufOwner *o);
+// Device blocks for buffers. Freed blocks are kept per device for reuse
+// instead of released: hipFree waits for the whole device, so a free while
+// another stream has work in flight (a reader's window inflating ahead) would
+// stall on it. dev_alloc returns a block of at least n bytes (*cap: its size)
+// on the current device; dev_free keeps it (cap 0: releases it); dev_flush
+// releases a device's kept blocks.
+void *dev_alloc(int ordinal, uint64_t n, uint64_t *cap, hipError_t *r);
+void dev_free(int ordinal, void *p, uint64_t cap);
+void dev_flush(int ordinal);
 
 int set_err(frhip_error *e, int kind, uint64_t offset, const char *fmt, ...);
 int hip_err(frhip_error *e, hipError_t r, const char *what);
@@ -57,6 +68,24 @@ int discover(const uint8_t *buf, uint64_t len, uint64_t base, std::vector<frhip_
 int inflate_device(frhip_ctx *ctx, const uint8_t *batch, uint64_t batch_len, const frhip_block_span *spans,
                    uint64_t n, frhip_buf **out, uint64_t *offsets, frhip_error *err, frhip_inflate_timings *t,
                    uint64_t align_at = ~0ull);
+// The stream and scratch one inflate runs with: the context's, or a
+// reader's own for the window it inflates ahead.
+struct InflateRun {
+    hipStream_t stream;
+    Scratch *in, *meta, *status, *rec;
+    PinnedScratch *pinned;
+};
+// inflate_device in two halves. inflate_launch plans, uploads and launches on
+// run.stream without waiting: *out holds the output (its ready event
+// recorded), out_off the members' output offsets, and *h_status (in
+// run.pinned) receives status, produced and CRC per member once the stream
+// passes its last copy. inflate_verify then checks them in the reference's
+// order and returns the first failing member (err set), or n.
+int inflate_launch(frhip_ctx *ctx, const InflateRun &run, const uint8_t *batch, uint64_t batch_len,
+                   const frhip_block_span *spans, uint64_t n, frhip_buf **out, std::vector<uint64_t> &out_off,
+                   const uint32_t **h_status, frhip_error *err, frhip_inflate_timings *t, uint64_t align_at = ~0ull);
+uint64_t inflate_verify(const frhip_block_span *spans, uint64_t n, const uint32_t *h_status,
+                        const std::vector<uint64_t> &out_off, uint64_t *offsets, frhip_error *err);
 hipError_t launch_crc32_blocks(const uint8_t *data, const uint64_t *off, const uint32_t *len, uint32_t n,
                                uint32_t *crc, const uint32_t *x2n, hipStream_t s);
 
@@ -68,6 +97,7 @@ struct frhip_buf {
     int ordinal;
     frhip::EventRef *ready = nullptr; // owned reference, or none
     frhip::BufOwner *owner = nullptr; // set: ptr lies inside owner's allocation, which buf_free releases
+    uint64_t cap = 0;                 // bytes of ptr's block from dev_alloc (0: none)
 };
 
 struct frhip_ctx {

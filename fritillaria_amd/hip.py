@@ -249,6 +249,7 @@ def lib():
                                               C.POINTER(C.POINTER(_u64)), C.POINTER(C.POINTER(_u64)),
                                               C.POINTER(_u64), C.POINTER(_u64), _ep]),
         "frhip_reader_carry_from": (C.c_int, [_vp, _u64, _ep]),
+        "frhip_reader_set_read_ahead": (C.c_int, [_vp, _u64, _u64, _ep]),
         "frhip_reader_free": (None, [_vp]),
         "frhip_pipeline_create": (C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(_vp), _ep]),
         "frhip_pipeline_free": (None, [_vp]),
@@ -1337,7 +1338,11 @@ class DeviceBatch:
 class DeviceBgzfReader:
     """DeviceBgzfReader over a BGZF stream in host memory."""
 
-    def __init__(self, data, codec: HipCodec, blocks_per_batch: int = 256):
+    def __init__(self, data, codec: HipCodec, blocks_per_batch: int = 256, read_ahead_members: int = 0,
+                 read_ahead_bytes: int = 0):
+        """read_ahead_*: bounds of the windows of members inflated in one
+        launch (0: the defaults); the next window is inflated while batches
+        are cut from the current one."""
         self._src = _u8(data)  # borrowed by the native reader: keep it alive
         self.codec = codec
         h = _vp()
@@ -1346,6 +1351,8 @@ class DeviceBgzfReader:
                                      max(1, blocks_per_batch), C.byref(h), C.byref(e))
         _raise(rc, e)
         self._h = h
+        if read_ahead_members or read_ahead_bytes:
+            _raise(lib().frhip_reader_set_read_ahead(h, read_ahead_members, read_ahead_bytes, C.byref(e)), e)
 
     def codec_name(self) -> str:
         return self.codec.device_name()

@@ -177,9 +177,9 @@ def test_empty_batch_decodes_to_no_records(codec, decoder):
     assert len(r) == 0 and r.tail() == 17
 
 
-def _reader_columns(codec, decoder, data, bpb):
+def _reader_columns(codec, decoder, data, bpb, read_ahead=0):
     import fritillaria_amd as fa
-    reader = fa.DeviceBgzfReader(data, codec, blocks_per_batch=bpb)
+    reader = fa.DeviceBgzfReader(data, codec, blocks_per_batch=bpb, read_ahead_members=read_ahead)
     first = True
     vo, pos, lseq, n_batches = [], [], [], 0
     while True:
@@ -216,6 +216,28 @@ def test_reader_loop_matches_whole_file(codec, decoder, oracle_mod, bpb):
         want_v = oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec)
         vo, pos, lseq, nb = _reader_columns(codec, decoder, s.compressed, bpb)
         assert np.array_equal(vo, want_v)
+        assert np.array_equal(pos, cols["position"]) and np.array_equal(lseq, cols["sequence_len"])
+
+
+@pytest.mark.parametrize("bpb", [1, 3, 8])
+@pytest.mark.parametrize("read_ahead", [6, 24, 64])
+def test_reader_loop_with_small_read_ahead_windows(codec, decoder, oracle_mod, bpb, read_ahead):
+    """Windows of a few members, so that many are inflated ahead on the
+    reader's stream, taken, and overlapped by the batches that straddle them
+    (long records whose carry exceeds the overlap take a refill instead): the
+    same records as the whole-file oracle."""
+    print(f"[gpu] reader loop blocks_per_batch={bpb} read-ahead {read_ahead} members", flush=True)
+    from fritillaria_amd import synth
+    for config, n in ((2, 20000), (3, 12)):
+        s = synth.config_bam(config, n_records=n)
+        d = oracle_mod.discover(s.compressed)
+        r = oracle_mod.inflate_batch(s.compressed, d.spans)
+        u = np.frombuffer(r.data, np.uint8)
+        rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, s.records_start)
+        cols = oracle_mod.bam_decode_columns(u, rec)
+        want_v = oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec)
+        vo, pos, lseq, nb = _reader_columns(codec, decoder, s.compressed, bpb, read_ahead)
+        assert np.array_equal(vo, want_v), (config, bpb, read_ahead)
         assert np.array_equal(pos, cols["position"]) and np.array_equal(lseq, cols["sequence_len"])
 
 
@@ -305,12 +327,13 @@ def test_speculation_holds_past_2_gib(codec, decoder, oracle_mod):
 
 
 @pytest.mark.parametrize("bpb", [1, 3])
-def test_reader_read_ahead_reports_a_bad_member_where_it_lies(codec, decoder, bpb):
+@pytest.mark.parametrize("read_ahead", [0, 8])
+def test_reader_read_ahead_reports_a_bad_member_where_it_lies(codec, decoder, bpb, read_ahead):
     """The reader inflates ahead of the batches it hands out; a member with a
     bad trailer CRC inside that window still fails exactly the batch that holds
     it (ChecksumMismatch at its offset), after every earlier batch was served
     with the records it holds, as when each batch is inflated on its own."""
-    print(f"[gpu] reader read-ahead with a bad member, blocks_per_batch={bpb}", flush=True)
+    print(f"[gpu] reader read-ahead with a bad member, blocks_per_batch={bpb}, windows {read_ahead}", flush=True)
     import fritillaria_amd as fa
     from helpers import bgzf_member
     rng = np.random.default_rng(77)
@@ -323,7 +346,7 @@ def test_reader_read_ahead_reports_a_bad_member_where_it_lies(codec, decoder, bp
     members[bad] = bgzf_member(payload, crc=zlib.crc32(payload) ^ 1)
     blob = b"".join(members) + bgzf(b"", eof=True)
     bad_off = sum(len(m) for m in members[:bad])
-    reader = fa.DeviceBgzfReader(blob, codec, blocks_per_batch=bpb)
+    reader = fa.DeviceBgzfReader(blob, codec, blocks_per_batch=bpb, read_ahead_members=read_ahead)
     first, served = True, 0
     with pytest.raises(fa.ChecksumMismatch) as ei:
         while True:
