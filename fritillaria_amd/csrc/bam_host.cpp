@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/frhip.h"
@@ -328,6 +329,12 @@ struct frhip_reader {
     hipStream_t ahead_stream = nullptr;
     Scratch ahead_in, ahead_meta, ahead_status, ahead_rec;
     PinnedScratch ahead_pinned;
+    // The window's upload and launch run on a helper thread: an upload from
+    // pageable memory holds up the thread that issues it until it is staged.
+    // The thread writes ahead_data, ahead_out_off, ahead_status_h and
+    // ahead_rc; they are read only after it is joined.
+    std::thread ahead_thread;
+    int ahead_rc = 0;
     bool ahead_pending = false;
     uint64_t ahead_c0 = 0;
     frhip_buf *ahead_data = nullptr;
@@ -362,6 +369,7 @@ void drop_cache(frhip_reader *r)
 void drop_ahead(frhip_reader *r)
 {
     if (!r->ahead_pending) return;
+    if (r->ahead_thread.joinable()) r->ahead_thread.join();
     (void)hipStreamSynchronize(r->ahead_stream);
     buf_free(r->ahead_data);
     r->ahead_data = nullptr;
@@ -464,17 +472,20 @@ void launch_ahead(frhip_reader *r)
         r->ahead_stream = nullptr;
         return;
     }
-    const InflateRun run{r->ahead_stream, &r->ahead_in, &r->ahead_meta, &r->ahead_status, &r->ahead_rec,
-                         &r->ahead_pinned};
-    frhip_error e{};
-    if (inflate_launch(r->ctx, run, r->src + from, window_bytes(r->ahead_spans), r->ahead_spans.data(),
-                       r->ahead_spans.size(), &r->ahead_data, r->ahead_out_off, &r->ahead_status_h, &e,
-                       nullptr) != FRHIP_OK) {
-        r->ahead_data = nullptr;
-        return;
-    }
     r->ahead_c0 = from;
+    r->ahead_data = nullptr;
+    r->ahead_rc = FRHIP_E_DEVICE;
     r->ahead_pending = true;
+    r->ahead_thread = std::thread([r, from]() {
+        const InflateRun run{r->ahead_stream, &r->ahead_in, &r->ahead_meta, &r->ahead_status, &r->ahead_rec,
+                             &r->ahead_pinned};
+        frhip_error e{};
+        frhip_buf *d = nullptr;
+        const int rc = inflate_launch(r->ctx, run, r->src + from, window_bytes(r->ahead_spans), r->ahead_spans.data(),
+                                      r->ahead_spans.size(), &d, r->ahead_out_off, &r->ahead_status_h, &e, nullptr);
+        r->ahead_data = rc == FRHIP_OK ? d : nullptr;
+        r->ahead_rc = rc;
+    });
 }
 
 // Makes the window inflated ahead the cache when it holds every member of
@@ -490,9 +501,15 @@ bool take_ahead(frhip_reader *r)
         return it != r->ahead_spans.end() && it->compressed_offset == coff;
     };
     if (!holds(r->spans[0].compressed_offset) || !holds(r->spans[k - 1].compressed_offset)) return false;
+    r->ahead_thread.join();
     r->ahead_pending = false;
     frhip_buf *data = r->ahead_data;
     r->ahead_data = nullptr;
+    if (r->ahead_rc != FRHIP_OK) {
+        (void)hipStreamSynchronize(r->ahead_stream);
+        buf_free(data);
+        return false;
+    }
     if (hipStreamSynchronize(r->ahead_stream) != hipSuccess) {
         buf_free(data);
         return false;
