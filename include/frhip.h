@@ -492,8 +492,8 @@ int frhip_reader_carry_from(frhip_reader *r, uint64_t tail, frhip_error *err);
 /* Bounds of the windows of members the reader inflates in one launch and
    cuts its batches from: at most `members` members and `bytes` compressed
    bytes each (0: max(32768, 16 x blocks_per_batch) members, 1 GiB). The next
-   window is inflated on the reader's own stream while batches are cut from
-   the current one. Setting them discards windows already inflated. No
+   window is uploaded and inflated on the reader's own stream, by a helper
+   thread, while batches are cut from the current one. Setting them discards windows already inflated. No
    reference counterpart: the reference inflates each batch on its own, and
    the batches and their errors are the same either way. */
 int frhip_reader_set_read_ahead(frhip_reader *r, uint64_t members, uint64_t bytes, frhip_error *err);
