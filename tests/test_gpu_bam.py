@@ -241,6 +241,31 @@ def test_reader_loop_with_small_read_ahead_windows(codec, decoder, oracle_mod, b
         assert np.array_equal(pos, cols["position"]) and np.array_equal(lseq, cols["sequence_len"])
 
 
+@pytest.mark.parametrize("bpb", [1, 4])
+def test_reader_from_pinned_memory_with_read_ahead(codec, decoder, oracle_mod, bpb):
+    """The stream in pinned host memory, so the windows read ahead are
+    uploaded asynchronously by the reader's helper thread while batches are
+    decoded: the same records as the whole-file oracle."""
+    print(f"[gpu] reader loop from pinned memory, blocks_per_batch={bpb}", flush=True)
+    import fritillaria_amd as fa
+    from fritillaria_amd import synth
+    s = synth.config_bam(2, n_records=20000)
+    d = oracle_mod.discover(s.compressed)
+    r = oracle_mod.inflate_batch(s.compressed, d.spans)
+    u = np.frombuffer(r.data, np.uint8)
+    rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, s.records_start)
+    cols = oracle_mod.bam_decode_columns(u, rec)
+    want_v = oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec)
+    pin = fa.PinnedBuffer(s.compressed.size)
+    pin.array[:] = s.compressed
+    try:
+        vo, pos, lseq, nb = _reader_columns(codec, decoder, pin.array, bpb, read_ahead=16)
+    finally:
+        pin.close()
+    assert np.array_equal(vo, want_v) and np.array_equal(pos, cols["position"])
+    assert np.array_equal(lseq, cols["sequence_len"])
+
+
 def test_reader_stream_ending_inside_a_record(codec, decoder):
     import fritillaria_amd as fa
     rng = np.random.default_rng(30)
