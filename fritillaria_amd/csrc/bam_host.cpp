@@ -476,7 +476,9 @@ void launch_ahead(frhip_reader *r)
     r->ahead_data = nullptr;
     r->ahead_rc = FRHIP_E_DEVICE;
     r->ahead_pending = true;
-    r->ahead_thread = std::thread([r, from]() {
+    // no thread to be had (std::system_error): no window ahead; the next
+    // batch past the cache refills it
+    auto work = [r, from]() {
         const InflateRun run{r->ahead_stream, &r->ahead_in, &r->ahead_meta, &r->ahead_status, &r->ahead_rec,
                              &r->ahead_pinned};
         frhip_error e{};
@@ -485,7 +487,12 @@ void launch_ahead(frhip_reader *r)
                                       r->ahead_spans.size(), &d, r->ahead_out_off, &r->ahead_status_h, &e, nullptr);
         r->ahead_data = rc == FRHIP_OK ? d : nullptr;
         r->ahead_rc = rc;
-    });
+    };
+    try {
+        r->ahead_thread = std::thread(work);
+    } catch (...) {
+        r->ahead_pending = false;
+    }
 }
 
 // Makes the window inflated ahead the cache when it holds every member of
