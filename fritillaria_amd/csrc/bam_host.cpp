@@ -476,8 +476,6 @@ void launch_ahead(frhip_reader *r)
     r->ahead_data = nullptr;
     r->ahead_rc = FRHIP_E_DEVICE;
     r->ahead_pending = true;
-    // no thread to be had (std::system_error): no window ahead; the next
-    // batch past the cache refills it
     auto work = [r, from]() {
         const InflateRun run{r->ahead_stream, &r->ahead_in, &r->ahead_meta, &r->ahead_status, &r->ahead_rec,
                              &r->ahead_pinned};
@@ -491,6 +489,8 @@ void launch_ahead(frhip_reader *r)
     try {
         r->ahead_thread = std::thread(work);
     } catch (...) {
+        // no thread to be had (std::system_error): no window ahead; the
+        // next batch past the cache refills it
         r->ahead_pending = false;
     }
 }
