@@ -114,3 +114,19 @@ def test_timed_calls_match_untimed_and_accumulate(hip_ctx, codec, decoder, oracl
     decoder.decode_timed(fa.DeviceInflateBatch(), 0, dt)
     assert dt.batches == 2 and dt.records == 3000 and dt.blocks == len(spans)
     assert dt.bytes == b0.byte_len() and dt.scan_ms > 0 and dt.decode_ms > 0
+
+
+def test_device_queries_and_the_two_phase_default(hip_ctx):
+    """The context's compute units, and the two-phase threshold's default: the
+    one-wave form's resident waves, 16 per compute unit (4,096 on 256 CUs)."""
+    import fritillaria_amd as fa
+    ctx = fa.HipContext(0)
+    try:
+        cu = ctx.compute_units()
+        assert cu > 0
+        assert ctx.two_phase_min_members() == 16 * cu
+        ctx.set_two_phase_min_members(123)
+        assert ctx.two_phase_min_members() == 123
+    finally:
+        ctx.close()
+    assert hip_ctx.compute_units() == cu
