@@ -177,9 +177,10 @@ def test_empty_batch_decodes_to_no_records(codec, decoder):
     assert len(r) == 0 and r.tail() == 17
 
 
-def _reader_columns(codec, decoder, data, bpb, read_ahead=0):
+def _reader_columns(codec, decoder, data, bpb, read_ahead=0, read_ahead_bytes=0):
     import fritillaria_amd as fa
-    reader = fa.DeviceBgzfReader(data, codec, blocks_per_batch=bpb, read_ahead_members=read_ahead)
+    reader = fa.DeviceBgzfReader(data, codec, blocks_per_batch=bpb, read_ahead_members=read_ahead,
+                                 read_ahead_bytes=read_ahead_bytes)
     first = True
     vo, pos, lseq, n_batches = [], [], [], 0
     while True:
@@ -264,6 +265,33 @@ def test_reader_from_pinned_memory_with_read_ahead(codec, decoder, oracle_mod, b
         pin.close()
     assert np.array_equal(vo, want_v) and np.array_equal(pos, cols["position"])
     assert np.array_equal(lseq, cols["sequence_len"])
+
+
+def test_reader_random_batch_and_window_bounds(codec, decoder, oracle_mod):
+    """Random batch budgets and read-ahead windows (member and byte bounds)
+    over the three BAM shapes: every combination reads the same records as
+    the whole-file oracle."""
+    from fritillaria_amd import synth
+    rng = np.random.default_rng(88)
+    files = []
+    for config, n in ((1, 3000), (2, 8000), (3, 10)):
+        s = synth.config_bam(config, n_records=n)
+        d = oracle_mod.discover(s.compressed)
+        r = oracle_mod.inflate_batch(s.compressed, d.spans)
+        u = np.frombuffer(r.data, np.uint8)
+        rc, rec, tail, _ = oracle_mod.bam_scan_blocked(u, r.offsets, s.records_start)
+        cols = oracle_mod.bam_decode_columns(u, rec)
+        files.append((config, s.compressed, oracle_mod.virtual_offsets(r.offsets, d.spans["compressed_offset"], rec),
+                      cols["position"]))
+    for trial in range(18):
+        config, data, want_v, want_pos = files[trial % 3]
+        bpb = int(rng.integers(1, 20))
+        ra = int(rng.integers(2, 80))
+        rab = int(rng.choice([0, int(rng.integers(64 << 10, 4 << 20))]))
+        print(f"[gpu] reader trial {trial}: config {config}, bpb {bpb}, window {ra} members / {rab} bytes",
+              flush=True)
+        vo, pos, lseq, nb = _reader_columns(codec, decoder, data, bpb, ra, rab)
+        assert np.array_equal(vo, want_v) and np.array_equal(pos, want_pos), (trial, config, bpb, ra, rab)
 
 
 def test_reader_stream_ending_inside_a_record(codec, decoder):
